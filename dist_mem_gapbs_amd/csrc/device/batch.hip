@@ -1,0 +1,622 @@
+// Batched 64-source BFS on gfx950: one 64-bit word per vertex, lane s = BFS
+// from source s (bit-parallel multi-source BFS; cf. Then et al., "The More
+// the Merrier: Efficient Multi-Source Graph Traversal", VLDB 2015).
+//
+// Every lane's depths equal those of a single-source BFS from its source
+// (src/bfs.cc:142-188 computes the same depths through DOBFS).
+//
+// Data in HBM for one PE block:
+//   F[L]   uint64[nloc] per level L   lanes whose BFS reaches u at depth L;
+//                                     kept for every level (depth output)
+//   seen   uint64[nloc]               lanes that have reached u at a depth
+//                                     < current level (committed lazily: a
+//                                     step at level L folds F[L] in)
+//   act    int32[2][nloc], aoff int64[2][nloc]
+//                                     active vertices (F[L][u] != 0) with an
+//                                     out-degree prefix, for top-down levels
+//   ctrl   batch control words
+//
+// Level L runs ONE kernel, either
+//   top-down  (push) over the active list: for v in out(u),
+//             new = F[L][u] & ~(seen[v] | F[L][v]); atomicOr into F[L+1][v];
+//             the first setter of F[L+1][v] appends v to the next list;
+//   bottom-up (pull) sweep over all u: need = lanes & ~(seen[u] | F[L][u]);
+//             OR F[L][v] over in(u) until every needed lane is found;
+//             F[L+1][u] = found & need (full overwrite).
+// The last workgroup picks the next direction: top-down while the next
+// frontier's out-edges are fewer than M / td_div (tuned).  A chain of these
+// launches is captured in a hipGraph.
+#include "batch.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace dmm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 4;
+constexpr int kEPB = kThreads * kItems;
+constexpr int kGroup = 8;  // lanes per vertex in the bottom-up sweep
+
+enum : int { kBTD = 1, kBBU = 2, kBDone = 4 };
+
+struct BCtrl {
+  unsigned long long packed;          // next active list: degsum << 30 | count
+  unsigned long long scanned[kSlots]; // edges scanned by sweeps
+  long long acount, aedges;           // current active list size / out-edges
+  int level, phase, done, cur;        // cur = active list buffer index
+  int nlevels;
+  int sweeps;
+  unsigned ticket;
+  int pad;
+  int level_kind[DMM_MAX_STEPS];
+  long long level_active[DMM_MAX_STEPS];
+  long long sweep_scanned_total;
+};
+
+struct BArgs {
+  const int64_t* out_off;
+  const int32_t* out_nbr;
+  const int64_t* in_off;
+  const int32_t* in_nbr;
+  int64_t nloc, vbegin;
+  uint64_t* levels;   // level_cap * nloc
+  int64_t level_cap;
+  uint64_t* seen;
+  int32_t* act0;
+  int32_t* act1;
+  int64_t* aoff0;
+  int64_t* aoff1;
+  BCtrl* ctrl;
+  uint64_t lanes;     // mask of used lanes
+  int64_t m;          // directed edges of the block (td threshold base)
+  int td_div;
+};
+
+__device__ __forceinline__ unsigned long long ld_agent(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct TdLds {
+  int32_t rel[kEPB + 2];
+  int64_t base[kEPB + 2];
+  int32_t u[kEPB + 2];
+  uint64_t f[kEPB + 2];
+  int32_t cv[kEPB];
+  int64_t scan[kThreads / 64 + 1];
+  int64_t i0, nv;
+  unsigned long long basec, based;
+  int nclaim;
+};
+
+struct SmallLds {
+  int64_t scan_c[kThreads / 64 + 1];
+  int64_t scan_d[kThreads / 64 + 1];
+  unsigned long long red[kThreads / 64];
+  unsigned long long basec, based;
+};
+
+union BLds {
+  TdLds td;
+  SmallLds sm;
+};
+
+__device__ void zero_next_next(const BArgs& a, int level) {
+  // F[level + 2] is written by atomics if level + 1 runs top-down.
+  if (level + 2 >= a.level_cap) return;
+  uint64_t* z = a.levels + (level + 2) * a.nloc;
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t i = gtid; i < a.nloc; i += gsize) z[i] = 0;
+}
+
+__device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
+  const int cur = c->cur;
+  const int32_t* q = cur ? a.act1 : a.act0;
+  const int64_t* qoff = cur ? a.aoff1 : a.aoff0;
+  int32_t* oq = cur ? a.act0 : a.act1;
+  int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
+  const int64_t F = c->acount;
+  const int64_t T = c->aedges;
+  const uint64_t* Fl = a.levels + level * a.nloc;
+  uint64_t* Fn = a.levels + (level + 1) * a.nloc;
+  const int tid = threadIdx.x;
+  for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
+    const int64_t e1 = min(T, e0 + kEPB);
+    if (tid == 0) {
+      int64_t lo = 0, hi = F;
+      while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= e0) lo = mid; else hi = mid;
+      }
+      int64_t i0 = lo;
+      hi = F;
+      while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= e1 - 1) lo = mid; else hi = mid;
+      }
+      L.i0 = i0;
+      L.nv = lo - i0 + 1;
+      L.nclaim = 0;
+    }
+    __syncthreads();
+    const int64_t i0 = L.i0;
+    // Listed vertices have out-degree >= 1: a chunk spans <= kEPB + 1 of them.
+    int nv = static_cast<int>(L.nv);
+    if (nv > kEPB + 2) {
+      if (tid == 0) c->pad = 1;
+      nv = kEPB + 2;
+    }
+    for (int i = tid; i < nv; i += kThreads) {
+      const int64_t gi = i0 + i;
+      const int64_t qo = qoff[gi];
+      const int32_t u = q[gi];
+      const int64_t lu = int64_t(u) - a.vbegin;
+      const uint64_t fu = Fl[lu];
+      L.rel[i] = static_cast<int32_t>(max(qo - e0, int64_t(0)));
+      L.u[i] = u;
+      L.f[i] = fu;
+      L.base[i] = a.out_off[lu] - qo;
+      // Lazy commit of level L into seen (idempotent if several chunks
+      // cover u).
+      a.seen[lu] = a.seen[lu] | fu;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      const int64_t e = e0 + tid + k * kThreads;
+      if (e < e1) {
+        const int32_t r = static_cast<int32_t>(e - e0);
+        int lo = 0, hi = nv;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (L.rel[mid] <= r) lo = mid; else hi = mid;
+        }
+        const int32_t v = a.out_nbr[L.base[lo] + e];
+        const int64_t lv = int64_t(v) - a.vbegin;
+        uint64_t nw = L.f[lo] & ~a.seen[lv];
+        if (nw) nw &= ~Fl[lv];
+        if (nw) {
+          const unsigned long long old =
+              atomicOr(reinterpret_cast<unsigned long long*>(&Fn[lv]), static_cast<unsigned long long>(nw));
+          if (old == 0ull) {
+            if (a.out_off[lv + 1] > a.out_off[lv]) {
+              const int slot = atomicAdd(&L.nclaim, 1);
+              L.cv[slot] = v;
+            }
+          }
+          // A vertex with no out-edges never enters the active list, so its
+          // lanes are committed to seen here (the only writers of seen[v]
+          // for such v are these atomics).
+          if (a.out_off[lv + 1] == a.out_off[lv])
+            atomicOr(reinterpret_cast<unsigned long long*>(&a.seen[lv]), static_cast<unsigned long long>(nw));
+        }
+      }
+    }
+    __syncthreads();
+    const int nclaim = L.nclaim;
+    int64_t cdeg[kItems];
+    int64_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+      const int idx = tid * kItems + j;
+      cdeg[j] = 0;
+      if (idx < nclaim) {
+        const int64_t lv = int64_t(L.cv[idx]) - a.vbegin;
+        cdeg[j] = a.out_off[lv + 1] - a.out_off[lv];
+      }
+      acc += cdeg[j];
+    }
+    int64_t total;
+    const int64_t excl = block_excl_scan<kThreads>(acc, L.scan, &total);
+    if (tid == 0) {
+      const unsigned long long add = (static_cast<unsigned long long>(total) << kCountBits) |
+                                     static_cast<unsigned long long>(nclaim);
+      const unsigned long long old = nclaim > 0 ? atomicAdd(&c->packed, add) : 0ull;
+      L.basec = old & kCountMask;
+      L.based = old >> kCountBits;
+    }
+    __syncthreads();
+    int64_t run = static_cast<int64_t>(L.based) + excl;
+    const int64_t basec = static_cast<int64_t>(L.basec);
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+      const int idx = tid * kItems + j;
+      if (idx < nclaim) {
+        oq[basec + idx] = L.cv[idx];
+        oqoff[basec + idx] = run;
+        run += cdeg[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Bottom-up sweep: kGroup lanes per vertex, strided over its in-neighbours,
+// OR-reduced across the group after every round of kGroup neighbours so the
+// group can stop as soon as every needed lane is found.
+__device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
+  const int cur = c->cur;
+  int32_t* oq = cur ? a.act0 : a.act1;
+  int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
+  const uint64_t* __restrict__ Fl = a.levels + level * a.nloc;
+  uint64_t* __restrict__ Fn = a.levels + (level + 1) * a.nloc;
+  const int64_t* __restrict__ in_off = a.in_off;
+  const int32_t* __restrict__ in_nbr = a.in_nbr;
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
+  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
+  unsigned long long scanned = 0;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t u = grp + it * ngrp;
+    uint64_t nxt = 0;
+    int64_t deg_u = 0;
+    if (u < a.nloc) {
+      const uint64_t s = a.seen[u] | Fl[u];
+      const uint64_t need = a.lanes & ~s;
+      const int64_t b = in_off[u], e = in_off[u + 1];
+      deg_u = a.out_off[u + 1] - a.out_off[u];
+      if (need && e > b) {
+        uint64_t acc = 0;
+        int64_t stop = e;
+        for (int64_t k0 = b; k0 < e; k0 += kGroup) {
+          const int64_t k = k0 + sub;
+          if (k < e) acc |= Fl[in_nbr[k]];
+#pragma unroll
+          for (int d = kGroup / 2; d >= 1; d >>= 1)
+            acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
+          if ((acc & need) == need) {
+            stop = min<int64_t>(k0 + kGroup, e);
+            break;
+          }
+        }
+        scanned += static_cast<unsigned long long>(stop - b);
+        nxt = acc & need;
+      }
+      if (sub == 0) {
+        Fn[u] = nxt;
+        if (Fl[u] | nxt) a.seen[u] = s | nxt;
+      }
+    }
+    // Append the group's vertex to the next active list (one packed atomic
+    // per workgroup per round).
+    const int64_t c1 = (sub == 0 && nxt && deg_u > 0) ? 1 : 0;
+    const int64_t d1 = c1 ? deg_u : 0;
+    int64_t tc, td;
+    const int64_t ec = block_excl_scan<kThreads>(c1, L.scan_c, &tc);
+    const int64_t ed = block_excl_scan<kThreads>(d1, L.scan_d, &td);
+    if (threadIdx.x == 0) {
+      const unsigned long long add = (static_cast<unsigned long long>(td) << kCountBits) |
+                                     static_cast<unsigned long long>(tc);
+      const unsigned long long old = tc > 0 ? atomicAdd(&c->packed, add) : 0ull;
+      L.basec = old & kCountMask;
+      L.based = old >> kCountBits;
+    }
+    __syncthreads();
+    if (c1) {
+      oq[static_cast<int64_t>(L.basec) + ec] = static_cast<int32_t>(u + a.vbegin);
+      oqoff[static_cast<int64_t>(L.basec) + ec] = static_cast<int64_t>(L.based) + ed;
+    }
+    __syncthreads();
+  }
+  // Each group counted its scan in every lane; keep one lane's count.
+  if (sub != 0) scanned = 0;
+  scanned = wave_sum(scanned);
+  if (lane_id() == 0 && scanned) atomicAdd(&c->scanned[blockIdx.x & (kSlots - 1)], scanned);
+}
+
+__device__ void epilogue(const BArgs& a, BCtrl* c, int phase) {
+  const unsigned long long packed = ld_agent(&c->packed);
+  c->packed = 0;
+  const long long cnt = static_cast<long long>(packed & kCountMask);
+  const long long degs = static_cast<long long>(packed >> kCountBits);
+  const int level = c->level;
+  if (level < DMM_MAX_STEPS) {
+    c->level_kind[level] = phase == kBTD ? DMM_STEP_TD : DMM_STEP_BU;
+    c->level_active[level] = c->acount;
+  }
+  if (phase == kBBU) {
+    unsigned long long s = 0;
+    for (int i = 0; i < kSlots; i++) {
+      s += ld_agent(&c->scanned[i]);
+      c->scanned[i] = 0;
+    }
+    c->sweep_scanned_total += static_cast<long long>(s);
+    c->sweeps = c->sweeps + 1;
+  }
+  c->acount = cnt;
+  c->aedges = degs;
+  c->cur = 1 - c->cur;
+  c->level = level + 1;
+  c->nlevels = level + 2;  // levels 0..level+1 hold frontiers
+  if (cnt == 0) {
+    c->phase = kBDone;
+    c->done = 1;
+    c->nlevels = level + 1;
+  } else if (level + 2 >= a.level_cap) {
+    c->phase = kBDone;  // out of level storage: the host grows and reruns
+    c->done = 3;
+  } else {
+    c->phase = (degs * a.td_div < a.m) ? kBTD : kBBU;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bstep(BArgs a) {
+  __shared__ BLds lds;
+  __shared__ int s_last;
+  BCtrl* c = a.ctrl;
+  const int phase = c->phase;
+  if (phase != kBTD && phase != kBBU) return;
+  const int level = c->level;
+  zero_next_next(a, level);
+  if (phase == kBTD)
+    td_phase(a, c, lds.td, level);
+  else
+    bu_phase(a, c, lds.sm, level);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(&c->ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    c->ticket = 0;
+    epilogue(a, c, phase);
+    __threadfence();
+  }
+}
+
+// Level 0: F[0] = sources' lanes, F[1] cleared, seen cleared; active list =
+// the distinct local sources.
+__global__ __launch_bounds__(kThreads) void k_binit(BArgs a, const int32_t* srcs, int nsrc) {
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t i = gtid; i < a.nloc; i += gsize) {
+    a.seen[i] = 0;
+    a.levels[i] = 0;
+    if (a.level_cap > 1) a.levels[a.nloc + i] = 0;
+  }
+}
+
+__global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
+  // Single workgroup: set the source lanes and build the level-0 list.
+  __shared__ int cnt;
+  BCtrl* c = a.ctrl;
+  if (threadIdx.x == 0) {
+    cnt = 0;
+    long long degs = 0;
+    for (int s = 0; s < nsrc; s++) {
+      const int64_t lu = int64_t(srcs[s]) - a.vbegin;
+      if (lu < 0 || lu >= a.nloc) continue;
+      const unsigned long long old =
+          atomicOr(reinterpret_cast<unsigned long long*>(&a.levels[lu]), 1ull << s);
+      // A source without out-edges reaches only itself: no list entry.
+      if (old == 0ull && a.out_off[lu + 1] > a.out_off[lu]) {
+        a.act0[cnt] = srcs[s];
+        a.aoff0[cnt] = degs;
+        degs += a.out_off[lu + 1] - a.out_off[lu];
+        cnt++;
+      }
+    }
+    c->packed = 0;
+    for (int i = 0; i < kSlots; i++) c->scanned[i] = 0;
+    c->acount = cnt;
+    c->aedges = degs;
+    c->level = 0;
+    c->done = 0;
+    c->cur = 0;
+    c->nlevels = 1;
+    c->sweeps = 0;
+    c->ticket = 0;
+    c->pad = 0;
+    c->sweep_scanned_total = 0;
+    c->phase = cnt == 0 ? kBDone : ((degs * a.td_div < a.m) ? kBTD : kBBU);
+    if (cnt == 0) c->done = 1;
+  }
+}
+
+// Depth output: per vertex, the level bits of each lane are gathered into
+// eight bit-slice masks (bit j of the depth), then lane s's byte is written
+// to depth[s * nloc + u]; consecutive threads write consecutive bytes.
+__global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int nsrc) {
+  const int nlev = a.ctrl->nlevels;
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+    uint64_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t reached = 0;
+    for (int L = 0; L < nlev; L++) {
+      const uint64_t f = a.levels[L * a.nloc + u];
+      reached |= f;
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if ((L >> j) & 1) m[j] |= f;
+    }
+#pragma unroll
+    for (int s = 0; s < 64; s++) {
+      if (s < nsrc) {
+        uint32_t d = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) d |= static_cast<uint32_t>((m[j] >> s) & 1ull) << j;
+        out[int64_t(s) * a.nloc + u] = ((reached >> s) & 1ull) ? static_cast<uint8_t>(d) : 255;
+      }
+    }
+  }
+}
+
+// Sum over lanes of the out-degrees of reached vertices (GTEPS numerator).
+__global__ __launch_bounds__(kThreads) void k_breach(BArgs a, unsigned long long* out) {
+  __shared__ unsigned long long red[kThreads / 64];
+  const int nlev = a.ctrl->nlevels;
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  unsigned long long acc = 0;
+  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+    uint64_t reached = 0;
+    for (int L = 0; L < nlev; L++) reached |= a.levels[L * a.nloc + u];
+    acc += static_cast<unsigned long long>(__popcll(reached)) *
+           static_cast<unsigned long long>(a.out_off[u + 1] - a.out_off[u]);
+  }
+  unsigned long long s = block_sum<kThreads>(acc, red);
+  if (threadIdx.x == 0 && s) atomicAdd(out, s);
+}
+
+BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
+  BArgs a;
+  a.out_off = ctx->d_out_off;
+  a.out_nbr = ctx->d_out_nbr;
+  a.in_off = ctx->d_in_off;
+  a.in_nbr = ctx->d_in_nbr;
+  a.nloc = ctx->nloc;
+  a.vbegin = ctx->vbegin;
+  a.levels = ctx->d_levels;
+  a.level_cap = ctx->level_cap;
+  a.seen = ctx->d_seen;
+  a.act0 = ctx->d_bact;
+  a.act1 = ctx->d_bact + ctx->nloc + 64;
+  a.aoff0 = ctx->d_baoff;
+  a.aoff1 = ctx->d_baoff + ctx->nloc + 64;
+  a.ctrl = reinterpret_cast<BCtrl*>(ctx->d_bctrl);
+  a.lanes = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
+  a.m = std::max<int64_t>(1, ctx->m_out);
+  a.td_div = std::max(1, td_div);
+  return a;
+}
+
+int ensure_levels(dmm_ctx* ctx, int64_t cap) {
+  if (ctx->d_levels && ctx->level_cap >= cap) return DMM_OK;
+  if (ctx->d_levels) dev_free(ctx->d_levels);
+  ctx->d_levels = nullptr;
+  ctx->level_cap = 0;
+  int rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_levels), sizeof(uint64_t) * cap * ctx->nloc,
+                     "batch level frontiers");
+  if (rc) return rc;
+  ctx->level_cap = cap;
+  if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
+  ctx->batch_graph = nullptr;
+  return DMM_OK;
+}
+
+}  // namespace
+
+size_t batch_ctrl_bytes() { return sizeof(BCtrl); }
+
+int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth,
+              const dmm_batch_params* params, dmm_batch_stats* stats) {
+  dmm_batch_params p = params ? *params : ctx->tuning.batch;
+  if (p.td_div <= 0) p.td_div = ctx->tuning.batch.td_div;
+  if (p.graph_levels <= 0) p.graph_levels = ctx->tuning.batch.graph_levels;
+  int rc = ensure_levels(ctx, std::max<int64_t>(ctx->level_cap, 32));
+  if (rc) return rc;
+  // Sources to the device (pinned staging).
+  std::copy(sources, sources + nsrc, static_cast<int32_t*>(ctx->h_bsrc));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bsrc, ctx->h_bsrc, sizeof(int32_t) * 64, hipMemcpyHostToDevice,
+                                    ctx->stream));
+  for (int attempt = 0; attempt < 4; attempt++) {
+    const int key = p.td_div * 1000 + p.graph_levels * 10 + (nsrc == 64 ? 1 : 0) + nsrc * 100000;
+    BArgs a = make_args(ctx, nsrc, p.td_div);
+    const int blocks = std::max(1, ctx->cu_count) * ctx->tuning.grid_per_cu;
+    if (!ctx->batch_graph || ctx->batch_graph_key != key) {
+      if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
+      ctx->batch_graph = nullptr;
+      hipGraph_t g = nullptr;
+      DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_binit, dim3(blocks), dim3(kThreads), 0, ctx->stream, a,
+                         static_cast<const int32_t*>(ctx->d_bsrc), nsrc);
+      hipLaunchKernelGGL(k_binit2, dim3(1), dim3(64), 0, ctx->stream, a,
+                         static_cast<const int32_t*>(ctx->d_bsrc), nsrc);
+      for (int i = 0; i < p.graph_levels; i++)
+        hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&ctx->batch_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
+      ctx->batch_graph_key = key;
+      if (!ctx->batch_cont) {
+        DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < 4; i++)
+          hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+        e = hipStreamEndCapture(ctx->stream, &g);
+        if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ctx->batch_cont, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
+      }
+    }
+    BCtrl* h = reinterpret_cast<BCtrl*>(ctx->h_bctrl);
+    DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->batch_graph, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_bctrl, sizeof(BCtrl), hipMemcpyDeviceToHost, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int guard = 0;
+    while (h->done == 0) {
+      if (++guard > 1000) return set_err(ctx, DMM_ERR_STATE, "batched BFS did not terminate");
+      DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->batch_cont, ctx->stream));
+      DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+      DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_bctrl, sizeof(BCtrl), hipMemcpyDeviceToHost, ctx->stream));
+      DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (h->done == 3) {
+      // More levels than storage: grow (up to 255 levels for uint8 depths).
+      if (ctx->level_cap >= 255)
+        return set_err(ctx, DMM_ERR_CAPACITY, "batched BFS deeper than 254 levels");
+      rc = ensure_levels(ctx, std::min<int64_t>(255, ctx->level_cap * 2));
+      if (rc) return rc;
+      if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
+      ctx->batch_cont = nullptr;
+      continue;
+    }
+    float ms = 0;
+    DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+    if (stats) {
+      stats->total_ms = ms;
+      stats->num_levels = h->nlevels;
+      stats->sweeps = h->sweeps;
+      const int nl = std::min(h->level, DMM_MAX_STEPS);
+      for (int i = 0; i < nl; i++) {
+        stats->level_kind[i] = h->level_kind[i];
+        stats->level_active[i] = h->level_active[i];
+      }
+      // Sweep bytes: per sweep, seen + F[L][u] reads, offsets, F[L+1] and
+      // seen writes (40 N + 8) plus 12 bytes per scanned edge (4 B ID +
+      // 8 B frontier word).
+      stats->sweep_alg_bytes =
+          h->sweeps * (40 * ctx->nloc + 8) + 12 * static_cast<int64_t>(h->sweep_scanned_total);
+      stats->sweep_ms = 0;
+    }
+    if (depth) {
+      if (!ctx->d_bdepth || ctx->bdepth_cap < int64_t(nsrc) * ctx->nloc) {
+        if (ctx->d_bdepth) dev_free(ctx->d_bdepth);
+        ctx->d_bdepth = nullptr;
+        rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bdepth), int64_t(nsrc) * ctx->nloc, "batch depth");
+        if (rc) return rc;
+        ctx->bdepth_cap = int64_t(nsrc) * ctx->nloc;
+      }
+      hipLaunchKernelGGL(k_bdepth, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, ctx->d_bdepth, nsrc);
+      DMM_HIP_CHECK(ctx, hipGetLastError());
+      DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_bdepth, int64_t(nsrc) * ctx->nloc, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+      DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (stats) {
+      unsigned long long* out = reinterpret_cast<unsigned long long*>(ctx->d_scratch64);
+      launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t*>(out), 1, 0);
+      hipLaunchKernelGGL(k_breach, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, out);
+      unsigned long long r = 0;
+      DMM_HIP_CHECK(ctx, hipMemcpyAsync(&r, out, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+      DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      stats->reached_edges = static_cast<int64_t>(r);
+    }
+    return DMM_OK;
+  }
+  return set_err(ctx, DMM_ERR_CAPACITY, "batched BFS level storage could not be grown");
+}
+
+}  // namespace dmm

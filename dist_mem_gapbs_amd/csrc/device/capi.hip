@@ -1,0 +1,739 @@
+// C ABI of the device engine (include/dmm_gapbs.h): context, graph upload,
+// DOBFS (device loop and host-driven steps), batched BFS, PageRank and the
+// autotuner entry points.
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../host/error.h"
+#include "batch.h"
+#include "bfs.h"
+#include "common.h"
+#include "pr.h"
+
+namespace dmm {
+
+int set_err(dmm_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  thread_error() = buf;
+  return code;
+}
+
+int dev_alloc(dmm_ctx* ctx, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_err(ctx, DMM_ERR_NOMEM, "hipMalloc of %zu bytes for %s failed: %s", bytes, what,
+                   hipGetErrorString(e));
+  }
+  ctx->dev_bytes += static_cast<int64_t>(bytes);
+  return DMM_OK;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+}  // namespace dmm
+
+using dmm::set_err;
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+void free_graph_state(dmm_ctx* ctx) {
+  if (ctx->bfs_graph) (void)hipGraphExecDestroy(ctx->bfs_graph);
+  if (ctx->bfs_cont) (void)hipGraphExecDestroy(ctx->bfs_cont);
+  if (ctx->pr_graph) (void)hipGraphExecDestroy(ctx->pr_graph);
+  if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
+  if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
+  ctx->bfs_graph = ctx->bfs_cont = ctx->pr_graph = ctx->batch_graph = ctx->batch_cont = nullptr;
+  ctx->pr_graph_key = -1;
+  ctx->batch_graph_key = -1;
+  ctx->bfs_graph_steps = 0;
+  void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
+                  ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],  ctx->d_blk,
+                  ctx->d_score,   ctx->d_contrib[0], ctx->d_contrib[1], ctx->d_score64,
+                  ctx->d_contrib64[0], ctx->d_contrib64[1], ctx->d_err_part, ctx->d_heavy,
+                  ctx->d_gcontrib, ctx->d_seen, ctx->d_levels, ctx->d_bact, ctx->d_baoff,
+                  ctx->d_bdepth};
+  for (void* p : bufs) dmm::dev_free(p);
+  if (ctx->directed) {
+    dmm::dev_free(ctx->d_in_off);
+    dmm::dev_free(ctx->d_in_nbr);
+  }
+  ctx->d_out_off = ctx->d_in_off = nullptr;
+  ctx->d_out_nbr = ctx->d_in_nbr = nullptr;
+  ctx->d_parent = ctx->d_depth = ctx->d_queue = nullptr;
+  ctx->d_qoff = ctx->d_blk = nullptr;
+  ctx->d_bm[0] = ctx->d_bm[1] = ctx->d_bm[2] = nullptr;
+  ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
+  ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
+  ctx->d_heavy = nullptr;
+  ctx->d_seen = ctx->d_levels = nullptr;
+  ctx->d_bact = nullptr;
+  ctx->d_baoff = nullptr;
+  ctx->d_bdepth = nullptr;
+  ctx->level_cap = 0;
+  ctx->bdepth_cap = 0;
+  ctx->has_graph = false;
+  ctx->bfs_valid = false;
+  ctx->dev_bytes = ctx->fixed_bytes;
+}
+
+int need_graph(dmm_ctx* ctx) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (!ctx->has_graph) return set_err(ctx, DMM_ERR_STATE, "no graph uploaded");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return DMM_OK;
+}
+
+// Validates one CSR block: monotone offsets starting at 0 and IDs in range.
+int check_csr(dmm_ctx* ctx, int64_t rows, const int64_t* off, const int32_t* nbr, int64_t n,
+              const char* which) {
+  if (!off) return set_err(ctx, DMM_ERR_INVALID, "%s offsets are NULL", which);
+  if (off[0] != 0) return set_err(ctx, DMM_ERR_GRAPH, "%s offsets must start at 0", which);
+  for (int64_t i = 0; i < rows; i++)
+    if (off[i + 1] < off[i]) return set_err(ctx, DMM_ERR_GRAPH, "%s offsets decrease at row %lld", which, (long long)i);
+  const int64_t m = off[rows];
+  if (m > 0 && !nbr) return set_err(ctx, DMM_ERR_INVALID, "%s neighbours are NULL", which);
+  for (int64_t e = 0; e < m; e++)
+    if (nbr[e] < 0 || nbr[e] >= n)
+      return set_err(ctx, DMM_ERR_GRAPH, "%s neighbour %lld = %d outside [0, %lld)", which, (long long)e, nbr[e],
+                     (long long)n);
+  return DMM_OK;
+}
+
+int ensure_pr_state(dmm_ctx* ctx, int mode) {
+  int rc;
+  const int64_t nl = ctx->nloc;
+  if (mode == DMM_PR_FP64) {
+    if (!ctx->d_score64) {
+      if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score64, sizeof(double) * nl, "FP64 scores"))) return rc;
+      if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib64[0], sizeof(double) * nl, "FP64 contrib"))) return rc;
+      if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib64[1], sizeof(double) * nl, "FP64 contrib"))) return rc;
+    }
+  } else if (!ctx->d_score) {
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score, sizeof(float) * nl, "scores"))) return rc;
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[0], sizeof(float) * nl, "contrib"))) return rc;
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
+  }
+  if (!ctx->d_err_part) {
+    const int64_t parts = dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx);
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
+    ctx->err_part_cap = parts;
+  }
+  return DMM_OK;
+}
+
+int ensure_batch_state(dmm_ctx* ctx) {
+  int rc;
+  if (!ctx->d_seen) {
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_seen, sizeof(uint64_t) * ctx->nloc, "batch seen"))) return rc;
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_bact, sizeof(int32_t) * 2 * (ctx->nloc + 64), "batch lists")))
+      return rc;
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_baoff, sizeof(int64_t) * 2 * (ctx->nloc + 64), "batch lists")))
+      return rc;
+  }
+  return DMM_OK;
+}
+
+int copy_trace(dmm_ctx* ctx, dmm_bfs_stats* st) {
+  dmm::BfsCtrl* h = ctx->h_ctrl;
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_ctrl, sizeof(dmm::BfsCtrl), hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int ns = std::min(h->step, DMM_MAX_STEPS);
+  st->num_steps = ns;
+  st->num_levels = 0;
+  int64_t bytes = 8 * (ctx->nloc + 1) + 4 * ctx->nloc;  // InitParent
+  int64_t bu_bytes = 0;
+  const int64_t nb8 = ctx->n / 8;
+  for (int i = 0; i < ns; i++) {
+    st->step_kind[i] = h->step_kind[i];
+    st->step_in[i] = h->step_in[i];
+    st->step_out[i] = h->step_out[i];
+    st->step_edges[i] = h->step_edges[i];
+    st->step_scout[i] = h->step_scout[i];
+    // Algorithmic bytes per step (the per-unit figures of the measurement
+    // plan): TD 16|F| + 8 E + 4|F'|; BU 20 U + 4 E + E/8 + N/8;
+    // conversions 4|F| + N/8.
+    switch (h->step_kind[i]) {
+      case DMM_STEP_TD:
+        bytes += 16 * h->step_in[i] + 8 * h->step_edges[i] + 4 * h->step_out[i];
+        st->num_levels++;
+        break;
+      case DMM_STEP_BU: {
+        const int64_t b = 20 * h->step_in[i] + 4 * h->step_edges[i] + h->step_edges[i] / 8 + nb8;
+        bytes += b;
+        bu_bytes += b;
+        st->num_levels++;
+        break;
+      }
+      default:
+        bytes += 4 * h->step_in[i] + nb8;
+    }
+  }
+  st->alg_bytes = bytes;
+  st->bu_alg_bytes = bu_bytes;
+  return DMM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* dmm_last_error(const struct dmm_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return dmm::thread_error().c_str();
+}
+
+const char* dmm_version(void) { return "dmm-gapbs-mi355x 0.1 (gfx950)"; }
+
+int dmm_device_count(int32_t* count) {
+  if (!count) return set_err(nullptr, DMM_ERR_INVALID, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count = 0;
+    (void)hipGetLastError();
+    return DMM_OK;
+  }
+  *count = c;
+  return DMM_OK;
+}
+
+int dmm_ctx_create(int32_t device, dmm_ctx** out) {
+  if (!out) return set_err(nullptr, DMM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return set_err(nullptr, DMM_ERR_NODEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= count)
+    return set_err(nullptr, DMM_ERR_INVALID, "device %d outside [0, %d)", device, count);
+  dmm_ctx* ctx = new (std::nothrow) dmm_ctx;
+  if (!ctx) return set_err(nullptr, DMM_ERR_NOMEM, "ctx allocation failed");
+  ctx->device = device;
+  auto bail = [&](int code) {
+    dmm_ctx_destroy(ctx);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(set_err(nullptr, DMM_ERR_HIP, "hipSetDevice failed"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return bail(set_err(nullptr, DMM_ERR_HIP, "hipGetDeviceProperties failed"));
+  ctx->cu_count = prop.multiProcessorCount;
+  ctx->arch = prop.gcnArchName;
+  // Default-priority, non-blocking stream.
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(set_err(nullptr, DMM_ERR_HIP, "stream creation failed"));
+  if (hipEventCreate(&ctx->ev_a) != hipSuccess || hipEventCreate(&ctx->ev_b) != hipSuccess)
+    return bail(set_err(nullptr, DMM_ERR_HIP, "event creation failed"));
+  ctx->step_events.resize(2 * DMM_MAX_STEPS + 2, nullptr);
+  for (auto& e : ctx->step_events)
+    if (hipEventCreate(&e) != hipSuccess) return bail(set_err(nullptr, DMM_ERR_HIP, "event creation failed"));
+  int rc;
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_ctrl, sizeof(dmm::BfsCtrl), "BFS control"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_params, 64, "BFS params"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_scratch64, 64, "scratch"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_prctrl, sizeof(dmm::PrCtrl), "PR control"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_prparams, dmm::kPrParamsBytes, "PR params"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_trace, sizeof(double) * DMM_MAX_PR_ITERS, "PR trace")))
+    return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_bctrl, dmm::batch_ctrl_bytes(), "batch control"))) return bail(rc);
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_bsrc, 64 * sizeof(int32_t), "batch sources"))) return bail(rc);
+  if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(dmm::BfsCtrl), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_params, 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_prctrl, sizeof(dmm::PrCtrl), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_prparams, dmm::kPrParamsBytes, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_bctrl, dmm::batch_ctrl_bytes(), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_bsrc, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    return bail(set_err(nullptr, DMM_ERR_NOMEM, "pinned host allocation failed"));
+  std::memset(ctx->h_bsrc, 0, 64 * sizeof(int32_t));
+  ctx->fixed_bytes = ctx->dev_bytes;
+  *out = ctx;
+  return DMM_OK;
+}
+
+void dmm_ctx_destroy(dmm_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_graph_state(ctx);
+  void* bufs[] = {ctx->d_ctrl, ctx->d_params, ctx->d_scratch64, ctx->d_prctrl, ctx->d_prparams,
+                  ctx->d_err_trace, ctx->d_bctrl, ctx->d_bsrc};
+  for (void* p : bufs) dmm::dev_free(p);
+  void* hbufs[] = {ctx->h_ctrl, ctx->h_params, ctx->h_prctrl, ctx->h_prparams, ctx->h_bctrl, ctx->h_bsrc};
+  for (void* p : hbufs)
+    if (p) (void)hipHostFree(p);
+  for (auto e : ctx->step_events)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+  if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int dmm_ctx_device_bytes(const dmm_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return set_err(nullptr, DMM_ERR_INVALID, "NULL argument");
+  *bytes = ctx->dev_bytes;
+  return DMM_OK;
+}
+
+int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t vend,
+                     const int64_t* out_offsets, const int32_t* out_neighs, const int64_t* in_offsets,
+                     const int32_t* in_neighs, int64_t num_edges_directed) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (num_nodes <= 0 || num_nodes > 0x7FFFFFFFll)
+    return set_err(ctx, DMM_ERR_INVALID, "num_nodes %lld outside [1, 2^31)", (long long)num_nodes);
+  if (vbegin < 0 || vend > num_nodes || vbegin >= vend)
+    return set_err(ctx, DMM_ERR_INVALID, "bad block [%lld, %lld)", (long long)vbegin, (long long)vend);
+  const int64_t rows = vend - vbegin;
+  if (rows >= (int64_t(1) << dmm::kCountBits))
+    return set_err(ctx, DMM_ERR_CAPACITY, "block of %lld vertices exceeds 2^30", (long long)rows);
+  int rc = check_csr(ctx, rows, out_offsets, out_neighs, num_nodes, "out");
+  if (rc) return rc;
+  const bool directed = in_offsets != nullptr;
+  if (directed && (rc = check_csr(ctx, rows, in_offsets, in_neighs, num_nodes, "in"))) return rc;
+  if (out_offsets[rows] >= (int64_t(1) << 34))
+    return set_err(ctx, DMM_ERR_CAPACITY, "block has more than 2^34 edges");
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  free_graph_state(ctx);
+  ctx->n = num_nodes;
+  ctx->vbegin = vbegin;
+  ctx->vend = vend;
+  ctx->nloc = rows;
+  ctx->words = (num_nodes + 63) / 64;
+  ctx->directed = directed;
+  ctx->m_out = out_offsets[rows];
+  ctx->m_in = directed ? in_offsets[rows] : ctx->m_out;
+  ctx->ned = num_edges_directed >= 0 ? num_edges_directed : ctx->m_out;
+  // Partition::max_width (src/partition.h:97-98) of the PE owning the block:
+  // the block width itself (every PE's width except the last PE's remainder).
+  ctx->beta_width = rows;
+  auto A = [&](void** p, size_t bytes, const char* what) { return dmm::dev_alloc(ctx, p, bytes, what); };
+  if ((rc = A((void**)&ctx->d_out_off, sizeof(int64_t) * (rows + 1), "out offsets")) ||
+      (rc = A((void**)&ctx->d_out_nbr, sizeof(int32_t) * ctx->m_out, "out neighbours")))
+    return rc;
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_off, out_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_nbr, out_neighs, sizeof(int32_t) * ctx->m_out, hipMemcpyHostToDevice));
+  if (directed) {
+    if ((rc = A((void**)&ctx->d_in_off, sizeof(int64_t) * (rows + 1), "in offsets")) ||
+        (rc = A((void**)&ctx->d_in_nbr, sizeof(int32_t) * ctx->m_in, "in neighbours")))
+      return rc;
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_off, in_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_nbr, in_neighs, sizeof(int32_t) * ctx->m_in, hipMemcpyHostToDevice));
+  } else {
+    ctx->d_in_off = ctx->d_out_off;
+    ctx->d_in_nbr = ctx->d_out_nbr;
+  }
+  // BFS state.
+  const int64_t nblk = std::max<int64_t>(1, dmm::ceil_div(ctx->words, 256 * 4));
+  if ((rc = A((void**)&ctx->d_parent, sizeof(int32_t) * rows, "parent")) ||
+      (rc = A((void**)&ctx->d_depth, sizeof(int32_t) * rows, "depth")) ||
+      (rc = A((void**)&ctx->d_queue, sizeof(int32_t) * (rows + 64), "queue")) ||
+      (rc = A((void**)&ctx->d_qoff, sizeof(int64_t) * (rows + 64), "queue offsets")) ||
+      (rc = A((void**)&ctx->d_bm[0], sizeof(uint64_t) * ctx->words, "bitmap")) ||
+      (rc = A((void**)&ctx->d_bm[1], sizeof(uint64_t) * ctx->words, "bitmap")) ||
+      (rc = A((void**)&ctx->d_bm[2], sizeof(uint64_t) * ctx->words, "bitmap")) ||
+      (rc = A((void**)&ctx->d_blk, sizeof(int64_t) * 3 * nblk, "b2q partials")))
+    return rc;
+  ctx->blk_cap = nblk;
+  // PageRank heavy-vertex list: rows with in-degree >= threshold get a whole
+  // wavefront each.
+  const int64_t* ioff = directed ? in_offsets : out_offsets;
+  std::vector<int32_t> heavy;
+  for (int64_t r = 0; r < rows; r++)
+    if (ioff[r + 1] - ioff[r] >= ctx->heavy_threshold) heavy.push_back(static_cast<int32_t>(r));
+  ctx->n_heavy = static_cast<int64_t>(heavy.size());
+  if ((rc = A((void**)&ctx->d_heavy, sizeof(int32_t) * std::max<size_t>(1, heavy.size()), "heavy list"))) return rc;
+  if (!heavy.empty())
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_heavy, heavy.data(), sizeof(int32_t) * heavy.size(), hipMemcpyHostToDevice));
+  ctx->has_graph = true;
+  return DMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BFS
+// ---------------------------------------------------------------------------
+
+int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* parent, int32_t* depth,
+            int32_t trace, dmm_bfs_stats* stats) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (source < 0 || source >= ctx->n)
+    return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
+  if (alpha <= 0 || beta <= 0) return set_err(ctx, DMM_ERR_INVALID, "alpha and beta must be positive");
+  const double t0 = now_ms();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (!trace) {
+    rc = dmm::bfs_device_loop(ctx, source, alpha, beta, stats);
+    if (rc) return rc;
+  } else {
+    // Host-driven replica of src/bfs.cc:142-188, one kernel per step, each
+    // bracketed by events for the per-step times.
+    std::vector<hipEvent_t>& ev = ctx->step_events;
+    DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+    if ((rc = dmm::bfs_init(ctx, source))) return rc;
+    int64_t edges_to_check = ctx->ned;
+    int64_t scout_count = ctx->h_ctrl->scout;
+    int64_t qsize = (ctx->h_ctrl->qe - ctx->h_ctrl->qb) + ctx->h_ctrl->window_zero;
+    int si = 0;
+    auto mark = [&](int i) -> int {
+      if (i < static_cast<int>(ev.size())) DMM_HIP_CHECK(ctx, hipEventRecord(ev[i], ctx->stream));
+      return DMM_OK;
+    };
+    std::vector<std::pair<int, int>> step_ev;  // (begin, end) event index per step
+    auto run = [&](auto fn) -> int {
+      int b = si++;
+      int r = mark(b);
+      if (r) return r;
+      r = fn();
+      if (r) return r;
+      int e = si++;
+      r = mark(e);
+      step_ev.push_back({b, e});
+      return r;
+    };
+    while (qsize != 0) {
+      if (step_ev.size() + 3 >= DMM_MAX_STEPS)
+        return set_err(ctx, DMM_ERR_CAPACITY, "BFS exceeded %d traced steps", DMM_MAX_STEPS);
+      if (scout_count > edges_to_check / alpha) {
+        int64_t awake_count = 0, old_awake_count;
+        if ((rc = run([&] { return dmm::bfs_queue_to_bitmap(ctx, &awake_count); }))) return rc;
+        do {
+          old_awake_count = awake_count;
+          if ((rc = run([&] { return dmm::bfs_bu_step(ctx, &awake_count); }))) return rc;
+          if (step_ev.size() + 2 >= DMM_MAX_STEPS)
+            return set_err(ctx, DMM_ERR_CAPACITY, "BFS exceeded %d traced steps", DMM_MAX_STEPS);
+        } while ((awake_count >= old_awake_count) || (awake_count > ctx->beta_width / beta));
+        if ((rc = run([&] { return dmm::bfs_bitmap_to_queue(ctx, &qsize); }))) return rc;
+        scout_count = 1;
+      } else {
+        edges_to_check -= scout_count;
+        if ((rc = run([&] { return dmm::bfs_td_step(ctx, &scout_count, &qsize); }))) return rc;
+      }
+    }
+    if ((rc = dmm::bfs_finish(ctx))) return rc;
+    DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_b));
+    if (stats) {
+      float ms = 0;
+      DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+      stats->total_ms = ms;
+      for (size_t i = 0; i < step_ev.size() && i < DMM_MAX_STEPS; i++) {
+        float sm = 0;
+        if (step_ev[i].second < static_cast<int>(ev.size()))
+          DMM_HIP_CHECK(ctx, hipEventElapsedTime(&sm, ev[step_ev[i].first], ev[step_ev[i].second]));
+        stats->step_ms[i] = sm;
+      }
+    }
+  }
+  if (stats) {
+    if ((rc = copy_trace(ctx, stats))) return rc;
+    if (trace) {
+      for (int i = 0; i < stats->num_steps; i++)
+        if (stats->step_kind[i] == DMM_STEP_BU) stats->bu_ms += stats->step_ms[i];
+    }
+    if ((rc = dmm::bfs_reach_stats(ctx, &stats->reached, &stats->reached_edges))) return rc;
+  }
+  if ((rc = dmm_bfs_read(ctx, parent, depth))) return rc;
+  if (stats) stats->host_ms = now_ms() - t0;
+  return DMM_OK;
+}
+
+int dmm_bfs_read(dmm_ctx* ctx, int32_t* parent, int32_t* depth) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "no BFS result on the ctx");
+  if (parent)
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(parent, ctx->d_parent, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+  if (depth)
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_depth, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
+}
+
+int dmm_step_init(dmm_ctx* ctx, int32_t source) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (source < 0 || source >= ctx->n)
+    return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
+  return dmm::bfs_init(ctx, source);
+}
+
+int dmm_step_td(dmm_ctx* ctx, int64_t* scout_count, int64_t* queue_size) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  return dmm::bfs_td_step(ctx, scout_count, queue_size);
+}
+
+int dmm_step_queue_to_bitmap(dmm_ctx* ctx, int64_t* awake_count) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  return dmm::bfs_queue_to_bitmap(ctx, awake_count);
+}
+
+int dmm_step_bu(dmm_ctx* ctx, int64_t* awake_count) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  return dmm::bfs_bu_step(ctx, awake_count);
+}
+
+int dmm_step_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  return dmm::bfs_bitmap_to_queue(ctx, queue_size);
+}
+
+int dmm_step_finish(dmm_ctx* ctx) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  return dmm::bfs_finish(ctx);
+}
+
+int dmm_state_get_parent(dmm_ctx* ctx, int32_t* parent) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!parent) return set_err(ctx, DMM_ERR_INVALID, "parent is NULL");
+  DMM_HIP_CHECK(ctx, hipMemcpy(parent, ctx->d_parent, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost));
+  return DMM_OK;
+}
+
+int dmm_state_set_parent(dmm_ctx* ctx, const int32_t* parent) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!parent) return set_err(ctx, DMM_ERR_INVALID, "parent is NULL");
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_parent, parent, sizeof(int32_t) * ctx->nloc, hipMemcpyHostToDevice));
+  return DMM_OK;
+}
+
+int dmm_state_get_queue(dmm_ctx* ctx, int32_t* queue, int64_t capacity, int64_t* size) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  const int64_t qb = ctx->h_ctrl->qb, qe = ctx->h_ctrl->qe;
+  if (size) *size = qe - qb;
+  if (qe - qb > capacity) return set_err(ctx, DMM_ERR_CAPACITY, "queue window of %lld > capacity", (long long)(qe - qb));
+  if (queue && qe > qb)
+    DMM_HIP_CHECK(ctx, hipMemcpy(queue, ctx->d_queue + qb, sizeof(int32_t) * (qe - qb), hipMemcpyDeviceToHost));
+  return DMM_OK;
+}
+
+// Sets the window to `queue` (with its degree prefix), placed at the start
+// of the queue array.
+__global__ void k_set_window(dmm::BfsCtrl* c, long long size, long long edges, long long zero) {
+  c->qb = 0;
+  c->qe = size;
+  c->qin = size;
+  c->window_edges = edges;
+  c->window_zero = zero;
+  c->packed = 0;
+  c->zero_deg = 0;
+}
+
+int dmm_state_set_queue(dmm_ctx* ctx, const int32_t* queue, int64_t size) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
+  if (size < 0 || size > ctx->nloc) return set_err(ctx, DMM_ERR_CAPACITY, "queue size %lld", (long long)size);
+  if (size > 0 && !queue) return set_err(ctx, DMM_ERR_INVALID, "queue is NULL");
+  // Degree prefix from the host copy of the offsets is not kept; fetch the
+  // needed offsets from the device.
+  std::vector<int64_t> hoff(ctx->nloc + 1);
+  DMM_HIP_CHECK(ctx, hipMemcpy(hoff.data(), ctx->d_out_off, sizeof(int64_t) * (ctx->nloc + 1), hipMemcpyDeviceToHost));
+  // Vertices of out-degree 0 are counted, not stored (the device queue
+  // invariant); the rest keep their order.
+  std::vector<int32_t> kept;
+  std::vector<int64_t> qoff;
+  kept.reserve(size);
+  qoff.reserve(size);
+  int64_t run = 0, zero = 0;
+  for (int64_t i = 0; i < size; i++) {
+    const int64_t lv = int64_t(queue[i]) - ctx->vbegin;
+    if (lv < 0 || lv >= ctx->nloc) return set_err(ctx, DMM_ERR_INVALID, "queue entry %d not local", queue[i]);
+    const int64_t d = hoff[lv + 1] - hoff[lv];
+    if (d == 0) {
+      zero++;
+      continue;
+    }
+    kept.push_back(queue[i]);
+    qoff.push_back(run);
+    run += d;
+  }
+  const int64_t nk = static_cast<int64_t>(kept.size());
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (nk > 0) {
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_queue, kept.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice));
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_qoff, qoff.data(), sizeof(int64_t) * nk, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_set_window, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl, (long long)nk, (long long)run,
+                     (long long)zero);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
+}
+
+int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
+  const int idx = which == 0 ? f : t;
+  DMM_HIP_CHECK(ctx, hipMemcpy(words, ctx->d_bm[idx], sizeof(uint64_t) * ctx->words, hipMemcpyDeviceToHost));
+  return DMM_OK;
+}
+
+int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
+  const int idx = which == 0 ? f : t;
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_bm[idx], words, sizeof(uint64_t) * ctx->words, hipMemcpyHostToDevice));
+  return DMM_OK;
+}
+
+__global__ void k_set_level(dmm::BfsCtrl* c, int level) { c->level = level; }
+
+int dmm_state_set_level(dmm_ctx* ctx, int32_t level) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl, level);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched BFS
+// ---------------------------------------------------------------------------
+
+int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth,
+                    const dmm_batch_params* params, dmm_batch_stats* stats) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!sources || nsrc < 1 || nsrc > 64) return set_err(ctx, DMM_ERR_INVALID, "nsrc %d outside [1, 64]", nsrc);
+  for (int i = 0; i < nsrc; i++)
+    if (sources[i] < 0 || sources[i] >= ctx->n)
+      return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", sources[i], (long long)ctx->n);
+  if (ctx->nloc != ctx->n)
+    return set_err(ctx, DMM_ERR_STATE, "batched BFS needs the whole graph on the ctx (vbegin = 0, vend = N)");
+  if ((rc = ensure_batch_state(ctx))) return rc;
+  const double t0 = now_ms();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  rc = dmm::batch_run(ctx, sources, nsrc, depth, params, stats);
+  if (rc) return rc;
+  if (stats) stats->host_ms = now_ms() - t0;
+  return DMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PageRank
+// ---------------------------------------------------------------------------
+
+int dmm_pr_pull(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, void* scores,
+                dmm_pr_stats* stats) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (max_iters < 0 || max_iters > DMM_MAX_PR_ITERS)
+    return set_err(ctx, DMM_ERR_INVALID, "max_iters %d outside [0, %d]", max_iters, DMM_MAX_PR_ITERS);
+  if (mode < DMM_PR_FP32_SERIAL || mode > DMM_PR_FP64) return set_err(ctx, DMM_ERR_INVALID, "bad mode %d", mode);
+  if (ctx->nloc != ctx->n)
+    return set_err(ctx, DMM_ERR_STATE, "dmm_pr_pull needs the whole graph; use dmm_pr_block_iteration for a block");
+  if ((rc = ensure_pr_state(ctx, mode))) return rc;
+  const double t0 = now_ms();
+  dmm_pr_stats local;
+  dmm_pr_stats* st = stats ? stats : &local;
+  std::memset(st, 0, sizeof(*st));
+  if ((rc = dmm::pr_run(ctx, max_iters, epsilon, mode, st))) return rc;
+  if (scores) {
+    if (mode == DMM_PR_FP64)
+      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->d_score64, sizeof(double) * ctx->nloc, hipMemcpyDeviceToHost));
+    else
+      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->d_score, sizeof(float) * ctx->nloc, hipMemcpyDeviceToHost));
+  }
+  st->host_ms = now_ms() - t0;
+  return DMM_OK;
+}
+
+int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib, float* scores_inout, int32_t mode,
+                           double* err, double* ms) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!global_contrib || !scores_inout) return set_err(ctx, DMM_ERR_INVALID, "NULL buffer");
+  if (mode != DMM_PR_FP32_SERIAL && mode != DMM_PR_FP32_KAHAN)
+    return set_err(ctx, DMM_ERR_INVALID, "block iteration supports the FP32 modes");
+  if ((rc = ensure_pr_state(ctx, mode))) return rc;
+  return dmm::pr_block_iteration(ctx, global_contrib, scores_inout, mode, err, ms);
+}
+
+// ---------------------------------------------------------------------------
+// Tuning
+// ---------------------------------------------------------------------------
+
+int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
+  if (!ctx || !t) return set_err(ctx, DMM_ERR_INVALID, "NULL argument");
+  if (t->alpha <= 0 || t->beta <= 0 || t->batch.td_div <= 0 || t->batch.graph_levels <= 0)
+    return set_err(ctx, DMM_ERR_INVALID, "tuning values must be positive");
+  ctx->tuning.alpha = t->alpha;
+  ctx->tuning.beta = t->beta;
+  ctx->tuning.batch = t->batch;
+  if (t->grid_per_cu > 0) ctx->tuning.grid_per_cu = std::min(32, t->grid_per_cu);
+  // Graphs captured with the old shape are rebuilt on next use.
+  if (ctx->bfs_graph) (void)hipGraphExecDestroy(ctx->bfs_graph);
+  if (ctx->bfs_cont) (void)hipGraphExecDestroy(ctx->bfs_cont);
+  if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
+  if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
+  ctx->bfs_graph = ctx->bfs_cont = ctx->batch_graph = ctx->batch_cont = nullptr;
+  ctx->bfs_graph_steps = 0;
+  ctx->batch_graph_key = -1;
+  return DMM_OK;
+}
+
+int dmm_get_tuning(const dmm_ctx* ctx, dmm_tuning* t) {
+  if (!ctx || !t) return set_err(nullptr, DMM_ERR_INVALID, "NULL argument");
+  std::memset(t, 0, sizeof(*t));
+  t->alpha = ctx->tuning.alpha;
+  t->beta = ctx->tuning.beta;
+  t->grid_per_cu = ctx->tuning.grid_per_cu;
+  t->td_edges_per_block = ctx->tuning.td_edges_per_block;
+  t->batch = ctx->tuning.batch;
+  t->plain_ms = ctx->tuning.plain_ms;
+  t->batch_ms = ctx->tuning.batch_ms;
+  return DMM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Shared device-side definitions: the context, control block, error
+// plumbing and wave64 helpers for gfx950 (CDNA4: 64-lane wavefronts).
+#ifndef DMM_DEVICE_COMMON_H_
+#define DMM_DEVICE_COMMON_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/dmm_gapbs.h"
+
+namespace dmm {
+
+// --------------------------------------------------------------------------
+// Control block shared by the BFS kernels (one per ctx, in HBM).
+// --------------------------------------------------------------------------
+constexpr int kSlots = 32;          // counters spread over slots (contention)
+constexpr int kCountBits = 30;      // packed reservation: count | degsum << 30
+constexpr uint64_t kCountMask = (uint64_t(1) << kCountBits) - 1;
+
+struct BfsCtrl {
+  // TD / BitmapToQueue output reservation: (degree sum << 30) | count.
+  unsigned long long packed;
+  // Claimed vertices of out-degree 0 (counted in |F| and scout, not queued).
+  unsigned long long zero_deg;
+  unsigned long long awake[kSlots];
+  unsigned long long edges[kSlots];      // edges examined (stats)
+  unsigned long long scanned[kSlots];    // vertices scanned (stats)
+  // Device-resident DOBFS loop state.
+  long long qb, qe, qin;                 // sliding-queue window and tail
+  long long window_edges;                // sum of out-degrees in the window
+  long long window_zero;                 // zero-degree members of the window
+  long long scout, edges_to_check, awake_count, old_awake;
+  int level;
+  int phase;                             // kPhase* below
+  int done;
+  int step;                              // steps taken (trace index)
+  int front;                             // index of the front bitmap (0/1)
+  int pad0;
+  // Per-step trace written by the device loop.
+  int step_kind[DMM_MAX_STEPS];
+  long long step_out[DMM_MAX_STEPS];
+  long long step_in[DMM_MAX_STEPS];
+  long long step_edges[DMM_MAX_STEPS];
+  long long step_scout[DMM_MAX_STEPS];
+  // Block-level partial sums for BitmapToQueue.
+  unsigned int ticket;
+  unsigned int pad1;
+};
+
+enum : int {
+  kPhaseDecide = 0,  // loop head: choose TD or BU
+  kPhaseTD = 1,      // run a top-down step
+  kPhaseBU = 2,      // run a bottom-up step
+  kPhaseB2Q = 3,     // convert the BU frontier back to a queue
+  kPhaseDone = 4,
+};
+
+struct PrCtrl {
+  double err;          // L1 change of the last iteration
+  unsigned int ticket; // iterations run
+  unsigned int pad;    // != 0 once converged (remaining iterations no-op)
+};
+
+struct Tuning {
+  int alpha = 15, beta = 18;           // src/bfs.cc:142
+  int bu_block = 256;
+  int td_edges_per_block = 1024;
+  int grid_per_cu = 8;                 // workgroups per CU for grid-stride steps
+  dmm_batch_params batch = {20, 256, 8, 0};
+  double plain_ms = 0, batch_ms = 0;
+};
+
+}  // namespace dmm
+
+struct dmm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  std::vector<hipEvent_t> step_events;  // per-step timing in trace mode
+  std::string err;
+  std::string arch;
+  int cu_count = 0;
+  int64_t dev_bytes = 0;    // device memory held
+  int64_t fixed_bytes = 0;  // ... of it by the ctx itself (not the graph)
+
+  // Graph (one PE's block).
+  int64_t n = 0, vbegin = 0, vend = 0, nloc = 0, words = 0;
+  int64_t m_out = 0, m_in = 0, ned = 0;
+  bool directed = false, has_graph = false;
+  int64_t* d_out_off = nullptr;
+  int32_t* d_out_nbr = nullptr;
+  int64_t* d_in_off = nullptr;   // == d_out_off when undirected
+  int32_t* d_in_nbr = nullptr;
+
+  // BFS state.
+  int32_t* d_parent = nullptr;   // nloc
+  int32_t* d_depth = nullptr;    // nloc
+  int32_t* d_queue = nullptr;    // nloc + 1 (sliding queue)
+  int64_t* d_qoff = nullptr;     // nloc + 1 (window-relative degree prefix)
+  uint64_t* d_bm[3] = {nullptr, nullptr, nullptr};  // words each
+  int64_t beta_width = 0;        // Partition::max_width of this PE
+  void* d_params = nullptr;      // BfsParams + num_edges_directed
+  void* h_params = nullptr;      // pinned staging for d_params
+  int64_t* d_scratch64 = nullptr;  // small reduction outputs
+  int64_t* d_blk = nullptr;      // BitmapToQueue block partials
+  int64_t blk_cap = 0;
+  dmm::BfsCtrl* d_ctrl = nullptr;
+  dmm::BfsCtrl* h_ctrl = nullptr;  // pinned mirror
+  // Host mirror of the sliding queue indices for the step API.
+  int64_t qb = 0, qe = 0, qin = 0, q_edges = 0, q_zero = 0;
+  int level = 0, front = 0;
+  bool bfs_valid = false;
+  hipGraphExec_t bfs_graph = nullptr;  // init + bfs_chain_len steps + finish
+  hipGraphExec_t bfs_cont = nullptr;   // 4 more steps + finish
+  int bfs_graph_steps = 0;
+  int bfs_chain_len = 16;
+
+  // PageRank state.
+  float* d_score = nullptr;
+  float* d_contrib[2] = {nullptr, nullptr};
+  double* d_score64 = nullptr;
+  double* d_contrib64[2] = {nullptr, nullptr};
+  double* d_err_part = nullptr;
+  int64_t err_part_cap = 0;
+  dmm::PrCtrl* d_prctrl = nullptr;
+  dmm::PrCtrl* h_prctrl = nullptr;
+  int32_t* d_heavy = nullptr;    // vertices with in-degree >= heavy threshold
+  int64_t n_heavy = 0;
+  int heavy_threshold = 64;
+  float* d_gcontrib = nullptr;   // PE-share: global contribution vector
+  double* d_err_trace = nullptr; // per-iteration L1 change
+  void* d_prparams = nullptr;
+  void* h_prparams = nullptr;
+  hipGraphExec_t pr_graph = nullptr;
+  int pr_graph_key = -1;
+  int pr_mode = -1;
+
+  // Batched BFS state.
+  uint64_t* d_seen = nullptr;
+  uint64_t* d_levels = nullptr;  // level_cap x nloc frontier words
+  int64_t level_cap = 0;
+  int32_t* d_bact = nullptr;     // 2 x (nloc + 64) active lists
+  int64_t* d_baoff = nullptr;    // 2 x (nloc + 64) degree prefixes
+  void* d_bctrl = nullptr;
+  void* h_bctrl = nullptr;
+  void* d_bsrc = nullptr;        // 64 sources
+  void* h_bsrc = nullptr;
+  uint8_t* d_bdepth = nullptr;
+  int64_t bdepth_cap = 0;
+  hipGraphExec_t batch_graph = nullptr;
+  hipGraphExec_t batch_cont = nullptr;
+  int batch_graph_key = -1;
+
+  dmm::Tuning tuning;
+};
+
+namespace dmm {
+
+int set_err(dmm_ctx* ctx, int code, const char* fmt, ...);
+
+#define DMM_HIP_CHECK(ctx, call)                                                  \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::dmm::set_err((ctx), DMM_ERR_HIP, "%s failed: %s (%s:%d)", #call,   \
+                            hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+// Device allocation that records failures as DMM_ERR_NOMEM.
+int dev_alloc(dmm_ctx* ctx, void** p, size_t bytes, const char* what);
+void dev_free(void* p);
+
+// ---- wave64 helpers -------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Exclusive rank of the calling lane among lanes whose bit is set in mask.
+__device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Block-wide exclusive scan for blockDim.x == kThreads; returns the
+// exclusive prefix and writes the block total into *total.  `scratch` must
+// hold kThreads / 64 + 1 elements of T in LDS.
+template <int kThreads, typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T* scratch, T* total) {
+  constexpr int kWaves = kThreads / 64;
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  T inc = wave_incl_scan(x);
+  if (lane == 63) scratch[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (int w = 0; w < kWaves; w++) {
+      T t = scratch[w];
+      scratch[w] = run;
+      run += t;
+    }
+    scratch[kWaves] = run;
+  }
+  __syncthreads();
+  T res = scratch[wave] + inc - x;
+  *total = scratch[kWaves];
+  __syncthreads();
+  return res;
+}
+
+template <int kThreads, typename T>
+__device__ __forceinline__ T block_sum(T x, T* scratch) {
+  constexpr int kWaves = kThreads / 64;
+  T s = wave_sum(x);
+  if (lane_id() == 0) scratch[threadIdx.x >> 6] = s;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kWaves; w++) r += scratch[w];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Fill kernels (used instead of hipMemsetAsync so the sequences can be
+// captured into hipGraphs).
+void launch_fill_u64(hipStream_t s, uint64_t* p, int64_t n, uint64_t v);
+void launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+
+}  // namespace dmm
+
+#endif  // DMM_DEVICE_COMMON_H_

@@ -1,0 +1,519 @@
+// Pull-direction PageRank on gfx950.
+//
+// Reference: PageRankPull, src/pr.cc:41-70 (oracle gapbs-verify/pr.cc:34-59):
+//   contrib[v] = score[v] / out_degree(v)
+//   score[u]   = (1 - d)/N + d * sum_{v in in(u)} contrib[v]      (d = 0.85f)
+//   error      = sum_u |score_new[u] - score_old[u]|  (double), stop < eps.
+// Data in HBM: in-CSR (int64 offsets, int32 IDs), score[nloc], contrib in two
+// buffers (the iteration reads one and writes the next), per-block error
+// partials and a small control block.  Each iteration is one pass over the
+// CSR: the kernel that finishes vertex u's sum also writes its new score and
+// its next contribution, so an iteration moves 8M + 20N bytes (FP32).
+//
+// Three arithmetic modes:
+//   FP32_SERIAL  the reference's arithmetic exactly: FP32 sums in ascending
+//                in-neighbour order (thread per vertex; one wave per heavy
+//                vertex folding 64 gathered values in lane order).
+//   FP32_KAHAN   FP32 storage and gathers, each sum compensated (Knuth's
+//                TwoSum error-free transformation, TAOCP Vol. 2 4.2.2;
+//                partial (sum, error) pairs merged with TwoSum as in
+//                Ogita-Rump-Oishi Sum2).  Result within ~1 ulp of the exact
+//                sum plus O(n u^2) -- far below the FP32 score rounding.
+//   FP64         FP64 scores, contributions and sums.
+// The L1 error is accumulated in FP64 per block and reduced in a fixed
+// order, so every mode is run-to-run deterministic.
+#include "pr.h"
+
+#include <algorithm>
+
+namespace dmm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kGroup = 8;         // lanes per light vertex (KAHAN / FP64)
+
+struct PrParams {
+  double eps;
+  float base_f, damp_f;           // (1 - 0.85f) / N, 0.85f
+  double base_d, damp_d;          // widened FP32 constants for FP64 mode
+  float init_f;
+  float pad;
+  double init_d;
+};
+
+struct PrArgs {
+  const int64_t* in_off;          // local rows
+  const int32_t* in_nbr;          // global IDs
+  const int64_t* out_off;         // local rows (out-degree for contrib)
+  const int32_t* heavy;           // local rows with in-degree >= kHeavy
+  int64_t nloc, n_heavy;
+  int heavy_threshold;
+  double* err_part;               // [light blocks | heavy blocks]
+  int light_blocks, heavy_blocks;
+  PrCtrl* ctrl;
+  double* err_trace;              // per-iteration error
+  const PrParams* prm;
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+struct Comp {
+  float s, c;
+  __device__ __forceinline__ void add(float x) {
+    float t, e;
+    two_sum(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void merge(float s2, float c2) {
+    float t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c = (c + c2) + e;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+__device__ __forceinline__ double block_err_sum(double x, double* red) {
+  double s = wave_sum(x);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; w++) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int64_t out_deg(const PrArgs& a, int64_t lu) {
+  return a.out_off[lu + 1] - a.out_off[lu];
+}
+
+// Finishes vertex lu given its incoming sum: new score, next contribution
+// and the |delta| term (src/pr.cc:61-63, 52-53).
+__device__ __forceinline__ double finish_f32(const PrArgs& a, int64_t lu, float incoming,
+                                             float* score, float* contrib_out) {
+  const float old_score = score[lu];
+  const float nw = a.prm->base_f + a.prm->damp_f * incoming;
+  score[lu] = nw;
+  contrib_out[lu] = nw / static_cast<float>(out_deg(a, lu));
+  return static_cast<double>(fabsf(nw - old_score));
+}
+
+__device__ __forceinline__ double finish_f64(const PrArgs& a, int64_t lu, double incoming,
+                                             double* score, double* contrib_out) {
+  const double old_score = score[lu];
+  const double nw = a.prm->base_d + a.prm->damp_d * incoming;
+  score[lu] = nw;
+  contrib_out[lu] = nw / static_cast<double>(out_deg(a, lu));
+  return fabs(nw - old_score);
+}
+
+// ---- FP32_SERIAL ----------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_serial_light(PrArgs a, const float* __restrict__ cin,
+                                                           float* __restrict__ cout,
+                                                           float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;  // converged
+  double err = 0;
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+    if (e - s >= a.heavy_threshold) continue;
+    float incoming_total = 0;
+    for (int64_t k = s; k < e; k++) incoming_total += cin[a.in_nbr[k]];
+    err += finish_f32(a, u, incoming_total, score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_serial_heavy(PrArgs a, const float* __restrict__ cin,
+                                                           float* __restrict__ cout,
+                                                           float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  double err = 0;
+  for (int64_t h = wave; h < a.n_heavy; h += nwaves) {
+    const int64_t u = a.heavy[h];
+    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+    float incoming_total = 0;  // wave-uniform running sum, ascending order
+    for (int64_t k = s; k < e; k += 64) {
+      const int64_t kk = k + lane;
+      const float x = kk < e ? cin[a.in_nbr[kk]] : 0.0f;
+      const int cnt = static_cast<int>(min<int64_t>(64, e - k));
+      for (int i = 0; i < cnt; i++) incoming_total += __shfl(x, i, 64);
+    }
+    if (lane == 0) err += finish_f32(a, u, incoming_total, score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+}
+
+// ---- FP32_KAHAN -------------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_kahan_light(PrArgs a, const float* __restrict__ cin,
+                                                          float* __restrict__ cout,
+                                                          float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
+  double err = 0;
+  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t u = grp + it * ngrp;
+    Comp acc{0.0f, 0.0f};
+    bool mine = false;
+    if (u < a.nloc) {
+      const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+      if (e - s < a.heavy_threshold) {
+        mine = true;
+        for (int64_t k = s + sub; k < e; k += kGroup) acc.add(cin[a.in_nbr[k]]);
+      }
+    }
+    // Merge the group's partials (fixed butterfly order).
+#pragma unroll
+    for (int d = kGroup / 2; d >= 1; d >>= 1) {
+      const float s2 = __shfl_xor(acc.s, d, kGroup);
+      const float c2 = __shfl_xor(acc.c, d, kGroup);
+      // Order the operands by lane so both partners compute the same pair.
+      if (sub & d) {
+        Comp o{s2, c2};
+        o.merge(acc.s, acc.c);
+        acc = o;
+      } else {
+        acc.merge(s2, c2);
+      }
+    }
+    if (mine && sub == 0) err += finish_f32(a, u, acc.value(), score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_kahan_heavy(PrArgs a, const float* __restrict__ cin,
+                                                          float* __restrict__ cout,
+                                                          float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  double err = 0;
+  for (int64_t h = wave; h < a.n_heavy; h += nwaves) {
+    const int64_t u = a.heavy[h];
+    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+    Comp acc{0.0f, 0.0f};
+    int64_t k = s + lane;
+    for (; k + 192 < e; k += 256) {
+      const float x0 = cin[a.in_nbr[k]];
+      const float x1 = cin[a.in_nbr[k + 64]];
+      const float x2 = cin[a.in_nbr[k + 128]];
+      const float x3 = cin[a.in_nbr[k + 192]];
+      acc.add(x0);
+      acc.add(x1);
+      acc.add(x2);
+      acc.add(x3);
+    }
+    for (; k < e; k += 64) acc.add(cin[a.in_nbr[k]]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const float s2 = __shfl_xor(acc.s, d, 64);
+      const float c2 = __shfl_xor(acc.c, d, 64);
+      if (lane & d) {
+        Comp o{s2, c2};
+        o.merge(acc.s, acc.c);
+        acc = o;
+      } else {
+        acc.merge(s2, c2);
+      }
+    }
+    if (lane == 0) err += finish_f32(a, u, acc.value(), score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+}
+
+// ---- FP64 -----------------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_f64_light(PrArgs a, const double* __restrict__ cin,
+                                                        double* __restrict__ cout,
+                                                        double* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
+  double err = 0;
+  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t u = grp + it * ngrp;
+    double acc = 0;
+    bool mine = false;
+    if (u < a.nloc) {
+      const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+      if (e - s < a.heavy_threshold) {
+        mine = true;
+        for (int64_t k = s + sub; k < e; k += kGroup) acc += cin[a.in_nbr[k]];
+      }
+    }
+#pragma unroll
+    for (int d = kGroup / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kGroup);
+    if (mine && sub == 0) err += finish_f64(a, u, acc, score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_f64_heavy(PrArgs a, const double* __restrict__ cin,
+                                                        double* __restrict__ cout,
+                                                        double* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  double err = 0;
+  for (int64_t h = wave; h < a.n_heavy; h += nwaves) {
+    const int64_t u = a.heavy[h];
+    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+    double acc = 0;
+    for (int64_t k = s + lane; k < e; k += 64) acc += cin[a.in_nbr[k]];
+    acc = wave_sum(acc);
+    if (lane == 0) err += finish_f64(a, u, acc, score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+}
+
+// ---- iteration bookkeeping --------------------------------------------------
+
+// Sums the error partials in a fixed order; records the error and the stop
+// decision (src/pr.cc:66-68).  ctrl->pad != 0 marks convergence and turns
+// the remaining captured iterations into no-ops; ctrl->ticket counts
+// iterations run.
+__global__ __launch_bounds__(kThreads) void k_reduce(PrArgs a) {
+  __shared__ double red[kThreads / 64];
+  PrCtrl* c = a.ctrl;
+  if (c->pad) return;
+  const int np = a.light_blocks + a.heavy_blocks;
+  double x = 0;
+  for (int i = threadIdx.x; i < np; i += kThreads) x += a.err_part[i];
+  double t = block_err_sum(x, red);
+  if (threadIdx.x == 0) {
+    const unsigned it = c->ticket;
+    c->err = t;
+    if (it < DMM_MAX_PR_ITERS) a.err_trace[it] = t;
+    c->ticket = it + 1;
+    if (t < a.prm->eps) c->pad = 1;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_init_f32(PrArgs a, float* score, float* contrib) {
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  const float init = a.prm->init_f;
+  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+    score[u] = init;
+    contrib[u] = init / static_cast<float>(out_deg(a, u));  // src/pr.cc:52-53
+  }
+  if (gtid == 0) {
+    a.ctrl->pad = 0;
+    a.ctrl->ticket = 0;
+    a.ctrl->err = 0;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_init_f64(PrArgs a, double* score, double* contrib) {
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  const double init = a.prm->init_d;
+  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+    score[u] = init;
+    contrib[u] = init / static_cast<double>(out_deg(a, u));
+  }
+  if (gtid == 0) {
+    a.ctrl->pad = 0;
+    a.ctrl->ticket = 0;
+    a.ctrl->err = 0;
+  }
+}
+
+__global__ void k_reset_ctrl(PrCtrl* c) {
+  c->pad = 0;
+  c->ticket = 0;
+  c->err = 0;
+}
+
+PrArgs make_args(dmm_ctx* ctx) {
+  PrArgs a;
+  a.in_off = ctx->d_in_off;
+  a.in_nbr = ctx->d_in_nbr;
+  a.out_off = ctx->d_out_off;
+  a.heavy = ctx->d_heavy;
+  a.nloc = ctx->nloc;
+  a.n_heavy = ctx->n_heavy;
+  a.heavy_threshold = ctx->heavy_threshold;
+  a.err_part = ctx->d_err_part;
+  a.light_blocks = pr_light_blocks(ctx);
+  a.heavy_blocks = pr_heavy_blocks(ctx);
+  a.ctrl = ctx->d_prctrl;
+  a.err_trace = ctx->d_err_trace;
+  a.prm = reinterpret_cast<const PrParams*>(ctx->d_prparams);
+  return a;
+}
+
+void launch_iteration(dmm_ctx* ctx, const PrArgs& a, int mode, int it, const void* gcontrib) {
+  const int lb = a.light_blocks, hb = a.heavy_blocks;
+  hipStream_t s = ctx->stream;
+  if (mode == DMM_PR_FP64) {
+    const double* cin = gcontrib ? static_cast<const double*>(gcontrib) : ctx->d_contrib64[it & 1];
+    double* cout = ctx->d_contrib64[(it + 1) & 1];
+    hipLaunchKernelGGL(k_f64_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
+    hipLaunchKernelGGL(k_f64_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
+  } else {
+    const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
+    float* cout = ctx->d_contrib[(it + 1) & 1];
+    if (mode == DMM_PR_FP32_SERIAL) {
+      hipLaunchKernelGGL(k_serial_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_serial_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+    } else {
+      hipLaunchKernelGGL(k_kahan_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_kahan_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+    }
+  }
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, s, a);
+}
+
+}  // namespace
+
+int pr_light_blocks(dmm_ctx* ctx) {
+  return static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8, ceil_div(ctx->nloc, kThreads))));
+}
+
+int pr_heavy_blocks(dmm_ctx* ctx) {
+  return static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8,
+                           ceil_div(ctx->n_heavy * 64, kThreads))));
+}
+
+int pr_run(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, dmm_pr_stats* stats) {
+  // Constants exactly as the reference computes them in float
+  // (src/pr.cc:39, 45-46), widened for the FP64 mode.
+  PrParams* p = reinterpret_cast<PrParams*>(ctx->h_prparams);
+  const float kDamp = 0.85f;
+  const float nf = static_cast<float>(ctx->n);
+  p->eps = epsilon;
+  p->damp_f = kDamp;
+  p->base_f = (1.0f - kDamp) / nf;
+  p->init_f = 1.0f / nf;
+  p->damp_d = static_cast<double>(kDamp);
+  p->base_d = static_cast<double>(p->base_f);
+  p->init_d = static_cast<double>(p->init_f);
+  p->pad = 0;
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_prparams, ctx->h_prparams, sizeof(PrParams),
+                                    hipMemcpyHostToDevice, ctx->stream));
+  // One captured graph per (mode, max_iters): init + max_iters iterations;
+  // iterations after convergence return at once.
+  const int key = mode * 100000 + max_iters;
+  if (!ctx->pr_graph || ctx->pr_graph_key != key) {
+    if (ctx->pr_graph) (void)hipGraphExecDestroy(ctx->pr_graph);
+    ctx->pr_graph = nullptr;
+    PrArgs a = make_args(ctx);
+    hipGraph_t g = nullptr;
+    DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int ib = pr_light_blocks(ctx);
+    if (mode == DMM_PR_FP64)
+      hipLaunchKernelGGL(k_init_f64, dim3(ib), dim3(kThreads), 0, ctx->stream, a, ctx->d_score64,
+                         ctx->d_contrib64[0]);
+    else
+      hipLaunchKernelGGL(k_init_f32, dim3(ib), dim3(kThreads), 0, ctx->stream, a, ctx->d_score,
+                         ctx->d_contrib[0]);
+    for (int it = 0; it < max_iters; it++) launch_iteration(ctx, a, mode, it, nullptr);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "PR capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->pr_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "PR instantiate failed: %s", hipGetErrorString(e));
+    ctx->pr_graph_key = key;
+  }
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->pr_graph, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_prctrl, ctx->d_prctrl, sizeof(PrCtrl), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->pr_mode = mode;
+  if (stats) {
+    float ms = 0;
+    DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+    const int iters = static_cast<int>(ctx->h_prctrl->ticket);
+    stats->iters = iters;
+    stats->total_ms = ms;
+    stats->iter_ms = iters > 0 ? ms / iters : 0;
+    const int nt = std::min(iters, DMM_MAX_PR_ITERS);
+    if (nt > 0)
+      DMM_HIP_CHECK(ctx, hipMemcpy(stats->err, ctx->d_err_trace, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    const int64_t m = ctx->m_in, nl = ctx->nloc;
+    stats->alg_bytes_per_iter = (mode == DMM_PR_FP64) ? 12 * m + 32 * nl + 8 : 8 * m + 20 * nl + 8;
+  }
+  return DMM_OK;
+}
+
+int pr_block_iteration(dmm_ctx* ctx, const float* gcontrib_host, float* scores_inout, int32_t mode,
+                       double* err, double* ms) {
+  if (!ctx->d_gcontrib) {
+    int rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_gcontrib), sizeof(float) * ctx->n,
+                       "global contribution vector");
+    if (rc) return rc;
+  }
+  PrParams* p = reinterpret_cast<PrParams*>(ctx->h_prparams);
+  const float kDamp = 0.85f;
+  const float nf = static_cast<float>(ctx->n);
+  p->eps = -1.0;
+  p->damp_f = kDamp;
+  p->base_f = (1.0f - kDamp) / nf;
+  p->init_f = 1.0f / nf;
+  p->damp_d = static_cast<double>(kDamp);
+  p->base_d = static_cast<double>(p->base_f);
+  p->init_d = static_cast<double>(p->init_f);
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_prparams, ctx->h_prparams, sizeof(PrParams),
+                                    hipMemcpyHostToDevice, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_gcontrib, gcontrib_host, sizeof(float) * ctx->n,
+                                    hipMemcpyHostToDevice, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_score, scores_inout, sizeof(float) * ctx->nloc,
+                                    hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_reset_ctrl, dim3(1), dim3(1), 0, ctx->stream, ctx->d_prctrl);
+  PrArgs a = make_args(ctx);
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+  launch_iteration(ctx, a, mode, 0, ctx->d_gcontrib);
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(scores_inout, ctx->d_score, sizeof(float) * ctx->nloc,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_prctrl, ctx->d_prctrl, sizeof(PrCtrl), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (err) *err = ctx->h_prctrl->err;
+  if (ms) {
+    float f = 0;
+    DMM_HIP_CHECK(ctx, hipEventElapsedTime(&f, ctx->ev_a, ctx->ev_b));
+    *ms = f;
+  }
+  return DMM_OK;
+}
+
+}  // namespace dmm

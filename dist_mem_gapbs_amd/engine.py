@@ -1,0 +1,228 @@
+"""Device engine: one GPU as one PE of the vertex-block layout.
+
+``Context`` wraps a dmm_ctx (include/dmm_gapbs.h).  Its methods mirror the
+reference's hot-path entry points:
+
+  DOBFS(source, alpha=15, beta=18)      src/bfs.cc:142-188
+  step_* (TDStep, BUStep, QueueToBitmap, BitmapToQueue, InitParent)
+                                        src/bfs.cc:57-140
+  PageRankPull(max_iters, epsilon)      src/pr.cc:41-70
+  bfs_batch64(sources)                  64 BFS lanes in one pass
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def device_count():
+    n = C.c_int32()
+    L.check(L.load().dmm_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    def __init__(self, device=0):
+        self._lib = L.load()
+        h = C.c_void_p()
+        L.check(self._lib.dmm_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.n = 0
+        self.vbegin = 0
+        self.vend = 0
+
+    # ---- lifetime -----------------------------------------------------------
+    def close(self):
+        if self._h:
+            self._lib.dmm_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        return L.check(rc, self._h)
+
+    @property
+    def nloc(self):
+        return self.vend - self.vbegin
+
+    # ---- graph --------------------------------------------------------------
+    def upload(self, g=None, *, num_nodes=None, out_offsets=None, out_neighs=None,
+               in_offsets=None, in_neighs=None, vbegin=0, vend=None, num_edges_directed=-1):
+        """Uploads a HostGraph (whole graph) or explicit block arrays."""
+        if g is not None:
+            num_nodes = g.num_nodes
+            out_offsets, out_neighs = g.out_offsets, g.out_neighs
+            if g.directed:
+                in_offsets, in_neighs = g.in_offsets, g.in_neighs
+            num_edges_directed = g.num_edges_directed
+        if vend is None:
+            vend = num_nodes
+        oo = L.i64(out_offsets)
+        on = L.i32(out_neighs)
+        io = L.i64(in_offsets) if in_offsets is not None else None
+        inn = L.i32(in_neighs) if in_neighs is not None else None
+        self._check(self._lib.dmm_graph_upload(
+            self._h, num_nodes, vbegin, vend, L.ptr(oo, C.c_int64), L.ptr(on, C.c_int32),
+            L.ptr(io, C.c_int64), L.ptr(inn, C.c_int32), num_edges_directed))
+        self.n, self.vbegin, self.vend = num_nodes, vbegin, vend
+        self.words = (num_nodes + 63) // 64
+
+    def device_bytes(self):
+        b = C.c_int64()
+        self._check(self._lib.dmm_ctx_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    # ---- DOBFS -----------------------------------------------------------------
+    def DOBFS(self, source, alpha=15, beta=18, trace=False, want_depth=True, read=True):
+        """Returns (parent, depth, stats).  parent/depth are None if read=False
+        (the result stays in HBM)."""
+        parent = np.empty(self.nloc, np.int32) if read else None
+        depth = np.empty(self.nloc, np.int32) if (read and want_depth) else None
+        st = L.BfsStats()
+        self._check(self._lib.dmm_bfs(self._h, int(source), alpha, beta, L.ptr(parent, C.c_int32),
+                                      L.ptr(depth, C.c_int32), 1 if trace else 0, C.byref(st)))
+        return parent, depth, st
+
+    def bfs_read(self, parent=True, depth=True):
+        p = np.empty(self.nloc, np.int32) if parent else None
+        d = np.empty(self.nloc, np.int32) if depth else None
+        self._check(self._lib.dmm_bfs_read(self._h, L.ptr(p, C.c_int32), L.ptr(d, C.c_int32)))
+        return p, d
+
+    # ---- single steps (host-driven mirror of the reference loop) --------------
+    def step_init(self, source):
+        self._check(self._lib.dmm_step_init(self._h, int(source)))
+
+    def step_td(self):
+        s, q = C.c_int64(), C.c_int64()
+        self._check(self._lib.dmm_step_td(self._h, C.byref(s), C.byref(q)))
+        return s.value, q.value
+
+    def step_queue_to_bitmap(self):
+        a = C.c_int64()
+        self._check(self._lib.dmm_step_queue_to_bitmap(self._h, C.byref(a)))
+        return a.value
+
+    def step_bu(self):
+        a = C.c_int64()
+        self._check(self._lib.dmm_step_bu(self._h, C.byref(a)))
+        return a.value
+
+    def step_bitmap_to_queue(self):
+        q = C.c_int64()
+        self._check(self._lib.dmm_step_bitmap_to_queue(self._h, C.byref(q)))
+        return q.value
+
+    def step_finish(self):
+        self._check(self._lib.dmm_step_finish(self._h))
+
+    def get_parent(self):
+        p = np.empty(self.nloc, np.int32)
+        self._check(self._lib.dmm_state_get_parent(self._h, L.ptr(p, C.c_int32)))
+        return p
+
+    def set_parent(self, p):
+        p = L.i32(p)
+        self._check(self._lib.dmm_state_set_parent(self._h, L.ptr(p, C.c_int32)))
+
+    def get_queue(self):
+        size = C.c_int64()
+        cap = self.nloc + 64
+        q = np.empty(cap, np.int32)
+        self._check(self._lib.dmm_state_get_queue(self._h, L.ptr(q, C.c_int32), cap, C.byref(size)))
+        return q[:size.value].copy()
+
+    def set_queue(self, q):
+        q = L.i32(q)
+        self._check(self._lib.dmm_state_set_queue(self._h, L.ptr(q, C.c_int32), len(q)))
+
+    def get_bitmap(self, which=0):
+        w = np.empty(self.words, np.uint64)
+        self._check(self._lib.dmm_state_get_bitmap(self._h, which, L.ptr(w, C.c_uint64)))
+        return w
+
+    def set_bitmap(self, which, words):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        self._check(self._lib.dmm_state_set_bitmap(self._h, which, L.ptr(w, C.c_uint64)))
+
+    def set_level(self, level):
+        self._check(self._lib.dmm_state_set_level(self._h, level))
+
+    # ---- batched BFS ---------------------------------------------------------------
+    def bfs_batch64(self, sources, want_depth=True, td_div=None, graph_levels=None):
+        """Returns (depth[nsrc, nloc] uint8 with 255 = unreached, stats)."""
+        src = L.i32(sources)
+        ns = len(src)
+        depth = np.empty((ns, self.nloc), np.uint8) if want_depth else None
+        params = None
+        if td_div is not None or graph_levels is not None:
+            t = self.get_tuning()
+            params = L.BatchParams(td_div if td_div is not None else t.batch.td_div,
+                                   t.batch.block,
+                                   graph_levels if graph_levels is not None else t.batch.graph_levels,
+                                   0)
+        st = L.BatchStats()
+        self._check(self._lib.dmm_bfs_batch64(self._h, L.ptr(src, C.c_int32), ns,
+                                              L.ptr(depth, C.c_uint8),
+                                              C.byref(params) if params is not None else None,
+                                              C.byref(st)))
+        return depth, st
+
+    # ---- PageRank ----------------------------------------------------------------------
+    def PageRankPull(self, max_iters=20, epsilon=0.0, mode=L.DMM_PR_FP32_SERIAL, read=True):
+        """Returns (scores, stats); scores float32 (FP32 modes) or float64."""
+        dt = np.float64 if mode == L.DMM_PR_FP64 else np.float32
+        scores = np.empty(self.nloc, dt) if read else None
+        st = L.PrStats()
+        self._check(self._lib.dmm_pr_pull(self._h, max_iters, float(epsilon), mode,
+                                          scores.ctypes.data_as(C.c_void_p) if read else None,
+                                          C.byref(st)))
+        return scores, st
+
+    def pr_block_iteration(self, global_contrib, scores, mode=L.DMM_PR_FP32_SERIAL):
+        gc = np.ascontiguousarray(global_contrib, dtype=np.float32)
+        sc = np.ascontiguousarray(scores, dtype=np.float32).copy()
+        err, ms = C.c_double(), C.c_double()
+        self._check(self._lib.dmm_pr_block_iteration(self._h, L.ptr(gc, C.c_float), L.ptr(sc, C.c_float),
+                                                     mode, C.byref(err), C.byref(ms)))
+        return sc, err.value, ms.value
+
+    # ---- tuning --------------------------------------------------------------------------
+    def get_tuning(self):
+        t = L.Tuning()
+        self._check(self._lib.dmm_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t):
+        self._check(self._lib.dmm_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, sources, budget=1):
+        src = L.i32(sources)
+        t = L.Tuning()
+        self._check(self._lib.dmm_autotune(self._h, L.ptr(src, C.c_int32), len(src), budget, C.byref(t)))
+        return t
+
+
+def stats_dict(st):
+    """Plain-dict view of a BfsStats (trace arrays cut to num_steps)."""
+    n = st.num_steps
+    names = {L.DMM_STEP_TD: "td", L.DMM_STEP_BU: "bu", L.DMM_STEP_QB: "e", L.DMM_STEP_BQ: "c"}
+    return {
+        "steps": [(names.get(st.step_kind[i], "?"), st.step_in[i], st.step_out[i], st.step_edges[i],
+                   st.step_scout[i], st.step_ms[i]) for i in range(n)],
+        "alg_bytes": st.alg_bytes, "bu_alg_bytes": st.bu_alg_bytes, "reached": st.reached,
+        "reached_edges": st.reached_edges, "total_ms": st.total_ms, "bu_ms": st.bu_ms,
+        "host_ms": st.host_ms,
+    }

@@ -1,0 +1,310 @@
+/*
+ * dmm_gapbs.h -- C ABI of the MI355X edge-traversal engine (direction-optimizing
+ * BFS and pull PageRank) for DMM-GAPBS-shaped host code.
+ *
+ * One GPU plays one processing element (PE) of the reference's vertex-block
+ * partition (src/partition.h:69-161).  The OpenSHMEM symmetric-heap arrays the
+ * reference exchanges per step (frontier bitmap, queue, parent, contributions)
+ * are arrays in HBM owned by a dmm_ctx.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: DMM_OK (0) or a negative
+ *     DMM_ERR_* code.  The message of the last failure is returned by
+ *     dmm_last_error(ctx) (or dmm_last_error(NULL) for calls without a ctx).
+ *     Nothing here calls exit() or abort().
+ *   - Host buffers are caller-owned.  Inputs are copied in; outputs are copied
+ *     out before the call returns.  No pointer is retained after a call.
+ *   - A dmm_ctx owns all device memory and one HIP stream.  Calls on one ctx
+ *     are synchronous with respect to host buffers and must not be made from
+ *     two threads at once.  Distinct ctxs are independent.
+ *   - Vertex IDs are int32 (src/benchmark.h:320 NodeID), CSR offsets int64
+ *     (gapbs-verify/graph.h:86 SGOffset).
+ */
+#ifndef DMM_GAPBS_H_
+#define DMM_GAPBS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define DMM_OK 0
+#define DMM_ERR_INVALID (-1)   /* bad argument (null pointer, bad source, ...) */
+#define DMM_ERR_NOMEM (-2)     /* host or device allocation failed             */
+#define DMM_ERR_HIP (-3)       /* HIP runtime error                            */
+#define DMM_ERR_STATE (-4)     /* call out of order (e.g. no graph uploaded)   */
+#define DMM_ERR_CAPACITY (-5)  /* a queue / level table would overflow         */
+#define DMM_ERR_NODEVICE (-6)  /* no usable GPU                                */
+#define DMM_ERR_GRAPH (-7)     /* malformed CSR (unsorted, out-of-range id)    */
+
+#define DMM_MAX_STEPS 256      /* per-BFS step trace capacity                  */
+#define DMM_MAX_PR_ITERS 1024  /* per-PageRank iteration trace capacity        */
+
+/* Last error message for ctx (ctx may be NULL for context-free calls). */
+const char* dmm_last_error(const struct dmm_ctx* ctx);
+
+/* Library version string. */
+const char* dmm_version(void);
+
+/* ======================================================================== */
+/* Host graph substrate: Generator / Builder / SourcePicker / Partition      */
+/* ======================================================================== */
+
+typedef struct dmm_host_graph dmm_host_graph;
+
+/* Replaces Generator::GenerateEL (src/generator.h:169-180; oracle
+ * gapbs-verify/generator.h:119-130): the synthetic edge list of 2^scale
+ * vertices and 2^scale*degree pairs, block-seeded mt19937 (kRandSeed =
+ * 27491095, src/util.h:25), uniform (MakeUniformEL, src/generator.h:98-124)
+ * or R-MAT A/B/C = .57/.19/.19 with the ID permutation (MakeRMatEL +
+ * PermuteIDs, src/generator.h:66-167).  pairs receives 2*num_pairs int32
+ * (u0, v0, u1, v1, ...).  nthreads <= 0 picks the hardware concurrency. */
+int dmm_generate_edgelist(int32_t scale, int32_t degree, int32_t uniform,
+                          int32_t nthreads, int32_t* pairs);
+
+/* Replaces BuilderBase::MakeGraph for synthetic input (src/builder.h:314-338
+ * with symmetrize forced by src/command_line.h:77-78): generate, build the
+ * CSR (count degrees, prefix sum, scatter), then squish (sort, unique, drop
+ * self-loops; src/builder.h:168-207).  num_nodes = max ID seen + 1
+ * (src/builder.h:72-89). */
+int dmm_build_graph(int32_t scale, int32_t degree, int32_t uniform,
+                    int32_t nthreads, dmm_host_graph** out);
+
+/* Replaces BuilderBase::MakeGraphFromEL + SquishGraph (src/builder.h:290-312,
+ * 209-226) for an explicit edge list.  num_nodes < 0 means max ID + 1.
+ * symmetrize != 0 builds the undirected graph; otherwise out- and in-CSR. */
+int dmm_build_graph_from_edgelist(int64_t num_pairs, const int32_t* pairs,
+                                  int64_t num_nodes, int32_t symmetrize,
+                                  int32_t nthreads, dmm_host_graph** out);
+
+int64_t dmm_host_graph_num_nodes(const dmm_host_graph* g);
+/* CSRGraph::num_edges_directed (src/graph.h:356-358). */
+int64_t dmm_host_graph_num_edges_directed(const dmm_host_graph* g);
+int32_t dmm_host_graph_directed(const dmm_host_graph* g);
+const int64_t* dmm_host_graph_out_offsets(const dmm_host_graph* g);
+const int32_t* dmm_host_graph_out_neighs(const dmm_host_graph* g);
+/* For undirected graphs these return the out arrays (src/graph.h:283). */
+const int64_t* dmm_host_graph_in_offsets(const dmm_host_graph* g);
+const int32_t* dmm_host_graph_in_neighs(const dmm_host_graph* g);
+void dmm_host_graph_free(dmm_host_graph* g);
+
+/* Replaces SourcePicker::PickNext x count (src/benchmark.h:49-87): mt19937
+ * seeded with kRandSeed, uniform over [0, num_nodes), rejecting out-degree 0.
+ * given_source >= 0 returns it every time (the -r flag). */
+int dmm_pick_sources(int64_t num_nodes, const int64_t* out_offsets,
+                     int32_t given_source, int32_t count, int32_t* out);
+
+/* Replaces Partition<T>(N) for PE pe of npes (src/partition.h:79-106). */
+int dmm_partition(int64_t num_nodes, int32_t npes, int32_t pe,
+                  int64_t* start, int64_t* end, int64_t* partition_width,
+                  int64_t* max_width);
+
+/* ======================================================================== */
+/* Device context                                                            */
+/* ======================================================================== */
+
+typedef struct dmm_ctx dmm_ctx;
+
+int dmm_device_count(int32_t* count);
+
+/* Creates a context on HIP device `device`; allocates its stream, events and
+ * control block.  The OpenSHMEM runtime state of the reference (shmem_init,
+ * pSync/pWrk work arrays, src/bfs.cc:248-261) has no counterpart beyond
+ * this. */
+int dmm_ctx_create(int32_t device, dmm_ctx** out);
+void dmm_ctx_destroy(dmm_ctx* ctx);
+
+/* Uploads one PE's share of a CSR graph (replaces the symmetric-heap CSR of
+ * src/builder.h:236-288 and the remote Neighborhood fetch of
+ * src/graph.h:138-162).
+ *   num_nodes        global vertex count N
+ *   [vbegin, vend)   this PE's vertex block (0, N for the whole graph)
+ *   out_offsets      (vend - vbegin + 1) int64 offsets into out_neighs
+ *   out_neighs       global int32 IDs, sorted ascending per vertex
+ *   in_offsets/in_neighs  the in-CSR block, or NULL for undirected graphs
+ *   num_edges_directed    CSRGraph::num_edges_directed() of the GLOBAL graph
+ *                         (src/graph.h:356-358); < 0 = this block's count.
+ * Replaces any previous graph on the ctx. */
+int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin,
+                     int64_t vend, const int64_t* out_offsets,
+                     const int32_t* out_neighs, const int64_t* in_offsets,
+                     const int32_t* in_neighs, int64_t num_edges_directed);
+
+/* Device memory currently held by the ctx, in bytes. */
+int dmm_ctx_device_bytes(const dmm_ctx* ctx, int64_t* bytes);
+
+/* ======================================================================== */
+/* Direction-optimizing BFS                                                  */
+/* ======================================================================== */
+
+#define DMM_STEP_TD 0  /* top-down step          (PrintStep "td") */
+#define DMM_STEP_BU 1  /* bottom-up step         (PrintStep "bu") */
+#define DMM_STEP_QB 2  /* QueueToBitmap          (PrintStep "e")  */
+#define DMM_STEP_BQ 3  /* BitmapToQueue          (PrintStep "c")  */
+
+typedef struct dmm_bfs_stats {
+  int32_t num_steps;
+  int32_t num_levels;                    /* TD + BU steps                 */
+  int32_t step_kind[DMM_MAX_STEPS];      /* DMM_STEP_*                    */
+  int64_t step_in[DMM_MAX_STEPS];        /* TD: |F|; BU: unvisited scanned;
+                                            QB/BQ: vertices converted      */
+  int64_t step_out[DMM_MAX_STEPS];       /* TD: |F'|; BU: awake_count     */
+  int64_t step_edges[DMM_MAX_STEPS];     /* edges examined                */
+  int64_t step_scout[DMM_MAX_STEPS];     /* TD: scout_count returned      */
+  float step_ms[DMM_MAX_STEPS];          /* device time per step (trace=1) */
+  int64_t alg_bytes;                     /* algorithmic HBM bytes, whole BFS */
+  int64_t bu_alg_bytes;                  /* ... of the bottom-up steps      */
+  int64_t reached;                       /* vertices with parent >= 0       */
+  int64_t reached_edges;                 /* sum of out-degree over reached  */
+  double total_ms;                       /* device time, InitParent..cleanup */
+  double bu_ms;                          /* device time in bottom-up steps   */
+  double host_ms;                        /* host wall time of the call       */
+} dmm_bfs_stats;
+
+/* Fast path of DOBFS (src/bfs.cc:142-188; oracle gapbs-verify/bfs.cc:
+ * 122-173): InitParent, the alpha/beta direction-optimizing level loop and
+ * the final "parent < -1 -> -1" cleanup, all on the device.  parent/depth
+ * (host, length vend - vbegin, may be NULL) receive the result; NULL leaves
+ * it in HBM for dmm_bfs_read.  depth[v] = BFS level, -1 if unreached.
+ * `trace` != 0 records per-step counters and per-step timing in stats (the
+ * loop is then host-driven one step at a time, like the reference's PrintStep
+ * trace); 0 runs the device-resident loop. */
+int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta,
+            int32_t* parent, int32_t* depth, int32_t trace,
+            dmm_bfs_stats* stats);
+
+/* Copies the last BFS result out of HBM (either pointer may be NULL). */
+int dmm_bfs_read(dmm_ctx* ctx, int32_t* parent, int32_t* depth);
+
+/* ---- single steps on the ctx's BFS state (host-driven DOBFS mirror) ---- */
+/* The state holds: parent[local], a sliding queue (window + in),
+ * bitmaps `front` and `curr` over all N vertices, and the level counter.   */
+
+/* InitParent (src/bfs.cc:131-140): parent = deg ? -deg : -1, parent[source]
+ * = source; queue = [source] (window slid); both bitmaps cleared; level 0. */
+int dmm_step_init(dmm_ctx* ctx, int32_t source);
+
+/* SHMEM_TDStep + slide_window (src/bfs.cc:81-100, 181-182).  *scout_count
+ * receives the scout count; *queue_size the new window size. */
+int dmm_step_td(dmm_ctx* ctx, int64_t* scout_count, int64_t* queue_size);
+
+/* QueueToBitmap (src/bfs.cc:104-112) into `front`, then slide_window
+ * (src/bfs.cc:166).  *awake_count receives the old window size. */
+int dmm_step_queue_to_bitmap(dmm_ctx* ctx, int64_t* awake_count);
+
+/* SHMEM_BUStep + swap (src/bfs.cc:57-78, 170-171). */
+int dmm_step_bu(dmm_ctx* ctx, int64_t* awake_count);
+
+/* BitmapToQueue (src/bfs.cc:115-125): ascending vertex order, slid. */
+int dmm_step_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size);
+
+/* Final cleanup (gapbs-verify/bfs.cc:168-171; src/bfs.cc:224-228). */
+int dmm_step_finish(dmm_ctx* ctx);
+
+/* State access for tests and for the PE-share runs. */
+int dmm_state_get_parent(dmm_ctx* ctx, int32_t* parent);
+int dmm_state_set_parent(dmm_ctx* ctx, const int32_t* parent);
+int dmm_state_get_queue(dmm_ctx* ctx, int32_t* queue, int64_t capacity,
+                        int64_t* size);
+/* Sets the current queue window (the vertices must be local to the PE). */
+int dmm_state_set_queue(dmm_ctx* ctx, const int32_t* queue, int64_t size);
+/* which: 0 = front, 1 = curr.  words = ceil(N / 64). */
+int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words);
+int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words);
+int dmm_state_set_level(dmm_ctx* ctx, int32_t level);
+
+/* ======================================================================== */
+/* Batched 64-source BFS (bit-lane multi-source BFS)                         */
+/* ======================================================================== */
+
+typedef struct dmm_batch_params {
+  int32_t td_div;        /* top-down while active edges < M / td_div       */
+  int32_t block;         /* threads per workgroup for the sweep kernels    */
+  int32_t graph_levels;  /* levels per captured hipGraph chunk             */
+  int32_t reserved;
+} dmm_batch_params;
+
+typedef struct dmm_batch_stats {
+  int32_t num_levels;
+  int32_t level_kind[DMM_MAX_STEPS];     /* DMM_STEP_TD / DMM_STEP_BU     */
+  int64_t level_active[DMM_MAX_STEPS];   /* vertices with new lanes       */
+  double total_ms;                       /* device time of the batch      */
+  double sweep_ms;                       /* ... of the bottom-up sweeps   */
+  int64_t sweep_alg_bytes;               /* algorithmic bytes of sweeps   */
+  int32_t sweeps;                        /* bottom-up sweeps run          */
+  int64_t reached_edges;                 /* sum over lanes of reached deg */
+  double host_ms;
+} dmm_batch_stats;
+
+/* Runs BFS from up to 64 sources at once; lane s of vertex v's 64-bit word
+ * records that v has been reached from sources[s].  depth (host, may be
+ * NULL) receives nsrc x (vend - vbegin) uint8 levels, source-major, 255 =
+ * unreached.  params may be NULL (defaults / autotuned values). */
+int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc,
+                    uint8_t* depth, const dmm_batch_params* params,
+                    dmm_batch_stats* stats);
+
+/* ======================================================================== */
+/* PageRank (pull)                                                           */
+/* ======================================================================== */
+
+#define DMM_PR_FP32_SERIAL 0  /* reference arithmetic: FP32, serial in-order */
+#define DMM_PR_FP32_KAHAN 1   /* FP32 scores, compensated per-vertex sums    */
+#define DMM_PR_FP64 2         /* FP64 scores and sums                        */
+
+typedef struct dmm_pr_stats {
+  int32_t iters;                         /* iterations run                 */
+  double err[DMM_MAX_PR_ITERS];          /* L1 change per iteration (FP64) */
+  double total_ms;                       /* device time, all iterations    */
+  double iter_ms;                        /* mean device time per iteration */
+  int64_t alg_bytes_per_iter;
+  double host_ms;
+} dmm_pr_stats;
+
+/* PageRankPull (src/pr.cc:41-70; oracle gapbs-verify/pr.cc:34-59): damping
+ * 0.85, scores initialised to 1/N, stop after max_iters or once the L1
+ * change < epsilon.  scores (host, may be NULL) receives vend - vbegin
+ * values: float for the FP32 modes, double for DMM_PR_FP64. */
+int dmm_pr_pull(dmm_ctx* ctx, int32_t max_iters, double epsilon,
+                int32_t mode, void* scores, dmm_pr_stats* stats);
+
+/* PE-share PageRank: one iteration over the uploaded block given the full
+ * global contribution vector (N floats, host) -- the in-HBM replacement of
+ * the per-edge shmem_getmem of src/pr.cc:58.  scores_inout holds the block's
+ * scores (vend - vbegin floats) and is updated in place; *err receives the
+ * block's L1 change.  mode: DMM_PR_FP32_SERIAL or DMM_PR_FP32_KAHAN. */
+int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
+                           float* scores_inout, int32_t mode, double* err,
+                           double* ms);
+
+/* ======================================================================== */
+/* Autotuner                                                                 */
+/* ======================================================================== */
+
+typedef struct dmm_tuning {
+  int32_t alpha;
+  int32_t beta;
+  int32_t grid_per_cu;     /* workgroups per CU of the grid-stride steps   */
+  int32_t td_edges_per_block; /* top-down edges per workgroup chunk (fixed) */
+  dmm_batch_params batch;
+  double plain_ms;         /* measured mean DOBFS time with these values   */
+  double batch_ms;         /* measured batch time with these values        */
+} dmm_tuning;
+
+/* Searches alpha/beta and launch shapes on this ctx's graph using the given
+ * sample sources; writes the best into *out and installs it on the ctx. */
+int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc,
+                 int32_t budget, dmm_tuning* out);
+int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t);
+int dmm_get_tuning(const dmm_ctx* ctx, dmm_tuning* t);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* DMM_GAPBS_H_ */

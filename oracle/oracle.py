@@ -1,0 +1,260 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg as the checker.  The product package never
+imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+REF_BFS = os.path.join(_HERE, "_ref", "libref_bfs.so")
+REF_PR = os.path.join(_HERE, "_ref", "libref_pr.so")
+
+_P64 = C.POINTER(C.c_int64)
+_P32 = C.POINTER(C.c_int32)
+_PU64 = C.POINTER(C.c_uint64)
+_PF = C.POINTER(C.c_float)
+_PD = C.POINTER(C.c_double)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run make -C oracle")
+        l = C.CDLL(LIB)
+        l.orc_init_parent.argtypes = [C.c_int64, _P64, C.c_int32, _P32]
+        l.orc_td_step.restype = C.c_int64
+        l.orc_td_step.argtypes = [_P64, _P32, _P32, _P32, C.c_int64, C.c_int64, _P64]
+        l.orc_bu_step.restype = C.c_int64
+        l.orc_bu_step.argtypes = [C.c_int64, _P64, _P32, _P32, _PU64, _PU64]
+        l.orc_queue_to_bitmap.argtypes = [_P32, C.c_int64, C.c_int64, _PU64]
+        l.orc_bitmap_to_queue.argtypes = [C.c_int64, _PU64, _P32, _P64]
+        l.orc_dobfs.restype = C.c_int32
+        l.orc_dobfs.argtypes = [C.c_int64, _P64, _P32, _P64, _P32, C.c_int64, C.c_int32, C.c_int32,
+                                C.c_int32, C.c_int64, _P32, _P32, _P64, C.c_int32]
+        l.orc_bfs_depth.argtypes = [C.c_int64, _P64, _P32, C.c_int32, _P32]
+        l.orc_bfs_verify.restype = C.c_int32
+        l.orc_bfs_verify.argtypes = [C.c_int64, _P64, _P32, _P64, _P32, C.c_int32, _P32, _P64]
+        l.orc_pr_pull_f32.restype = C.c_int32
+        l.orc_pr_pull_f32.argtypes = [C.c_int64, _P64, _P64, _P32, C.c_int32, C.c_double, _PF, _PD]
+        l.orc_pr_pull_f64.restype = C.c_int32
+        l.orc_pr_pull_f64.argtypes = [C.c_int64, _P64, _P64, _P32, C.c_int32, C.c_double, C.c_int32,
+                                      _PD, _PD]
+        l.orc_pr_residual_f32.restype = C.c_double
+        l.orc_pr_residual_f32.argtypes = [C.c_int64, _P64, _P32, _PF]
+        l.orc_dmm_spmd_bfs.restype = C.c_int32
+        l.orc_dmm_spmd_bfs.argtypes = [C.c_int64, _P64, _P32, C.c_int64, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, _P32]
+        _lib = l
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class CSR:
+    """Plain CSR arrays (undirected unless in_* given)."""
+
+    def __init__(self, n, out_offsets, out_neighs, in_offsets=None, in_neighs=None,
+                 num_edges_directed=None):
+        self.n = int(n)
+        self.oo = np.ascontiguousarray(out_offsets, np.int64)
+        self.on = np.ascontiguousarray(out_neighs, np.int32)
+        self.io = self.oo if in_offsets is None else np.ascontiguousarray(in_offsets, np.int64)
+        self.inn = self.on if in_neighs is None else np.ascontiguousarray(in_neighs, np.int32)
+        self.ned = int(self.oo[-1]) if num_edges_directed is None else int(num_edges_directed)
+
+    @classmethod
+    def of(cls, g):
+        """From a dist_mem_gapbs_amd HostGraph."""
+        if g.directed:
+            return cls(g.num_nodes, g.out_offsets, g.out_neighs, g.in_offsets, g.in_neighs,
+                       g.num_edges_directed)
+        return cls(g.num_nodes, g.out_offsets, g.out_neighs, num_edges_directed=g.num_edges_directed)
+
+
+def init_parent(g, source):
+    p = np.empty(g.n, np.int32)
+    lib().orc_init_parent(g.n, _p(g.oo, C.c_int64), source, _p(p, C.c_int32))
+    return p
+
+
+def dobfs(g, source, alpha=15, beta=18, beta_width=None, trace_cap=512):
+    parent = np.empty(g.n, np.int32)
+    kinds = np.empty(trace_cap, np.int32)
+    counts = np.empty(trace_cap, np.int64)
+    ns = lib().orc_dobfs(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), _p(g.io, C.c_int64),
+                         _p(g.inn, C.c_int32), g.ned, source, alpha, beta,
+                         g.n if beta_width is None else beta_width, _p(parent, C.c_int32),
+                         _p(kinds, C.c_int32), _p(counts, C.c_int64), trace_cap)
+    if ns < 0:
+        raise RuntimeError("oracle trace capacity exceeded")
+    return parent, list(zip(kinds[:ns].tolist(), counts[:ns].tolist()))
+
+
+def bfs_depth(g, source):
+    d = np.empty(g.n, np.int32)
+    lib().orc_bfs_depth(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), source, _p(d, C.c_int32))
+    return d
+
+
+def bfs_verify(g, source, parent):
+    parent = np.ascontiguousarray(parent, np.int32)
+    bad = C.c_int64(-1)
+    ok = lib().orc_bfs_verify(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), _p(g.io, C.c_int64),
+                              _p(g.inn, C.c_int32), source, _p(parent, C.c_int32), C.byref(bad))
+    return bool(ok), bad.value
+
+
+def td_step(g, parent, queue, qb, qe, qin):
+    q = C.c_int64(qin)
+    scout = lib().orc_td_step(_p(g.oo, C.c_int64), _p(g.on, C.c_int32), _p(parent, C.c_int32),
+                              _p(queue, C.c_int32), qb, qe, C.byref(q))
+    return scout, q.value
+
+
+def bu_step(g, parent, front, nxt):
+    return lib().orc_bu_step(g.n, _p(g.io, C.c_int64), _p(g.inn, C.c_int32), _p(parent, C.c_int32),
+                             _p(front, C.c_uint64), _p(nxt, C.c_uint64))
+
+
+def queue_to_bitmap(queue, qb, qe, bm):
+    lib().orc_queue_to_bitmap(_p(queue, C.c_int32), qb, qe, _p(bm, C.c_uint64))
+
+
+def bitmap_to_queue(n, bm, queue, qin):
+    q = C.c_int64(qin)
+    lib().orc_bitmap_to_queue(n, _p(bm, C.c_uint64), _p(queue, C.c_int32), C.byref(q))
+    return q.value
+
+
+def pr_f32(g, max_iters=20, eps=1e-4):
+    s = np.empty(g.n, np.float32)
+    tr = np.zeros(max(1, max_iters), np.float64)
+    it = lib().orc_pr_pull_f32(g.n, _p(g.oo, C.c_int64), _p(g.io, C.c_int64), _p(g.inn, C.c_int32),
+                               max_iters, eps, _p(s, C.c_float), _p(tr, C.c_double))
+    return s, it, tr[:it]
+
+
+def pr_f64(g, max_iters=20, eps=1e-4, exact_constants=False):
+    s = np.empty(g.n, np.float64)
+    tr = np.zeros(max(1, max_iters), np.float64)
+    it = lib().orc_pr_pull_f64(g.n, _p(g.oo, C.c_int64), _p(g.io, C.c_int64), _p(g.inn, C.c_int32),
+                               max_iters, eps, 1 if exact_constants else 0, _p(s, C.c_double),
+                               _p(tr, C.c_double))
+    return s, it, tr[:it]
+
+
+def pr_residual(g, scores):
+    s = np.ascontiguousarray(scores, np.float32)
+    return lib().orc_pr_residual_f32(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), _p(s, C.c_float))
+
+
+def dmm_spmd_bfs(g, source, npes, alpha=15, beta=18):
+    parent = np.empty(g.n, np.int32)
+    rc = lib().orc_dmm_spmd_bfs(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), g.ned, source, alpha,
+                                beta, npes, _p(parent, C.c_int32))
+    if rc != 0:
+        raise RuntimeError("spmd bfs failed")
+    return parent
+
+
+# ---- oracle/_ref: the reference's own gapbs-verify kernels ---------------------
+
+def ref_available():
+    return os.path.exists(REF_BFS) and os.path.exists(REF_PR)
+
+
+_ref = {}
+
+
+def _ref_libs():
+    if not _ref:
+        b = C.CDLL(REF_BFS)
+        b.ref_make_graph.argtypes = [C.c_int, C.c_int, C.c_int, _P64, C.POINTER(_P64), C.POINTER(_P32)]
+        b.ref_free.argtypes = [C.c_void_p]
+        b.ref_graph_create.restype = C.c_void_p
+        b.ref_graph_create.argtypes = [C.c_int64, _P64, _P32]
+        b.ref_graph_free.argtypes = [C.c_void_p]
+        b.ref_dobfs.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _P32]
+        b.ref_bfs_verify.argtypes = [C.c_void_p, C.c_int32, _P32]
+        b.ref_pick_sources.argtypes = [C.c_void_p, C.c_int32, _P32]
+        p = C.CDLL(REF_PR)
+        p.ref_pr_graph_create.restype = C.c_void_p
+        p.ref_pr_graph_create.argtypes = [C.c_int64, _P64, _P32]
+        p.ref_pr_graph_free.argtypes = [C.c_void_p]
+        p.ref_pagerank.argtypes = [C.c_void_p, C.c_int32, C.c_double, _PF]
+        _ref["bfs"], _ref["pr"] = b, p
+    return _ref["bfs"], _ref["pr"]
+
+
+def ref_make_graph(scale, degree, uniform):
+    b, _ = _ref_libs()
+    n = C.c_int64()
+    o = _P64()
+    nb = _P32()
+    if b.ref_make_graph(scale, degree, 1 if uniform else 0, C.byref(n), C.byref(o), C.byref(nb)) != 0:
+        raise RuntimeError("reference graph build failed")
+    n = n.value
+    offs = np.ctypeslib.as_array(o, (n + 1,)).copy()
+    m = int(offs[-1])
+    neigh = np.ctypeslib.as_array(nb, (max(m, 1),))[:m].copy()
+    b.ref_free(C.cast(o, C.c_void_p))
+    b.ref_free(C.cast(nb, C.c_void_p))
+    return offs, neigh
+
+
+class RefGraph:
+    """The reference's CSRGraph built from CSR arrays (both kernels)."""
+
+    def __init__(self, g):
+        b, p = _ref_libs()
+        self.g = g
+        self.hb = b.ref_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32))
+        self.hp = p.ref_pr_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32))
+
+    def dobfs(self, source, alpha=15, beta=18):
+        b, _ = _ref_libs()
+        parent = np.empty(self.g.n, np.int32)
+        b.ref_dobfs(self.hb, source, alpha, beta, _p(parent, C.c_int32))
+        return parent
+
+    def verify(self, source, parent):
+        b, _ = _ref_libs()
+        parent = np.ascontiguousarray(parent, np.int32)
+        return bool(b.ref_bfs_verify(self.hb, source, _p(parent, C.c_int32)))
+
+    def pick_sources(self, count):
+        b, _ = _ref_libs()
+        out = np.empty(count, np.int32)
+        b.ref_pick_sources(self.hb, count, _p(out, C.c_int32))
+        return out
+
+    def pagerank(self, max_iters=20, eps=1e-4):
+        _, p = _ref_libs()
+        s = np.empty(self.g.n, np.float32)
+        p.ref_pagerank(self.hp, max_iters, eps, _p(s, C.c_float))
+        return s
+
+    def close(self):
+        b, p = _ref_libs()
+        if self.hb:
+            b.ref_graph_free(self.hb)
+            self.hb = None
+        if self.hp:
+            p.ref_pr_graph_free(self.hp)
+            self.hp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
