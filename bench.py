@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X DOBFS / batched BFS / PageRank engine.
+
+Workload (the headline configuration): plain direction-optimizing BFS, one
+source per launch, on a Kronecker scale-22 graph (edge factor 16, A/B/C =
+.57/.19/.19, symmetrized, deduplicated, int32 IDs), 64 SourcePicker sources,
+alpha = 15, beta = 18.  A "step" is one DOBFS from one source with the graph
+resident in HBM.  value = aggregate GTEPS over the K timed steps (Graph500
+convention: edges of the reached component / 2 per source, summed, over the
+wall-clock time of the K steps).
+
+Also reported on the same line:
+  roofline      the bottom-up step kernel (k_step<2>) -- algorithmic bytes of
+                its launches over their device time (per-step HIP events in
+                trace mode), against 8 TB/s HBM3E
+  cpu_baseline  the reference's own DOBFS (gapbs-verify/bfs.cc built into
+                oracle/_ref, OpenMP) on a bounded sample of the same sources
+  kron24        Kronecker scale 24: plain DOBFS GTEPS, batched 64-source BFS
+                aggregate GTEPS and its ratio to plain, PageRankPull
+                edges/s/iter and HBM fraction (FP32 compensated, FP32
+                reference-order, FP64), and the FP32 error versus FP64.
+Synthetic graphs from the reference generator (no datasets).
+"""
+import argparse
+import contextlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def quiet_stdout():
+    """Silences C-level stdout (the reference kernels print per-step lines)."""
+    sys.stdout.flush()
+    fd = os.dup(1)
+    dn = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(dn, 1)
+    try:
+        yield
+    finally:
+        sys.stdout.flush()
+        os.dup2(fd, 1)
+        os.close(fd)
+        os.close(dn)
+
+
+def bu_roofline(ctx, sources, nverts):
+    """Per-step trace: bytes and device time of the bottom-up step kernel."""
+    import dist_mem_gapbs_amd._lib as L
+    tot_b, tot_ms, launches = 0, 0.0, 0
+    for s in sources:
+        _, _, st = ctx.DOBFS(int(s), trace=True, read=False)
+        for i in range(st.num_steps):
+            if st.step_kind[i] == L.DMM_STEP_BU:
+                u, e = st.step_in[i], st.step_edges[i]
+                tot_b += 20 * u + 4 * e + e // 8 + nverts // 8
+                tot_ms += st.step_ms[i]
+                launches += 1
+    return tot_b, tot_ms, launches
+
+
+def load_traffic(kernel, workload):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(og, sources, m_comp):
+    from oracle import oracle as orc
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if orc.ref_available():
+        rg = orc.RefGraph(og)
+        with quiet_stdout():
+            rg.dobfs(int(sources[0]))  # warm
+            t0 = time.perf_counter()
+            for s in sources:
+                rg.dobfs(int(s))
+            dt = time.perf_counter() - t0
+        rg.close()
+        kind = "reference"
+    else:
+        cores = 1
+        t0 = time.perf_counter()
+        for s in sources:
+            orc.dobfs(og, int(s))
+        dt = time.perf_counter() - t0
+        kind = "port"
+    edges = sum(m_comp[int(s)] for s in sources)
+    return {"value": edges / dt / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind,
+            "sample": f"{len(sources)} DOBFS trials (same Kronecker-22 graph and sources), "
+                      f"{dt:.2f} s"}
+
+
+def kron24_leg(dmm, args, tuned):
+    import dist_mem_gapbs_amd._lib as L
+    out = {}
+    t0 = time.time()
+    g = dmm.build_graph(args.big_scale, 16, uniform=False)
+    log(f"kron{args.big_scale}: n={g.num_nodes} m={g.num_edges_directed} built in {time.time() - t0:.1f}s")
+    srcs = dmm.pick_sources(g, 64)
+    deg = np.diff(g.out_offsets)
+    with dmm.Context(0) as ctx:
+        ctx.upload(g)
+        if tuned:
+            t = ctx.get_tuning()
+            t.alpha, t.beta = tuned.get("alpha", 15), tuned.get("beta", 18)
+            t.batch.td_div = tuned.get("td_div", t.batch.td_div)
+            t.grid_per_cu = tuned.get("grid_per_cu", t.grid_per_cu)
+            ctx.set_tuning(t)
+        # plain DOBFS on 16 sources (alpha/beta = 15/18)
+        mc = {}
+        for s in srcs[:16]:
+            _, _, st = ctx.DOBFS(int(s), read=False)
+            mc[int(s)] = st.reached_edges // 2
+        t0 = time.perf_counter()
+        for s in srcs[:16]:
+            ctx.DOBFS(int(s), read=False, with_stats=False)
+        dt = time.perf_counter() - t0
+        plain = sum(mc[int(s)] for s in srcs[:16]) / dt / 1e9
+        # batched 64 lanes
+        ctx.bfs_batch64(srcs, want_depth=False)
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, bst = ctx.bfs_batch64(srcs, want_depth=False)
+            wall = time.perf_counter() - t0
+            best = (wall, bst) if best is None or wall < best[0] else best
+        wall, bst = best
+        batched = bst.reached_edges / 2 / wall / 1e9
+        out["plain_gteps"] = plain
+        out["batched_gteps"] = batched
+        out["batched_over_plain"] = batched / plain
+        out["batched_levels"] = bst.num_levels
+        out["batched_sweep_bytes_per_s"] = (bst.sweep_alg_bytes / (bst.total_ms / 1e3)
+                                            if bst.total_ms > 0 else None)
+        # PageRank (eps = 0 forces 20 iterations for the timing)
+        pr = {}
+        sc64, _ = ctx.PageRankPull(20, 0.0, L.DMM_PR_FP64)
+        for name, mode in (("fp32_kahan", L.DMM_PR_FP32_KAHAN), ("fp32_reference_order", L.DMM_PR_FP32_SERIAL),
+                           ("fp64", L.DMM_PR_FP64)):
+            ctx.PageRankPull(20, 0.0, mode, read=False)
+            sc, pst = ctx.PageRankPull(20, 0.0, mode)
+            m = g.num_edges_directed
+            ent = {"iter_ms": pst.iter_ms, "edges_per_s_per_iter": m / (pst.iter_ms / 1e3),
+                   "alg_bytes_per_iter": pst.alg_bytes_per_iter,
+                   "hbm_frac": pst.alg_bytes_per_iter / (pst.iter_ms / 1e3) / HBM_PEAK}
+            if mode != L.DMM_PR_FP64:
+                d = np.abs(sc.astype(np.float64) - sc64)
+                ent.update({"max_abs_err_vs_fp64": float(d.max()), "l1_err_vs_fp64": float(d.sum()),
+                            "max_rel_err_vs_fp64": float((d / np.maximum(sc64, 1e-300)).max())})
+            pr[name] = ent
+        # convergence run at the reference tolerance
+        _, pst = ctx.PageRankPull(20, 1e-4, L.DMM_PR_FP32_KAHAN, read=False)
+        pr["iters_at_eps_1e-4"] = pst.iters
+        out["pagerank"] = pr
+    del deg
+    g.free()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="timed DOBFS trials")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--big-scale", type=int, default=24)
+    ap.add_argument("--no-big", action="store_true", help="skip the Kronecker-24 leg")
+    ap.add_argument("--cpu-sources", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        log("bench.py measures one GPU: this build has no multi-GPU path "
+            "(one MI355X plays one PE of the vertex-block layout). Run with --gpus 1.")
+        sys.exit(2)
+
+    import dist_mem_gapbs_amd as dmm
+    from oracle import oracle as orc
+
+    tuned = None
+    tp = os.path.join(ROOT, "dist_mem_gapbs_amd", "tuning.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tuned_all = json.load(f)
+    else:
+        tuned_all = {}
+
+    t0 = time.time()
+    g = dmm.build_graph(args.scale, 16, uniform=False)
+    log(f"kron{args.scale}: n={g.num_nodes} m={g.num_edges_directed} built in {time.time() - t0:.1f}s")
+    srcs = dmm.pick_sources(g, 64)
+    n = g.num_nodes
+    ned = g.num_edges_directed
+    with dmm.Context(0) as ctx:
+        ctx.upload(g)
+        tuned = tuned_all.get(f"kron-{args.scale}-16")
+        # m_comp per source (Graph500: reached out-edges / 2) and warm-up.
+        m_comp = {}
+        for s in srcs:
+            _, _, st = ctx.DOBFS(int(s), read=False)
+            m_comp[int(s)] = st.reached_edges // 2
+        for i in range(args.warmup):
+            ctx.DOBFS(int(srcs[i % 64]), read=False, with_stats=False)
+        order = [int(srcs[i % 64]) for i in range(args.steps)]
+        try:
+            import torch
+            sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+        except Exception:
+            sync = lambda: None
+        sync()
+        t0 = time.perf_counter()
+        for s in order:
+            ctx.DOBFS(s, read=False, with_stats=False)
+        sync()
+        wall = time.perf_counter() - t0
+        edges = sum(m_comp[s] for s in order)
+        gteps = edges / wall / 1e9
+        # device-only time of the same trials
+        dev_ms = []
+        for s in order[:64]:
+            _, _, st = ctx.DOBFS(s, read=False)
+            dev_ms.append(st.total_ms)
+        dev_gteps = sum(m_comp[s] for s in order[:64]) / (sum(dev_ms) / 1e3) / 1e9
+        # roofline of the bottom-up step kernel
+        b, ms, launches = bu_roofline(ctx, srcs[:16], n)
+        achieved = b / (ms / 1e3)
+        traffic = load_traffic("k_step<2>", f"kron-{args.scale}-16")
+        roof = {"bound": "hbm", "kernel": "k_step<2> (bottom-up step)", "achieved": achieved / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": traffic, "launches": launches, "avg_launch_us": ms * 1e3 / max(1, launches),
+                "alg_bytes_per_launch": b / max(1, launches)}
+    og = orc.CSR.of(g)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(og, srcs[:args.cpu_sources], m_comp)
+        log(f"cpu baseline: {cpu}")
+    del og
+    g.free()
+
+    extra = None
+    if not args.no_big:
+        extra = kron24_leg(dmm, args, tuned_all.get(f"kron-{args.big_scale}-16"))
+
+    line = {
+        "metric": "BFS GTEPS & PR edges/s/iter, batched/FP32 vs plain GPU, % HBM peak",
+        "value": gteps, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"plain DOBFS, Kronecker scale {args.scale} edge factor 16, "
+                               f"64 sources, alpha 15 beta 18, one source per launch",
+                   "num_nodes": n, "num_edges_directed": ned,
+                   "parallelism": "1 GPU = 1 PE"},
+        "device_gteps": dev_gteps,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        f"kron{args.big_scale}": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

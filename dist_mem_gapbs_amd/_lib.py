@@ -126,6 +126,7 @@ SIGNATURES = {
     "dmm_host_graph_in_neighs": (_PI32, [_P]),
     "dmm_host_graph_free": (None, [_P]),
     "dmm_pick_sources": (_I32, [_I64, _PI64, _I32, _I32, _PI32]),
+    "dmm_rng_sequence": (_I32, [_I32, _I64, _I64, _I64, _PI64]),
     "dmm_partition": (_I32, [_I64, _I32, _I32, _PI64, _PI64, _PI64, _PI64]),
     "dmm_device_count": (_I32, [_PI32]),
     "dmm_ctx_create": (_I32, [_I32, C.POINTER(_P)]),

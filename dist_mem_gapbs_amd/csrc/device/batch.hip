@@ -16,16 +16,18 @@
 //                                     out-degree prefix, for top-down levels
 //   ctrl   batch control words
 //
-// Level L runs ONE kernel, either
+// Level L runs one grid-stride kernel, either
 //   top-down  (push) over the active list: for v in out(u),
 //             new = F[L][u] & ~(seen[v] | F[L][v]); atomicOr into F[L+1][v];
-//             the first setter of F[L+1][v] appends v to the next list;
 //   bottom-up (pull) sweep over all u: need = lanes & ~(seen[u] | F[L][u]);
 //             OR F[L][v] over in(u) until every needed lane is found;
-//             F[L+1][u] = found & need (full overwrite).
-// The last workgroup picks the next direction: top-down while the next
-// frontier's out-edges are fewer than M / td_div (tuned).  A chain of these
-// launches is captured in a hipGraph.
+//             F[L+1][u] = found & need (full overwrite);
+// and a one-workgroup post kernel reduces the per-workgroup partials and
+// picks the next direction: top-down while the next frontier's out-edges
+// are fewer than M / td_div (tuned).  The active list for a top-down level
+// is compacted from F[L+1] in ascending vertex order (COUNT, scan, WRITE),
+// so no phase uses a same-address atomic.  A chain of these launches is
+// captured in a hipGraph.
 #include "batch.h"
 
 #include <algorithm>
@@ -40,20 +42,19 @@ constexpr int kItems = 4;
 constexpr int kEPB = kThreads * kItems;
 constexpr int kGroup = 8;  // lanes per vertex in the bottom-up sweep
 
-enum : int { kBTD = 1, kBBU = 2, kBDone = 4 };
+enum : int { kBTD = 1, kBBU = 2, kBCount = 3, kBWrite = 4, kBDone = 5 };
 
 struct BCtrl {
-  unsigned long long packed;          // next active list: degsum << 30 | count
-  unsigned long long scanned[kSlots]; // edges scanned by sweeps
-  long long acount, aedges;           // current active list size / out-edges
+  long long acount, aedges;           // active list size / its out-edges
+  long long pend_cnt, pend_degs;      // counted next frontier
+  long long sweep_scanned_total;      // edges scanned by all sweeps
   int level, phase, done, cur;        // cur = active list buffer index
   int nlevels;
   int sweeps;
-  unsigned ticket;
+  int error;
   int pad;
   int level_kind[DMM_MAX_STEPS];
   long long level_active[DMM_MAX_STEPS];
-  long long sweep_scanned_total;
 };
 
 struct BArgs {
@@ -70,32 +71,24 @@ struct BArgs {
   int64_t* aoff0;
   int64_t* aoff1;
   BCtrl* ctrl;
+  int64_t* part;      // [gridDim.x][kPartWidth]
   uint64_t lanes;     // mask of used lanes
   int64_t m;          // directed edges of the block (td threshold base)
   int td_div;
 };
-
-__device__ __forceinline__ unsigned long long ld_agent(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 struct TdLds {
   int32_t rel[kEPB + 2];
   int64_t base[kEPB + 2];
   int32_t u[kEPB + 2];
   uint64_t f[kEPB + 2];
-  int32_t cv[kEPB];
-  int64_t scan[kThreads / 64 + 1];
   int64_t i0, nv;
-  unsigned long long basec, based;
-  int nclaim;
 };
 
 struct SmallLds {
+  int64_t red[kPartWidth][kThreads / 64];
   int64_t scan_c[kThreads / 64 + 1];
   int64_t scan_d[kThreads / 64 + 1];
-  unsigned long long red[kThreads / 64];
-  unsigned long long basec, based;
 };
 
 union BLds {
@@ -116,8 +109,6 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
   const int cur = c->cur;
   const int32_t* q = cur ? a.act1 : a.act0;
   const int64_t* qoff = cur ? a.aoff1 : a.aoff0;
-  int32_t* oq = cur ? a.act0 : a.act1;
-  int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
   const int64_t F = c->acount;
   const int64_t T = c->aedges;
   const uint64_t* Fl = a.levels + level * a.nloc;
@@ -128,25 +119,24 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
     if (tid == 0) {
       int64_t lo = 0, hi = F;
       while (hi - lo > 1) {
-        int64_t mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         if (qoff[mid] <= e0) lo = mid; else hi = mid;
       }
-      int64_t i0 = lo;
+      const int64_t i0 = lo;
       hi = F;
       while (hi - lo > 1) {
-        int64_t mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         if (qoff[mid] <= e1 - 1) lo = mid; else hi = mid;
       }
       L.i0 = i0;
       L.nv = lo - i0 + 1;
-      L.nclaim = 0;
     }
     __syncthreads();
     const int64_t i0 = L.i0;
     // Listed vertices have out-degree >= 1: a chunk spans <= kEPB + 1 of them.
     int nv = static_cast<int>(L.nv);
     if (nv > kEPB + 2) {
-      if (tid == 0) c->pad = 1;
+      if (tid == 0) c->error = 1;
       nv = kEPB + 2;
     }
     for (int i = tid; i < nv; i += kThreads) {
@@ -160,7 +150,7 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
       L.f[i] = fu;
       L.base[i] = a.out_off[lu] - qo;
       // Lazy commit of level L into seen (idempotent if several chunks
-      // cover u).
+      // cover u; the only writers of seen[u] in this step).
       a.seen[lu] = a.seen[lu] | fu;
     }
     __syncthreads();
@@ -171,23 +161,17 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
         const int32_t r = static_cast<int32_t>(e - e0);
         int lo = 0, hi = nv;
         while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
+          const int mid = (lo + hi) >> 1;
           if (L.rel[mid] <= r) lo = mid; else hi = mid;
         }
         const int32_t v = a.out_nbr[L.base[lo] + e];
         const int64_t lv = int64_t(v) - a.vbegin;
+        // seen[v] may or may not hold F[L][v] yet (lazy commit): mask both.
         uint64_t nw = L.f[lo] & ~a.seen[lv];
         if (nw) nw &= ~Fl[lv];
         if (nw) {
-          const unsigned long long old =
-              atomicOr(reinterpret_cast<unsigned long long*>(&Fn[lv]), static_cast<unsigned long long>(nw));
-          if (old == 0ull) {
-            if (a.out_off[lv + 1] > a.out_off[lv]) {
-              const int slot = atomicAdd(&L.nclaim, 1);
-              L.cv[slot] = v;
-            }
-          }
-          // A vertex with no out-edges never enters the active list, so its
+          atomicOr(reinterpret_cast<unsigned long long*>(&Fn[lv]), static_cast<unsigned long long>(nw));
+          // A vertex without out-edges never enters an active list, so its
           // lanes are committed to seen here (the only writers of seen[v]
           // for such v are these atomics).
           if (a.out_off[lv + 1] == a.out_off[lv])
@@ -196,51 +180,16 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
       }
     }
     __syncthreads();
-    const int nclaim = L.nclaim;
-    int64_t cdeg[kItems];
-    int64_t acc = 0;
-#pragma unroll
-    for (int j = 0; j < kItems; j++) {
-      const int idx = tid * kItems + j;
-      cdeg[j] = 0;
-      if (idx < nclaim) {
-        const int64_t lv = int64_t(L.cv[idx]) - a.vbegin;
-        cdeg[j] = a.out_off[lv + 1] - a.out_off[lv];
-      }
-      acc += cdeg[j];
-    }
-    int64_t total;
-    const int64_t excl = block_excl_scan<kThreads>(acc, L.scan, &total);
-    if (tid == 0) {
-      const unsigned long long add = (static_cast<unsigned long long>(total) << kCountBits) |
-                                     static_cast<unsigned long long>(nclaim);
-      const unsigned long long old = nclaim > 0 ? atomicAdd(&c->packed, add) : 0ull;
-      L.basec = old & kCountMask;
-      L.based = old >> kCountBits;
-    }
-    __syncthreads();
-    int64_t run = static_cast<int64_t>(L.based) + excl;
-    const int64_t basec = static_cast<int64_t>(L.basec);
-#pragma unroll
-    for (int j = 0; j < kItems; j++) {
-      const int idx = tid * kItems + j;
-      if (idx < nclaim) {
-        oq[basec + idx] = L.cv[idx];
-        oqoff[basec + idx] = run;
-        run += cdeg[j];
-      }
-    }
-    __syncthreads();
   }
+  if (threadIdx.x < kPartWidth) a.part[blockIdx.x * kPartWidth + threadIdx.x] = 0;
 }
 
 // Bottom-up sweep: kGroup lanes per vertex, strided over its in-neighbours,
 // OR-reduced across the group after every round of kGroup neighbours so the
-// group can stop as soon as every needed lane is found.
+// group can stop as soon as every needed lane is found.  Partials: edges
+// scanned, vertices of the next frontier (out-degree >= 1) and their
+// out-degree sum.
 __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
-  const int cur = c->cur;
-  int32_t* oq = cur ? a.act0 : a.act1;
-  int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
   const uint64_t* __restrict__ Fl = a.levels + level * a.nloc;
   uint64_t* __restrict__ Fn = a.levels + (level + 1) * a.nloc;
   const int64_t* __restrict__ in_off = a.in_off;
@@ -248,127 +197,217 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
   const int sub = threadIdx.x & (kGroup - 1);
   const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
-  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
-  unsigned long long scanned = 0;
-  for (int64_t it = 0; it < iters; it++) {
-    const int64_t u = grp + it * ngrp;
+  int64_t scanned = 0, cnt = 0, degs = 0;
+  for (int64_t u = grp; u < a.nloc; u += ngrp) {
+    const uint64_t s = a.seen[u] | Fl[u];
+    const uint64_t need = a.lanes & ~s;
+    const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
-    int64_t deg_u = 0;
-    if (u < a.nloc) {
-      const uint64_t s = a.seen[u] | Fl[u];
-      const uint64_t need = a.lanes & ~s;
-      const int64_t b = in_off[u], e = in_off[u + 1];
-      deg_u = a.out_off[u + 1] - a.out_off[u];
-      if (need && e > b) {
-        uint64_t acc = 0;
-        int64_t stop = e;
-        for (int64_t k0 = b; k0 < e; k0 += kGroup) {
-          const int64_t k = k0 + sub;
-          if (k < e) acc |= Fl[in_nbr[k]];
+    if (need && e > b) {
+      uint64_t acc = 0;
+      int64_t stop = e;
+      for (int64_t k0 = b; k0 < e; k0 += kGroup) {
+        const int64_t k = k0 + sub;
+        if (k < e) acc |= Fl[in_nbr[k]];
 #pragma unroll
-          for (int d = kGroup / 2; d >= 1; d >>= 1)
-            acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
-          if ((acc & need) == need) {
-            stop = min<int64_t>(k0 + kGroup, e);
-            break;
-          }
+        for (int d = kGroup / 2; d >= 1; d >>= 1)
+          acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
+        if ((acc & need) == need) {
+          stop = min<int64_t>(k0 + kGroup, e);
+          break;
         }
-        scanned += static_cast<unsigned long long>(stop - b);
-        nxt = acc & need;
       }
-      if (sub == 0) {
-        Fn[u] = nxt;
-        if (Fl[u] | nxt) a.seen[u] = s | nxt;
+      scanned += stop - b;
+      nxt = acc & need;
+    }
+    if (sub == 0) {
+      Fn[u] = nxt;
+      if (Fl[u] | nxt) a.seen[u] = s | nxt;
+      if (nxt) {
+        const int64_t d = a.out_off[u + 1] - a.out_off[u];
+        if (d > 0) {
+          cnt++;
+          degs += d;
+        }
       }
     }
-    // Append the group's vertex to the next active list (one packed atomic
-    // per workgroup per round).
-    const int64_t c1 = (sub == 0 && nxt && deg_u > 0) ? 1 : 0;
-    const int64_t d1 = c1 ? deg_u : 0;
-    int64_t tc, td;
-    const int64_t ec = block_excl_scan<kThreads>(c1, L.scan_c, &tc);
-    const int64_t ed = block_excl_scan<kThreads>(d1, L.scan_d, &td);
-    if (threadIdx.x == 0) {
-      const unsigned long long add = (static_cast<unsigned long long>(td) << kCountBits) |
-                                     static_cast<unsigned long long>(tc);
-      const unsigned long long old = tc > 0 ? atomicAdd(&c->packed, add) : 0ull;
-      L.basec = old & kCountMask;
-      L.based = old >> kCountBits;
-    }
-    __syncthreads();
-    if (c1) {
-      oq[static_cast<int64_t>(L.basec) + ec] = static_cast<int32_t>(u + a.vbegin);
-      oqoff[static_cast<int64_t>(L.basec) + ec] = static_cast<int64_t>(L.based) + ed;
-    }
-    __syncthreads();
   }
   // Each group counted its scan in every lane; keep one lane's count.
   if (sub != 0) scanned = 0;
-  scanned = wave_sum(scanned);
-  if (lane_id() == 0 && scanned) atomicAdd(&c->scanned[blockIdx.x & (kSlots - 1)], scanned);
+  const int64_t s0 = block_sum<kThreads>(scanned, L.red[0]);
+  const int64_t s1 = block_sum<kThreads>(cnt, L.red[1]);
+  const int64_t s2 = block_sum<kThreads>(degs, L.red[2]);
+  if (threadIdx.x == 0) {
+    int64_t* p = a.part + blockIdx.x * kPartWidth;
+    p[0] = s0;
+    p[1] = s1;
+    p[2] = s2;
+    p[3] = 0;
+  }
 }
 
-__device__ void epilogue(const BArgs& a, BCtrl* c, int phase) {
-  const unsigned long long packed = ld_agent(&c->packed);
-  c->packed = 0;
-  const long long cnt = static_cast<long long>(packed & kCountMask);
-  const long long degs = static_cast<long long>(packed >> kCountBits);
-  const int level = c->level;
-  if (level < DMM_MAX_STEPS) {
-    c->level_kind[level] = phase == kBTD ? DMM_STEP_TD : DMM_STEP_BU;
-    c->level_active[level] = c->acount;
-  }
-  if (phase == kBBU) {
-    unsigned long long s = 0;
-    for (int i = 0; i < kSlots; i++) {
-      s += ld_agent(&c->scanned[i]);
-      c->scanned[i] = 0;
+// COUNT / WRITE over the frontier words F[level]: contiguous vertex ranges
+// per workgroup in ascending order; listed = nonzero word and out-degree >= 1.
+template <bool kWrite>
+__device__ void count_or_write(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
+  const uint64_t* Fl = a.levels + level * a.nloc;
+  const int64_t vpb = (a.nloc + gridDim.x - 1) / gridDim.x;
+  const int64_t v0 = min(a.nloc, int64_t(blockIdx.x) * vpb), v1 = min(a.nloc, v0 + vpb);
+  const int64_t vpt = (vpb + kThreads - 1) / kThreads;
+  const int64_t t0 = min(v1, v0 + threadIdx.x * vpt), t1 = min(v1, t0 + vpt);
+  int64_t cnt = 0, degs = 0;
+  for (int64_t u = t0; u < t1; u++) {
+    if (Fl[u]) {
+      const int64_t d = a.out_off[u + 1] - a.out_off[u];
+      if (d > 0) {
+        cnt++;
+        degs += d;
+      }
     }
-    c->sweep_scanned_total += static_cast<long long>(s);
-    c->sweeps = c->sweeps + 1;
   }
-  c->acount = cnt;
-  c->aedges = degs;
-  c->cur = 1 - c->cur;
-  c->level = level + 1;
-  c->nlevels = level + 2;  // levels 0..level+1 hold frontiers
-  if (cnt == 0) {
-    c->phase = kBDone;
-    c->done = 1;
-    c->nlevels = level + 1;
-  } else if (level + 2 >= a.level_cap) {
-    c->phase = kBDone;  // out of level storage: the host grows and reruns
-    c->done = 3;
-  } else {
-    c->phase = (degs * a.td_div < a.m) ? kBTD : kBBU;
+  if (!kWrite) {
+    const int64_t sc = block_sum<kThreads>(cnt, L.red[0]);
+    const int64_t sd = block_sum<kThreads>(degs, L.red[1]);
+    if (threadIdx.x == 0) {
+      int64_t* p = a.part + blockIdx.x * kPartWidth;
+      p[0] = sc;
+      p[1] = sd;
+      p[2] = 0;
+      p[3] = 0;
+    }
+    return;
+  }
+  const int cur = c->cur;
+  int32_t* oq = cur ? a.act0 : a.act1;
+  int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
+  int64_t tc, td;
+  int64_t pos = a.part[blockIdx.x * kPartWidth + 0] + block_excl_scan<kThreads>(cnt, L.scan_c, &tc);
+  int64_t run = a.part[blockIdx.x * kPartWidth + 1] + block_excl_scan<kThreads>(degs, L.scan_d, &td);
+  for (int64_t u = t0; u < t1; u++) {
+    if (Fl[u]) {
+      const int64_t d = a.out_off[u + 1] - a.out_off[u];
+      if (d > 0) {
+        oq[pos] = static_cast<int32_t>(u + a.vbegin);
+        oqoff[pos] = run;
+        pos++;
+        run += d;
+      }
+    }
   }
 }
 
 __global__ __launch_bounds__(kThreads) void k_bstep(BArgs a) {
   __shared__ BLds lds;
-  __shared__ int s_last;
   BCtrl* c = a.ctrl;
   const int phase = c->phase;
-  if (phase != kBTD && phase != kBBU) return;
   const int level = c->level;
-  zero_next_next(a, level);
-  if (phase == kBTD)
-    td_phase(a, c, lds.td, level);
+  switch (phase) {
+    case kBTD:
+      zero_next_next(a, level);
+      td_phase(a, c, lds.td, level);
+      break;
+    case kBBU:
+      zero_next_next(a, level);
+      bu_phase(a, c, lds.sm, level);
+      break;
+    case kBCount: count_or_write<false>(a, c, lds.sm, level); break;
+    case kBWrite: count_or_write<true>(a, c, lds.sm, level); break;
+    default: break;
+  }
+}
+
+__device__ void choose_next(const BArgs& a, BCtrl* c, long long cnt, long long degs, bool have_list) {
+  c->pend_cnt = cnt;
+  c->pend_degs = degs;
+  if (cnt == 0) {
+    c->phase = kBDone;
+    c->done = 1;
+    c->nlevels = c->level + 1;  // F[level] is empty
+    return;
+  }
+  if (c->level + 1 >= a.level_cap) {
+    c->phase = kBDone;  // out of level storage: the host grows and reruns
+    c->done = 3;
+    return;
+  }
+  if (degs * a.td_div < a.m)
+    c->phase = have_list ? kBWrite : kBCount;
   else
-    bu_phase(a, c, lds.sm, level);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(&c->ticket, 1u);
-    s_last = (t == gridDim.x - 1);
+    c->phase = kBBU;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bpost(BArgs a, int nblocks) {
+  __shared__ int64_t red[kPartWidth][kThreads / 64];
+  __shared__ int64_t sc[kThreads / 64 + 1];
+  __shared__ int64_t sd[kThreads / 64 + 1];
+  BCtrl* c = a.ctrl;
+  const int phase = c->phase;
+  if (phase != kBTD && phase != kBBU && phase != kBCount && phase != kBWrite) return;
+  int64_t s0 = 0, s1 = 0, s2 = 0;
+  if (phase == kBBU || phase == kBCount) {
+    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+      s0 += a.part[b * kPartWidth + 0];
+      s1 += a.part[b * kPartWidth + 1];
+      s2 += a.part[b * kPartWidth + 2];
+    }
+    s0 = block_sum<kThreads>(s0, red[0]);
+    s1 = block_sum<kThreads>(s1, red[1]);
+    s2 = block_sum<kThreads>(s2, red[2]);
   }
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    c->ticket = 0;
-    epilogue(a, c, phase);
-    __threadfence();
+  if (phase == kBCount) {
+    int64_t carry_c = 0, carry_d = 0;
+    for (int base = 0; base < nblocks; base += kThreads) {
+      const int b = base + threadIdx.x;
+      const int64_t vc = b < nblocks ? a.part[b * kPartWidth + 0] : 0;
+      const int64_t vd = b < nblocks ? a.part[b * kPartWidth + 1] : 0;
+      int64_t tc, td;
+      const int64_t ec = block_excl_scan<kThreads>(vc, sc, &tc);
+      const int64_t ed = block_excl_scan<kThreads>(vd, sd, &td);
+      if (b < nblocks) {
+        a.part[b * kPartWidth + 0] = carry_c + ec;
+        a.part[b * kPartWidth + 1] = carry_d + ed;
+      }
+      carry_c += tc;
+      carry_d += td;
+    }
   }
+  if (threadIdx.x != 0) return;
+  const int level = c->level;
+  if (phase == kBTD || phase == kBBU) {
+    if (level < DMM_MAX_STEPS) {
+      c->level_kind[level] = phase == kBTD ? DMM_STEP_TD : DMM_STEP_BU;
+      c->level_active[level] = c->acount;
+    }
+    c->level = level + 1;
+    c->nlevels = level + 2;  // F[0 .. level + 1] hold frontiers
+    if (phase == kBTD) {
+      c->phase = kBCount;
+      if (c->level + 1 >= a.level_cap) {
+        c->phase = kBDone;
+        c->done = 3;
+      }
+      return;
+    }
+    c->sweeps = c->sweeps + 1;
+    c->sweep_scanned_total += s0;
+    // A sweep's partials are not in vertex order: a top-down level next
+    // needs COUNT + WRITE.
+    c->acount = s1;
+    c->aedges = s2;
+    choose_next(a, c, s1, s2, false);
+    return;
+  }
+  if (phase == kBCount) {
+    c->acount = s0;
+    c->aedges = s1;
+    choose_next(a, c, s0, s1, true);
+    return;
+  }
+  // WRITE: the written list is the active list of this level.
+  c->cur = 1 - c->cur;
+  c->acount = c->pend_cnt;
+  c->aedges = c->pend_degs;
+  c->phase = kBTD;
 }
 
 // Level 0: F[0] = sources' lanes, F[1] cleared, seen cleared; active list =
@@ -384,39 +423,40 @@ __global__ __launch_bounds__(kThreads) void k_binit(BArgs a, const int32_t* srcs
 }
 
 __global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
-  // Single workgroup: set the source lanes and build the level-0 list.
-  __shared__ int cnt;
+  // One thread: set the source lanes and build the level-0 list.
   BCtrl* c = a.ctrl;
-  if (threadIdx.x == 0) {
-    cnt = 0;
-    long long degs = 0;
-    for (int s = 0; s < nsrc; s++) {
-      const int64_t lu = int64_t(srcs[s]) - a.vbegin;
-      if (lu < 0 || lu >= a.nloc) continue;
-      const unsigned long long old =
-          atomicOr(reinterpret_cast<unsigned long long*>(&a.levels[lu]), 1ull << s);
-      // A source without out-edges reaches only itself: no list entry.
-      if (old == 0ull && a.out_off[lu + 1] > a.out_off[lu]) {
-        a.act0[cnt] = srcs[s];
-        a.aoff0[cnt] = degs;
-        degs += a.out_off[lu + 1] - a.out_off[lu];
-        cnt++;
-      }
+  if (threadIdx.x != 0) return;
+  int cnt = 0;
+  long long degs = 0;
+  for (int s = 0; s < nsrc; s++) {
+    const int64_t lu = int64_t(srcs[s]) - a.vbegin;
+    if (lu < 0 || lu >= a.nloc) continue;
+    const uint64_t old = a.levels[lu];
+    a.levels[lu] = old | (1ull << s);
+    // A source without out-edges reaches only itself: no list entry.
+    if (old == 0ull && a.out_off[lu + 1] > a.out_off[lu]) {
+      a.act0[cnt] = srcs[s];
+      a.aoff0[cnt] = degs;
+      degs += a.out_off[lu + 1] - a.out_off[lu];
+      cnt++;
     }
-    c->packed = 0;
-    for (int i = 0; i < kSlots; i++) c->scanned[i] = 0;
-    c->acount = cnt;
-    c->aedges = degs;
-    c->level = 0;
-    c->done = 0;
-    c->cur = 0;
-    c->nlevels = 1;
-    c->sweeps = 0;
-    c->ticket = 0;
-    c->pad = 0;
-    c->sweep_scanned_total = 0;
-    c->phase = cnt == 0 ? kBDone : ((degs * a.td_div < a.m) ? kBTD : kBBU);
-    if (cnt == 0) c->done = 1;
+  }
+  c->acount = cnt;
+  c->aedges = degs;
+  c->pend_cnt = cnt;
+  c->pend_degs = degs;
+  c->level = 0;
+  c->done = 0;
+  c->cur = 0;
+  c->nlevels = 1;
+  c->sweeps = 0;
+  c->error = 0;
+  c->sweep_scanned_total = 0;
+  if (cnt == 0) {
+    c->phase = kBDone;
+    c->done = 1;
+  } else {
+    c->phase = (degs * a.td_div < a.m) ? kBTD : kBBU;
   }
 }
 
@@ -450,20 +490,27 @@ __global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int 
 }
 
 // Sum over lanes of the out-degrees of reached vertices (GTEPS numerator).
-__global__ __launch_bounds__(kThreads) void k_breach(BArgs a, unsigned long long* out) {
-  __shared__ unsigned long long red[kThreads / 64];
+__global__ __launch_bounds__(kThreads) void k_breach(BArgs a) {
+  __shared__ int64_t red[kThreads / 64];
   const int nlev = a.ctrl->nlevels;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  unsigned long long acc = 0;
+  int64_t acc = 0;
   for (int64_t u = gtid; u < a.nloc; u += gsize) {
     uint64_t reached = 0;
     for (int L = 0; L < nlev; L++) reached |= a.levels[L * a.nloc + u];
-    acc += static_cast<unsigned long long>(__popcll(reached)) *
-           static_cast<unsigned long long>(a.out_off[u + 1] - a.out_off[u]);
+    acc += static_cast<int64_t>(__popcll(reached)) * (a.out_off[u + 1] - a.out_off[u]);
   }
-  unsigned long long s = block_sum<kThreads>(acc, red);
-  if (threadIdx.x == 0 && s) atomicAdd(out, s);
+  const int64_t s = block_sum<kThreads>(acc, red);
+  if (threadIdx.x == 0) a.part[blockIdx.x * kPartWidth] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_breach_sum(BArgs a, int nblocks, int64_t* out) {
+  __shared__ int64_t red[kThreads / 64];
+  int64_t s = 0;
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) s += a.part[b * kPartWidth];
+  s = block_sum<kThreads>(s, red);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
@@ -482,6 +529,7 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   a.aoff0 = ctx->d_baoff;
   a.aoff1 = ctx->d_baoff + ctx->nloc + 64;
   a.ctrl = reinterpret_cast<BCtrl*>(ctx->d_bctrl);
+  a.part = ctx->d_part;
   a.lanes = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
   a.m = std::max<int64_t>(1, ctx->m_out);
   a.td_div = std::max(1, td_div);
@@ -497,8 +545,10 @@ int ensure_levels(dmm_ctx* ctx, int64_t cap) {
                      "batch level frontiers");
   if (rc) return rc;
   ctx->level_cap = cap;
+  // Captured chains hold the old level pointer.
   if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
-  ctx->batch_graph = nullptr;
+  if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
+  ctx->batch_graph = ctx->batch_cont = nullptr;
   return DMM_OK;
 }
 
@@ -518,36 +568,41 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bsrc, ctx->h_bsrc, sizeof(int32_t) * 64, hipMemcpyHostToDevice,
                                     ctx->stream));
   for (int attempt = 0; attempt < 4; attempt++) {
-    const int key = p.td_div * 1000 + p.graph_levels * 10 + (nsrc == 64 ? 1 : 0) + nsrc * 100000;
+    const int key = p.td_div * 1000 + p.graph_levels * 10 + nsrc * 1000000 + ctx->tuning.grid_per_cu;
     BArgs a = make_args(ctx, nsrc, p.td_div);
-    const int blocks = std::max(1, ctx->cu_count) * ctx->tuning.grid_per_cu;
-    if (!ctx->batch_graph || ctx->batch_graph_key != key) {
+    const int blocks = static_cast<int>(std::min<int64_t>(
+        int64_t(std::max(1, ctx->cu_count)) * std::max(1, std::min(kMaxGridPerCU, ctx->tuning.grid_per_cu)),
+        ctx->part_cap));
+    if (!ctx->batch_graph || !ctx->batch_cont || ctx->batch_graph_key != key) {
       if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
-      ctx->batch_graph = nullptr;
+      if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
+      ctx->batch_graph = ctx->batch_cont = nullptr;
       hipGraph_t g = nullptr;
       DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       hipLaunchKernelGGL(k_binit, dim3(blocks), dim3(kThreads), 0, ctx->stream, a,
                          static_cast<const int32_t*>(ctx->d_bsrc), nsrc);
       hipLaunchKernelGGL(k_binit2, dim3(1), dim3(64), 0, ctx->stream, a,
                          static_cast<const int32_t*>(ctx->d_bsrc), nsrc);
-      for (int i = 0; i < p.graph_levels; i++)
+      for (int i = 0; i < p.graph_levels; i++) {
         hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kThreads), 0, ctx->stream, a, blocks);
+      }
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
       e = hipGraphInstantiate(&ctx->batch_graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
-      ctx->batch_graph_key = key;
-      if (!ctx->batch_cont) {
-        DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < 4; i++)
-          hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
-        e = hipStreamEndCapture(ctx->stream, &g);
-        if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&ctx->batch_cont, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
+      DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < 6; i++) {
+        hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kThreads), 0, ctx->stream, a, blocks);
       }
+      e = hipStreamEndCapture(ctx->stream, &g);
+      if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&ctx->batch_cont, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
+      ctx->batch_graph_key = key;
     }
     BCtrl* h = reinterpret_cast<BCtrl*>(ctx->h_bctrl);
     DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
@@ -556,21 +611,20 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
     DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_bctrl, sizeof(BCtrl), hipMemcpyDeviceToHost, ctx->stream));
     DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int guard = 0;
-    while (h->done == 0) {
-      if (++guard > 1000) return set_err(ctx, DMM_ERR_STATE, "batched BFS did not terminate");
+    while (h->done == 0 && !h->error) {
+      if (++guard > 100000) return set_err(ctx, DMM_ERR_STATE, "batched BFS did not terminate");
       DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->batch_cont, ctx->stream));
       DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
       DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_bctrl, sizeof(BCtrl), hipMemcpyDeviceToHost, ctx->stream));
       DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
+    if (h->error) return set_err(ctx, DMM_ERR_STATE, "batched BFS active list invariant violated");
     if (h->done == 3) {
       // More levels than storage: grow (up to 255 levels for uint8 depths).
       if (ctx->level_cap >= 255)
         return set_err(ctx, DMM_ERR_CAPACITY, "batched BFS deeper than 254 levels");
       rc = ensure_levels(ctx, std::min<int64_t>(255, ctx->level_cap * 2));
       if (rc) return rc;
-      if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
-      ctx->batch_cont = nullptr;
       continue;
     }
     float ms = 0;
@@ -606,13 +660,13 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
       DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
     if (stats) {
-      unsigned long long* out = reinterpret_cast<unsigned long long*>(ctx->d_scratch64);
-      launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t*>(out), 1, 0);
-      hipLaunchKernelGGL(k_breach, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, out);
-      unsigned long long r = 0;
+      int64_t* out = ctx->d_scratch64;
+      hipLaunchKernelGGL(k_breach, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL(k_breach_sum, dim3(1), dim3(kThreads), 0, ctx->stream, a, blocks, out);
+      int64_t r = 0;
       DMM_HIP_CHECK(ctx, hipMemcpyAsync(&r, out, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
       DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-      stats->reached_edges = static_cast<int64_t>(r);
+      stats->reached_edges = r;
     }
     return DMM_OK;
   }

@@ -68,11 +68,11 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->batch_graph_key = -1;
   ctx->bfs_graph_steps = 0;
   void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
-                  ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],  ctx->d_blk,
+                  ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],
                   ctx->d_score,   ctx->d_contrib[0], ctx->d_contrib[1], ctx->d_score64,
                   ctx->d_contrib64[0], ctx->d_contrib64[1], ctx->d_err_part, ctx->d_heavy,
                   ctx->d_gcontrib, ctx->d_seen, ctx->d_levels, ctx->d_bact, ctx->d_baoff,
-                  ctx->d_bdepth};
+                  ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -81,11 +81,13 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_out_off = ctx->d_in_off = nullptr;
   ctx->d_out_nbr = ctx->d_in_nbr = nullptr;
   ctx->d_parent = ctx->d_depth = ctx->d_queue = nullptr;
-  ctx->d_qoff = ctx->d_blk = nullptr;
+  ctx->d_qoff = nullptr;
   ctx->d_bm[0] = ctx->d_bm[1] = ctx->d_bm[2] = nullptr;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
   ctx->d_heavy = nullptr;
+  ctx->d_items = ctx->d_supers = ctx->d_partial = nullptr;
+  ctx->n_items = ctx->n_super = ctx->n_slots = 0;
   ctx->d_seen = ctx->d_levels = nullptr;
   ctx->d_bact = nullptr;
   ctx->d_baoff = nullptr;
@@ -136,7 +138,7 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
   }
   if (!ctx->d_err_part) {
-    const int64_t parts = dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx);
+    const int64_t parts = dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx) + dmm::pr_super_blocks(ctx);
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
     ctx->err_part_cap = parts;
   }
@@ -261,6 +263,10 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
     return bail(rc);
   if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_bctrl, dmm::batch_ctrl_bytes(), "batch control"))) return bail(rc);
   if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_bsrc, 64 * sizeof(int32_t), "batch sources"))) return bail(rc);
+  ctx->part_cap = int64_t(std::max(1, ctx->cu_count)) * dmm::kMaxGridPerCU;
+  if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_part, sizeof(int64_t) * dmm::kPartWidth * ctx->part_cap,
+                           "workgroup partials")))
+    return bail(rc);
   if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(dmm::BfsCtrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&ctx->h_params, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_prctrl, sizeof(dmm::PrCtrl), hipHostMallocDefault) != hipSuccess ||
@@ -280,7 +286,7 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_graph_state(ctx);
   void* bufs[] = {ctx->d_ctrl, ctx->d_params, ctx->d_scratch64, ctx->d_prctrl, ctx->d_prparams,
-                  ctx->d_err_trace, ctx->d_bctrl, ctx->d_bsrc};
+                  ctx->d_err_trace, ctx->d_bctrl, ctx->d_bsrc, ctx->d_part};
   for (void* p : bufs) dmm::dev_free(p);
   void* hbufs[] = {ctx->h_ctrl, ctx->h_params, ctx->h_prctrl, ctx->h_prparams, ctx->h_bctrl, ctx->h_bsrc};
   for (void* p : hbufs)
@@ -348,17 +354,14 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
     ctx->d_in_nbr = ctx->d_out_nbr;
   }
   // BFS state.
-  const int64_t nblk = std::max<int64_t>(1, dmm::ceil_div(ctx->words, 256 * 4));
   if ((rc = A((void**)&ctx->d_parent, sizeof(int32_t) * rows, "parent")) ||
       (rc = A((void**)&ctx->d_depth, sizeof(int32_t) * rows, "depth")) ||
       (rc = A((void**)&ctx->d_queue, sizeof(int32_t) * (rows + 64), "queue")) ||
       (rc = A((void**)&ctx->d_qoff, sizeof(int64_t) * (rows + 64), "queue offsets")) ||
       (rc = A((void**)&ctx->d_bm[0], sizeof(uint64_t) * ctx->words, "bitmap")) ||
       (rc = A((void**)&ctx->d_bm[1], sizeof(uint64_t) * ctx->words, "bitmap")) ||
-      (rc = A((void**)&ctx->d_bm[2], sizeof(uint64_t) * ctx->words, "bitmap")) ||
-      (rc = A((void**)&ctx->d_blk, sizeof(int64_t) * 3 * nblk, "b2q partials")))
+      (rc = A((void**)&ctx->d_bm[2], sizeof(uint64_t) * ctx->words, "bitmap")))
     return rc;
-  ctx->blk_cap = nblk;
   // PageRank heavy-vertex list: rows with in-degree >= threshold get a whole
   // wavefront each.
   const int64_t* ioff = directed ? in_offsets : out_offsets;
@@ -369,6 +372,36 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
   if ((rc = A((void**)&ctx->d_heavy, sizeof(int32_t) * std::max<size_t>(1, heavy.size()), "heavy list"))) return rc;
   if (!heavy.empty())
     DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_heavy, heavy.data(), sizeof(int32_t) * heavy.size(), hipMemcpyHostToDevice));
+  // Work items of the compensated / FP64 modes: heavy rows whole, rows with
+  // more than kPrSplit in-edges cut into kPrSegment-edge segments.
+  std::vector<dmm::PrItem> items;
+  std::vector<dmm::PrSuper> supers;
+  int32_t slots = 0;
+  for (int32_t r : heavy) {
+    const int64_t b = ioff[r], e = ioff[r + 1];
+    if (e - b < dmm::kPrSplit) {
+      items.push_back({r, -1, b, e});
+      continue;
+    }
+    dmm::PrSuper sp{r, slots, 0, 0};
+    for (int64_t x = b; x < e; x += dmm::kPrSegment) {
+      items.push_back({r, slots++, x, std::min(e, x + dmm::kPrSegment)});
+      sp.count++;
+    }
+    supers.push_back(sp);
+  }
+  ctx->n_items = static_cast<int64_t>(items.size());
+  ctx->n_super = static_cast<int64_t>(supers.size());
+  ctx->n_slots = slots;
+  if ((rc = A(&ctx->d_items, sizeof(dmm::PrItem) * std::max<size_t>(1, items.size()), "PR items")) ||
+      (rc = A(&ctx->d_supers, sizeof(dmm::PrSuper) * std::max<size_t>(1, supers.size()), "PR split rows")) ||
+      (rc = A(&ctx->d_partial, 16 * std::max<int64_t>(1, slots), "PR partials")))
+    return rc;
+  if (!items.empty())
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_items, items.data(), sizeof(dmm::PrItem) * items.size(), hipMemcpyHostToDevice));
+  if (!supers.empty())
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_supers, supers.data(), sizeof(dmm::PrSuper) * supers.size(),
+                                 hipMemcpyHostToDevice));
   ctx->has_graph = true;
   return DMM_OK;
 }
@@ -384,6 +417,8 @@ int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* 
   if (source < 0 || source >= ctx->n)
     return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
   if (alpha <= 0 || beta <= 0) return set_err(ctx, DMM_ERR_INVALID, "alpha and beta must be positive");
+  if (ctx->nloc != ctx->n)
+    return set_err(ctx, DMM_ERR_STATE, "dmm_bfs needs the whole graph; drive a PE block with the step API");
   const double t0 = now_ms();
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (!trace) {
@@ -399,21 +434,20 @@ int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* 
     int64_t scout_count = ctx->h_ctrl->scout;
     int64_t qsize = (ctx->h_ctrl->qe - ctx->h_ctrl->qb) + ctx->h_ctrl->window_zero;
     int si = 0;
-    auto mark = [&](int i) -> int {
-      if (i < static_cast<int>(ev.size())) DMM_HIP_CHECK(ctx, hipEventRecord(ev[i], ctx->stream));
-      return DMM_OK;
-    };
     std::vector<std::pair<int, int>> step_ev;  // (begin, end) event index per step
+    // Each step's kernels are bracketed by a pair of events recorded inside
+    // the step functions (not the host-side readback that follows them).
     auto run = [&](auto fn) -> int {
       int b = si++;
-      int r = mark(b);
-      if (r) return r;
-      r = fn();
-      if (r) return r;
       int e = si++;
-      r = mark(e);
+      const bool have = e < static_cast<int>(ev.size());
+      ctx->trace_ev_begin = have ? ev[b] : nullptr;
+      ctx->trace_ev_end = have ? ev[e] : nullptr;
+      int r = fn();
+      ctx->trace_ev_begin = ctx->trace_ev_end = nullptr;
+      if (r) return r;
       step_ev.push_back({b, e});
-      return r;
+      return DMM_OK;
     };
     while (qsize != 0) {
       if (step_ev.size() + 3 >= DMM_MAX_STEPS)
@@ -557,8 +591,6 @@ __global__ void k_set_window(dmm::BfsCtrl* c, long long size, long long edges, l
   c->qin = size;
   c->window_edges = edges;
   c->window_zero = zero;
-  c->packed = 0;
-  c->zero_deg = 0;
 }
 
 int dmm_state_set_queue(dmm_ctx* ctx, const int32_t* queue, int64_t size) {

@@ -19,43 +19,43 @@ namespace dmm {
 constexpr int kSlots = 32;          // counters spread over slots (contention)
 constexpr int kCountBits = 30;      // packed reservation: count | degsum << 30
 constexpr uint64_t kCountMask = (uint64_t(1) << kCountBits) - 1;
+constexpr int kMaxGridPerCU = 32;   // upper bound of workgroups per CU
+constexpr int kPartWidth = 4;       // int64 partials per workgroup
 
 struct BfsCtrl {
-  // TD / BitmapToQueue output reservation: (degree sum << 30) | count.
-  unsigned long long packed;
-  // Claimed vertices of out-degree 0 (counted in |F| and scout, not queued).
-  unsigned long long zero_deg;
-  unsigned long long awake[kSlots];
-  unsigned long long edges[kSlots];      // edges examined (stats)
-  unsigned long long scanned[kSlots];    // vertices scanned (stats)
-  // Device-resident DOBFS loop state.
-  long long qb, qe, qin;                 // sliding-queue window and tail
-  long long window_edges;                // sum of out-degrees in the window
-  long long window_zero;                 // zero-degree members of the window
+  // Sliding-queue window [qb, qe), tail qin; the window's out-degree sum and
+  // its members of out-degree 0 (counted, not stored).
+  long long qb, qe, qin;
+  long long window_edges, window_zero;
+  // DOBFS loop state (src/bfs.cc:158-186).
   long long scout, edges_to_check, awake_count, old_awake;
+  // Results of the last frontier count, waiting for the queue write.
+  long long pend_cnt, pend_degs, pend_zero;
+  // Inputs of the last top-down step, for its trace record.
+  long long td_in, td_edges;
   int level;
-  int phase;                             // kPhase* below
-  int done;
-  int step;                              // steps taken (trace index)
-  int front;                             // index of the front bitmap (0/1)
-  int pad0;
-  // Per-step trace written by the device loop.
+  int phase;       // kPhase* below
+  int done;        // 1: loop finished, 2: cleanup done
+  int step;        // trace records written
+  int front;       // bitmap roles: front | spare << 2 | scratch << 4
+  int count_src;   // 0: count after a TD step, 1: after the BU loop
+  int error;       // != 0: a device-side invariant was violated
+  int pad;
+  // Per-step trace.
   int step_kind[DMM_MAX_STEPS];
   long long step_out[DMM_MAX_STEPS];
   long long step_in[DMM_MAX_STEPS];
   long long step_edges[DMM_MAX_STEPS];
   long long step_scout[DMM_MAX_STEPS];
-  // Block-level partial sums for BitmapToQueue.
-  unsigned int ticket;
-  unsigned int pad1;
 };
 
 enum : int {
-  kPhaseDecide = 0,  // loop head: choose TD or BU
-  kPhaseTD = 1,      // run a top-down step
-  kPhaseBU = 2,      // run a bottom-up step
-  kPhaseB2Q = 3,     // convert the BU frontier back to a queue
-  kPhaseDone = 4,
+  kPhaseIdle = 0,    // nothing scheduled (host-driven steps)
+  kPhaseTD = 1,      // top-down step: window -> claims in a bitmap
+  kPhaseBU = 2,      // bottom-up step: front bitmap -> next bitmap
+  kPhaseCount = 3,   // count the front bitmap (vertices, out-degrees)
+  kPhaseWrite = 4,   // write the counted front as the next queue window
+  kPhaseDone = 5,
 };
 
 struct PrCtrl {
@@ -69,7 +69,7 @@ struct Tuning {
   int bu_block = 256;
   int td_edges_per_block = 1024;
   int grid_per_cu = 8;                 // workgroups per CU for grid-stride steps
-  dmm_batch_params batch = {20, 256, 8, 0};
+  dmm_batch_params batch = {20, 256, 16, 0};
   double plain_ms = 0, batch_ms = 0;
 };
 
@@ -80,6 +80,8 @@ struct dmm_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   std::vector<hipEvent_t> step_events;  // per-step timing in trace mode
+  hipEvent_t trace_ev_begin = nullptr;  // set while a traced step runs
+  hipEvent_t trace_ev_end = nullptr;
   std::string err;
   std::string arch;
   int cu_count = 0;
@@ -105,8 +107,8 @@ struct dmm_ctx {
   void* d_params = nullptr;      // BfsParams + num_edges_directed
   void* h_params = nullptr;      // pinned staging for d_params
   int64_t* d_scratch64 = nullptr;  // small reduction outputs
-  int64_t* d_blk = nullptr;      // BitmapToQueue block partials
-  int64_t blk_cap = 0;
+  int64_t* d_part = nullptr;     // per-workgroup partials (kPartWidth each)
+  int64_t part_cap = 0;          // workgroups the partials can hold
   dmm::BfsCtrl* d_ctrl = nullptr;
   dmm::BfsCtrl* h_ctrl = nullptr;  // pinned mirror
   // Host mirror of the sliding queue indices for the step API.
@@ -129,6 +131,12 @@ struct dmm_ctx {
   dmm::PrCtrl* h_prctrl = nullptr;
   int32_t* d_heavy = nullptr;    // vertices with in-degree >= heavy threshold
   int64_t n_heavy = 0;
+  void* d_items = nullptr;       // PrItem list (compensated / FP64 modes)
+  int64_t n_items = 0;
+  void* d_supers = nullptr;      // PrSuper list
+  int64_t n_super = 0;
+  void* d_partial = nullptr;     // double2 per segment
+  int64_t n_slots = 0;
   int heavy_threshold = 64;
   float* d_gcontrib = nullptr;   // PE-share: global contribution vector
   double* d_err_trace = nullptr; // per-iteration L1 change

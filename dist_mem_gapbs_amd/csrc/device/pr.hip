@@ -47,10 +47,13 @@ struct PrArgs {
   const int32_t* in_nbr;          // global IDs
   const int64_t* out_off;         // local rows (out-degree for contrib)
   const int32_t* heavy;           // local rows with in-degree >= kHeavy
-  int64_t nloc, n_heavy;
+  const PrItem* items;            // heavy rows / segments of split rows
+  const PrSuper* supers;          // split rows and their segment slots
+  double2* partial;               // per-segment partial sums
+  int64_t nloc, n_heavy, n_items, n_super;
   int heavy_threshold;
-  double* err_part;               // [light blocks | heavy blocks]
-  int light_blocks, heavy_blocks;
+  double* err_part;               // [light | heavy-or-items | super] blocks
+  int light_blocks, heavy_blocks, super_blocks;
   PrCtrl* ctrl;
   double* err_trace;              // per-iteration error
   const PrParams* prm;
@@ -203,7 +206,13 @@ __global__ __launch_bounds__(kThreads) void k_kahan_light(PrArgs a, const float*
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kThreads) void k_kahan_heavy(PrArgs a, const float* __restrict__ cin,
+
+// Heavy rows (in-degree in [64, split)) and the segments of split rows, one
+// wavefront per item: lanes stride the item's edge range (coalesced ID
+// loads, four gathers in flight), then a fixed butterfly merges the
+// 64 compensated partials.  A whole row is finished in place; a segment
+// stores its (sum, error) pair for the combine kernel.
+__global__ __launch_bounds__(kThreads) void k_kahan_items(PrArgs a, const float* __restrict__ cin,
                                                           float* __restrict__ cout,
                                                           float* __restrict__ score) {
   __shared__ double red[kThreads / 64];
@@ -212,12 +221,11 @@ __global__ __launch_bounds__(kThreads) void k_kahan_heavy(PrArgs a, const float*
   const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
   double err = 0;
-  for (int64_t h = wave; h < a.n_heavy; h += nwaves) {
-    const int64_t u = a.heavy[h];
-    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+  for (int64_t h = wave; h < a.n_items; h += nwaves) {
+    const PrItem it = a.items[h];
     Comp acc{0.0f, 0.0f};
-    int64_t k = s + lane;
-    for (; k + 192 < e; k += 256) {
+    int64_t k = it.b + lane;
+    for (; k + 192 < it.e; k += 256) {
       const float x0 = cin[a.in_nbr[k]];
       const float x1 = cin[a.in_nbr[k + 64]];
       const float x2 = cin[a.in_nbr[k + 128]];
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_kahan_heavy(PrArgs a, const float*
       acc.add(x2);
       acc.add(x3);
     }
-    for (; k < e; k += 64) acc.add(cin[a.in_nbr[k]]);
+    for (; k < it.e; k += 64) acc.add(cin[a.in_nbr[k]]);
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
       const float s2 = __shfl_xor(acc.s, d, 64);
@@ -240,10 +248,77 @@ __global__ __launch_bounds__(kThreads) void k_kahan_heavy(PrArgs a, const float*
         acc.merge(s2, c2);
       }
     }
-    if (lane == 0) err += finish_f32(a, u, acc.value(), score, cout);
+    if (lane == 0) {
+      if (it.slot < 0)
+        err += finish_f32(a, it.v, acc.value(), score, cout);
+      else
+        a.partial[it.slot] = make_double2(acc.s, acc.c);
+    }
   }
   double t = block_err_sum(err, red);
   if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+}
+
+// Merges each split row's segment partials in segment order and finishes it.
+__global__ __launch_bounds__(kThreads) void k_kahan_super(PrArgs a, float* __restrict__ cout,
+                                                          float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  double err = 0;
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < a.n_super) {
+    const PrSuper sp = a.supers[i];
+    const double2 p0 = a.partial[sp.first];
+    Comp acc{static_cast<float>(p0.x), static_cast<float>(p0.y)};
+    for (int j = 1; j < sp.count; j++) {
+      const double2 p = a.partial[sp.first + j];
+      acc.merge(static_cast<float>(p.x), static_cast<float>(p.y));
+    }
+    err = finish_f32(a, sp.v, acc.value(), score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + a.heavy_blocks + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_f64_items(PrArgs a, const double* __restrict__ cin,
+                                                        double* __restrict__ cout,
+                                                        double* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  double err = 0;
+  for (int64_t h = wave; h < a.n_items; h += nwaves) {
+    const PrItem it = a.items[h];
+    double acc = 0;
+    for (int64_t k = it.b + lane; k < it.e; k += 64) acc += cin[a.in_nbr[k]];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      if (it.slot < 0)
+        err += finish_f64(a, it.v, acc, score, cout);
+      else
+        a.partial[it.slot] = make_double2(acc, 0.0);
+    }
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_f64_super(PrArgs a, double* __restrict__ cout,
+                                                        double* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  double err = 0;
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < a.n_super) {
+    const PrSuper sp = a.supers[i];
+    double acc = 0;
+    for (int j = 0; j < sp.count; j++) acc += a.partial[sp.first + j].x;
+    err = finish_f64(a, sp.v, acc, score, cout);
+  }
+  double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.light_blocks + a.heavy_blocks + blockIdx.x] = t;
 }
 
 // ---- FP64 -----------------------------------------------------------------
@@ -277,26 +352,6 @@ __global__ __launch_bounds__(kThreads) void k_f64_light(PrArgs a, const double* 
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kThreads) void k_f64_heavy(PrArgs a, const double* __restrict__ cin,
-                                                        double* __restrict__ cout,
-                                                        double* __restrict__ score) {
-  __shared__ double red[kThreads / 64];
-  if (a.ctrl->pad) return;
-  const int lane = lane_id();
-  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
-  double err = 0;
-  for (int64_t h = wave; h < a.n_heavy; h += nwaves) {
-    const int64_t u = a.heavy[h];
-    const int64_t s = a.in_off[u], e = a.in_off[u + 1];
-    double acc = 0;
-    for (int64_t k = s + lane; k < e; k += 64) acc += cin[a.in_nbr[k]];
-    acc = wave_sum(acc);
-    if (lane == 0) err += finish_f64(a, u, acc, score, cout);
-  }
-  double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
-}
 
 // ---- iteration bookkeeping --------------------------------------------------
 
@@ -308,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(PrArgs a) {
   __shared__ double red[kThreads / 64];
   PrCtrl* c = a.ctrl;
   if (c->pad) return;
-  const int np = a.light_blocks + a.heavy_blocks;
+  const int np = a.light_blocks + a.heavy_blocks + a.super_blocks;
   double x = 0;
   for (int i = threadIdx.x; i < np; i += kThreads) x += a.err_part[i];
   double t = block_err_sum(x, red);
@@ -351,6 +406,12 @@ __global__ __launch_bounds__(kThreads) void k_init_f64(PrArgs a, double* score, 
   }
 }
 
+// The serial mode has no combine step: clear the combine blocks' partials.
+__global__ void k_zero_err(PrArgs a) {
+  for (int i = threadIdx.x; i < a.super_blocks; i += blockDim.x)
+    a.err_part[a.light_blocks + a.heavy_blocks + i] = 0.0;
+}
+
 __global__ void k_reset_ctrl(PrCtrl* c) {
   c->pad = 0;
   c->ticket = 0;
@@ -363,12 +424,18 @@ PrArgs make_args(dmm_ctx* ctx) {
   a.in_nbr = ctx->d_in_nbr;
   a.out_off = ctx->d_out_off;
   a.heavy = ctx->d_heavy;
+  a.items = static_cast<const PrItem*>(ctx->d_items);
+  a.supers = static_cast<const PrSuper*>(ctx->d_supers);
+  a.partial = static_cast<double2*>(ctx->d_partial);
   a.nloc = ctx->nloc;
   a.n_heavy = ctx->n_heavy;
+  a.n_items = ctx->n_items;
+  a.n_super = ctx->n_super;
   a.heavy_threshold = ctx->heavy_threshold;
   a.err_part = ctx->d_err_part;
   a.light_blocks = pr_light_blocks(ctx);
   a.heavy_blocks = pr_heavy_blocks(ctx);
+  a.super_blocks = pr_super_blocks(ctx);
   a.ctrl = ctx->d_prctrl;
   a.err_trace = ctx->d_err_trace;
   a.prm = reinterpret_cast<const PrParams*>(ctx->d_prparams);
@@ -378,20 +445,24 @@ PrArgs make_args(dmm_ctx* ctx) {
 void launch_iteration(dmm_ctx* ctx, const PrArgs& a, int mode, int it, const void* gcontrib) {
   const int lb = a.light_blocks, hb = a.heavy_blocks;
   hipStream_t s = ctx->stream;
+  const int sb = a.super_blocks;
   if (mode == DMM_PR_FP64) {
     const double* cin = gcontrib ? static_cast<const double*>(gcontrib) : ctx->d_contrib64[it & 1];
     double* cout = ctx->d_contrib64[(it + 1) & 1];
     hipLaunchKernelGGL(k_f64_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
-    hipLaunchKernelGGL(k_f64_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
+    hipLaunchKernelGGL(k_f64_items, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
+    hipLaunchKernelGGL(k_f64_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score64);
   } else {
     const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
     float* cout = ctx->d_contrib[(it + 1) & 1];
     if (mode == DMM_PR_FP32_SERIAL) {
       hipLaunchKernelGGL(k_serial_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
       hipLaunchKernelGGL(k_serial_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(kThreads), 0, s, a);
     } else {
       hipLaunchKernelGGL(k_kahan_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_kahan_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_kahan_items, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_kahan_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score);
     }
   }
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, s, a);
@@ -405,9 +476,13 @@ int pr_light_blocks(dmm_ctx* ctx) {
 }
 
 int pr_heavy_blocks(dmm_ctx* ctx) {
+  const int64_t work = std::max(ctx->n_heavy, ctx->n_items);
   return static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8,
-                           ceil_div(ctx->n_heavy * 64, kThreads))));
+      1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8, ceil_div(work * 64, kThreads))));
+}
+
+int pr_super_blocks(dmm_ctx* ctx) {
+  return static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_super, kThreads)));
 }
 
 int pr_run(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, dmm_pr_stats* stats) {

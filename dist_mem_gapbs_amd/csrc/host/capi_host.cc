@@ -149,3 +149,27 @@ extern "C" int dmm_partition(int64_t num_nodes, int32_t npes, int32_t pe, int64_
   if (max_width) *max_width = p.max_width;
   return DMM_OK;
 }
+
+extern "C" int dmm_rng_sequence(int32_t kind, int64_t seed, int64_t param, int64_t count, int64_t* out) {
+  if (count < 0 || (count > 0 && !out)) return fail(nullptr, DMM_ERR_INVALID, "bad output buffer");
+  dmm::Engine rng(static_cast<dmm::Engine::result_type>(seed));
+  if (kind == 0) {
+    if (param < 1 || param > 0xFFFFFFFFll) return fail(nullptr, DMM_ERR_INVALID, "range outside [1, 2^32)");
+    for (int64_t i = 0; i < count; i++) out[i] = dmm::uniform_below(rng, static_cast<uint32_t>(param));
+  } else if (kind == 1) {
+    for (int64_t i = 0; i < count; i++) {
+      float f = dmm::uniform_unit_float(rng);
+      uint32_t b;
+      std::memcpy(&b, &f, sizeof(b));
+      out[i] = b;
+    }
+  } else if (kind == 2) {
+    std::vector<int32_t> p(count);
+    for (int64_t i = 0; i < count; i++) p[i] = static_cast<int32_t>(i);
+    dmm::shuffle_range(p.data(), count, rng);
+    for (int64_t i = 0; i < count; i++) out[i] = p[i];
+  } else {
+    return fail(nullptr, DMM_ERR_INVALID, "unknown kind %d", kind);
+  }
+  return DMM_OK;
+}

@@ -85,14 +85,17 @@ class Context:
         return b.value
 
     # ---- DOBFS -----------------------------------------------------------------
-    def DOBFS(self, source, alpha=15, beta=18, trace=False, want_depth=True, read=True):
+    def DOBFS(self, source, alpha=15, beta=18, trace=False, want_depth=True, read=True,
+              with_stats=True):
         """Returns (parent, depth, stats).  parent/depth are None if read=False
-        (the result stays in HBM)."""
+        (the result stays in HBM); stats is None if with_stats=False (the call
+        then does nothing beyond the BFS itself)."""
         parent = np.empty(self.nloc, np.int32) if read else None
         depth = np.empty(self.nloc, np.int32) if (read and want_depth) else None
-        st = L.BfsStats()
+        st = L.BfsStats() if with_stats else None
         self._check(self._lib.dmm_bfs(self._h, int(source), alpha, beta, L.ptr(parent, C.c_int32),
-                                      L.ptr(depth, C.c_int32), 1 if trace else 0, C.byref(st)))
+                                      L.ptr(depth, C.c_int32), 1 if trace else 0,
+                                      C.byref(st) if st is not None else None))
         return parent, depth, st
 
     def bfs_read(self, parent=True, depth=True):

@@ -99,6 +99,14 @@ void dmm_host_graph_free(dmm_host_graph* g);
 int dmm_pick_sources(int64_t num_nodes, const int64_t* out_offsets,
                      int32_t given_source, int32_t count, int32_t* out);
 
+/* The generator's explicit random-number mappings, exposed so they can be
+ * pinned against the C++ library's own results (tests/golden/std_kats.json):
+ *   kind 0: `count` draws uniform in [0, param) from mt19937(seed)
+ *   kind 1: `count` uniform floats in [0, 1) from mt19937(seed), as IEEE bits
+ *   kind 2: the identity permutation of length count shuffled by mt19937(seed)
+ * No counterpart in the reference (which calls std:: distributions). */
+int dmm_rng_sequence(int32_t kind, int64_t seed, int64_t param, int64_t count, int64_t* out);
+
 /* Replaces Partition<T>(N) for PE pe of npes (src/partition.h:79-106). */
 int dmm_partition(int64_t num_nodes, int32_t npes, int32_t pe,
                   int64_t* start, int64_t* end, int64_t* partition_width,

@@ -74,11 +74,13 @@ class CSR:
 
     @classmethod
     def of(cls, g):
-        """From a dist_mem_gapbs_amd HostGraph."""
+        """From a dist_mem_gapbs_amd HostGraph (arrays copied: the CSR must
+        outlive the HostGraph, whose arrays are views of native memory)."""
         if g.directed:
-            return cls(g.num_nodes, g.out_offsets, g.out_neighs, g.in_offsets, g.in_neighs,
-                       g.num_edges_directed)
-        return cls(g.num_nodes, g.out_offsets, g.out_neighs, num_edges_directed=g.num_edges_directed)
+            return cls(g.num_nodes, g.out_offsets.copy(), g.out_neighs.copy(), g.in_offsets.copy(),
+                       g.in_neighs.copy(), g.num_edges_directed)
+        return cls(g.num_nodes, g.out_offsets.copy(), g.out_neighs.copy(),
+                   num_edges_directed=g.num_edges_directed)
 
 
 def init_parent(g, source):

@@ -1,0 +1,111 @@
+"""GPU parity of the batched 64-source BFS: every lane's depths equal the
+single-source BFS depths from its source (oracle serial BFS), including the
+uneven batch (a disjoint union of Kronecker graphs of scales 12..18)."""
+import numpy as np
+import pytest
+
+from conftest import csr_from_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def expected(orc, og, srcs):
+    out = np.empty((len(srcs), og.n), np.uint8)
+    for i, s in enumerate(srcs):
+        d = orc.bfs_depth(og, int(s))
+        out[i] = np.where(d < 0, 255, d)
+    return out
+
+
+@pytest.mark.parametrize("scale,degree,uniform", [(14, 16, False), (14, 8, True), (17, 16, False)])
+def test_batch64_depths(ctx, dmm, orc, graph, scale, degree, uniform):
+    g = graph(scale, degree, uniform)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = dmm.pick_sources(g, 64)
+    depth, st = ctx.bfs_batch64(srcs)
+    assert np.array_equal(depth, expected(orc, og, srcs))
+    # GTEPS numerator: sum over lanes of the reached out-degrees
+    deg = np.diff(og.oo)
+    assert st.reached_edges == int(sum(deg[expected(orc, og, [s])[0] != 255].sum() for s in srcs))
+
+
+@pytest.mark.parametrize("td_div", [1, 4, 20, 10 ** 6])
+def test_batch64_direction_settings_same_depths(ctx, dmm, orc, graph, td_div):
+    g = graph(13, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = dmm.pick_sources(g, 64)
+    depth, _ = ctx.bfs_batch64(srcs, td_div=td_div, graph_levels=6)
+    assert np.array_equal(depth, expected(orc, og, srcs))
+
+
+def test_batch_partial_and_duplicate_lanes(ctx, dmm, orc, graph):
+    g = graph(13, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = dmm.pick_sources(g, 5)
+    srcs = np.array([srcs[0], srcs[1], srcs[0], srcs[2], srcs[3], srcs[4], srcs[1]], np.int32)
+    depth, _ = ctx.bfs_batch64(srcs)
+    assert depth.shape == (7, og.n)
+    assert np.array_equal(depth, expected(orc, og, srcs))
+    d1, _ = ctx.bfs_batch64(srcs[:1])
+    assert np.array_equal(d1, expected(orc, og, srcs[:1]))
+
+
+def test_batch_zero_degree_and_deep(ctx, dmm, orc):
+    n = 250  # depths up to 249: within the uint8 depth range (255 = unreached)
+    pairs = [[i, i + 1] for i in range(n - 1)]
+    g = csr_from_pairs(dmm, pairs, n + 3)  # vertices n..n+2 isolated
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = np.array([0, 125, n + 1, n - 1], np.int32)
+    depth, st = ctx.bfs_batch64(srcs)
+    assert np.array_equal(depth, expected(orc, og, srcs))
+    assert st.num_levels >= n - 1
+
+
+def test_batch_too_deep_is_an_error(ctx, dmm):
+    g = csr_from_pairs(dmm, [[i, i + 1] for i in range(299)])
+    ctx.upload(g)
+    with pytest.raises(dmm.DmmError) as e:
+        ctx.bfs_batch64(np.array([0], np.int32))
+    assert e.value.code == -5  # DMM_ERR_CAPACITY: depths beyond 254
+
+
+def test_batch_directed(ctx, dmm, orc):
+    rng = np.random.default_rng(11)
+    pairs = rng.integers(0, 3000, size=(15000, 2))
+    g = dmm.from_edgelist(pairs, 3000, symmetrize=False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = dmm.pick_sources(g, 64)
+    depth, _ = ctx.bfs_batch64(srcs)
+    assert np.array_equal(depth, expected(orc, og, srcs))
+
+
+def test_uneven_batch_disjoint_union(ctx, dmm, orc):
+    """32 Kronecker graphs, scales drawn from 12..18 (mt19937(kRandSeed),
+    uniform over [12, 18]), graph i relabelled into its own ID range; one
+    source per graph (its first SourcePicker draw) in lanes 0..31."""
+    import ctypes as C
+    lib = dmm.load()
+    draws = np.empty(32, np.int64)
+    lib.dmm_rng_sequence(0, 27491095, 7, 32, draws.ctypes.data_as(C.POINTER(C.c_int64)))
+    scales = (12 + draws).tolist()
+    offs, nbrs, srcs = [np.zeros(1, np.int64)], [], []
+    base_v, base_e = 0, 0
+    for sc in scales:
+        gi = dmm.build_graph(sc, 16, False)
+        srcs.append(base_v + int(dmm.pick_sources(gi, 1)[0]))
+        offs.append(gi.out_offsets[1:] + base_e)
+        nbrs.append(gi.out_neighs + base_v)
+        base_v += gi.num_nodes
+        base_e += int(gi.out_offsets[-1])
+    oo = np.concatenate(offs)
+    on = np.concatenate(nbrs).astype(np.int32)
+    og = orc.CSR(base_v, oo, on)
+    ctx.upload(num_nodes=base_v, out_offsets=oo, out_neighs=on, num_edges_directed=base_e)
+    srcs = np.array(srcs, np.int32)
+    depth, st = ctx.bfs_batch64(srcs)
+    assert np.array_equal(depth, expected(orc, og, srcs))

@@ -1,0 +1,119 @@
+"""GPU parity of PageRankPull (src/pr.cc:41-70).
+
+FP32_SERIAL reproduces the reference's arithmetic bit for bit (same scores,
+same iteration count as the oracle's restatement, which is itself bit-exact
+against the reference's PageRankPull); FP32_KAHAN is within FP32 rounding of
+the FP64 oracle (max relative error < 4e-7, absolute < 1e-6 per vertex);
+FP64 agrees with the FP64 oracle to 1e-13 relative."""
+import numpy as np
+import pytest
+
+from conftest import csr_from_pairs
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"serial": 0, "kahan": 1, "fp64": 2}
+
+
+@pytest.mark.parametrize("scale,degree,uniform", [(12, 16, False), (15, 8, True), (17, 16, False)])
+def test_pr_serial_bit_exact(ctx, orc, graph, scale, degree, uniform):
+    g = graph(scale, degree, uniform)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    for eps in (1e-4, 0.0):
+        s, st = ctx.PageRankPull(20, eps, MODES["serial"])
+        ref, it, tr = orc.pr_f32(og, 20, eps)
+        assert st.iters == it
+        assert np.array_equal(s, ref)
+        # FP64 error sums agree to rounding (different summation order)
+        assert np.allclose(np.array(st.err[:it]), tr, rtol=1e-9, atol=1e-18)
+
+
+@pytest.mark.parametrize("scale,degree,uniform", [(12, 16, False), (17, 16, False), (15, 8, True)])
+def test_pr_kahan_and_fp64_vs_oracle(ctx, orc, graph, scale, degree, uniform):
+    g = graph(scale, degree, uniform)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    ref64, it64, _ = orc.pr_f64(og, 20, 0.0)
+    sk, stk = ctx.PageRankPull(20, 0.0, MODES["kahan"])
+    s64, st64 = ctx.PageRankPull(20, 0.0, MODES["fp64"])
+    assert stk.iters == 20 and st64.iters == 20
+    dk = np.abs(sk.astype(np.float64) - ref64)
+    assert dk.max() < 1e-6
+    assert (dk / ref64).max() < 4e-7
+    assert (np.abs(s64 - ref64) / ref64).max() < 1e-13
+    # compensated sums beat the reference's plain FP32 order on relative error
+    ss, _ = ctx.PageRankPull(20, 0.0, MODES["serial"])
+    ds = np.abs(ss.astype(np.float64) - ref64) / ref64
+    assert (dk / ref64).max() <= ds.max()
+
+
+def test_pr_stop_rule_matches(ctx, orc, graph):
+    g = graph(14, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    for eps in (1e-2, 1e-4, 1e-6):
+        _, st = ctx.PageRankPull(50, eps, MODES["kahan"])
+        _, it, _ = orc.pr_f64(og, 50, eps)
+        assert st.iters == it
+
+
+def test_pr_deterministic(ctx, graph):
+    g = graph(15, 16, False)
+    ctx.upload(g)
+    for mode in MODES.values():
+        a, _ = ctx.PageRankPull(10, 0.0, mode)
+        b, _ = ctx.PageRankPull(10, 0.0, mode)
+        assert np.array_equal(a, b)
+
+
+def test_pr_residual_verifier(ctx, orc, graph):
+    g = graph(14, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    s, _ = ctx.PageRankPull(60, 0.0, MODES["kahan"])
+    assert orc.pr_residual(og, s) < 1e-6
+
+
+def test_pr_small_graphs(ctx, dmm, orc):
+    for pairs, n in (([[0, 1]], 3), ([[0, i] for i in range(1, 5000)], -1), ([[i, i + 1] for i in range(99)], -1)):
+        g = csr_from_pairs(dmm, pairs, n)
+        og = orc.CSR.of(g)
+        ctx.upload(g)
+        s, st = ctx.PageRankPull(20, 1e-4, MODES["serial"])
+        ref, it, _ = orc.pr_f32(og, 20, 1e-4)
+        assert st.iters == it and np.array_equal(s, ref)
+        sk, _ = ctx.PageRankPull(20, 0.0, MODES["kahan"])
+        r64, _, _ = orc.pr_f64(og, 20, 0.0)
+        assert (np.abs(sk - r64) / r64).max() < 4e-7
+
+
+def test_pr_directed(ctx, dmm, orc):
+    rng = np.random.default_rng(3)
+    pairs = rng.integers(0, 4000, size=(30000, 2))
+    g = dmm.from_edgelist(pairs, 4000, symmetrize=False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    s, st = ctx.PageRankPull(20, 1e-4, MODES["serial"])
+    ref, it, _ = orc.pr_f32(og, 20, 1e-4)
+    assert st.iters == it and np.array_equal(s, ref)
+
+
+def test_pr_pe_block_iteration(ctx, dmm, orc, graph):
+    """One PE block (np = 8, pe = 3) given the global contributions: one
+    iteration equals the full-graph iteration restricted to the block."""
+    hg = graph(14, 16, False)
+    g = orc.CSR.of(hg)
+    n = g.n
+    s1, _, _ = orc.pr_f32(g, 1, 0.0)
+    s2, _, _ = orc.pr_f32(g, 2, 0.0)
+    deg = np.diff(g.oo).astype(np.float32)
+    with np.errstate(divide="ignore"):
+        contrib = (s1 / deg).astype(np.float32)
+    p = dmm.partition(n, 8, 3)
+    b, e = p["start"], p["end"]
+    ctx.upload(num_nodes=n, out_offsets=g.oo[b:e + 1] - g.oo[b], out_neighs=g.on[g.oo[b]:g.oo[e]],
+               vbegin=b, vend=e, num_edges_directed=g.ned)
+    got, err, _ = ctx.pr_block_iteration(contrib, s1[b:e], MODES["serial"])
+    assert np.array_equal(got, s2[b:e])
+    assert err == pytest.approx(float(np.abs(s2[b:e].astype(np.float64) - s1[b:e]).sum()), rel=1e-9)
