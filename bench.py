@@ -174,6 +174,34 @@ def kron24_leg(dmm, args, tuned):
     return out
 
 
+def only_leg(dmm, args):
+    """One leg on Kronecker --big-scale, printed as JSON (profiling runs)."""
+    import dist_mem_gapbs_amd._lib as L
+    g = dmm.build_graph(args.big_scale, 16, uniform=False)
+    srcs = dmm.pick_sources(g, 64)
+    out = {"leg": args.only, "scale": args.big_scale}
+    with dmm.Context(0) as ctx:
+        ctx.upload(g)
+        if args.only == "bfs":
+            ms = []
+            for s in srcs[:args.steps]:
+                _, _, st = ctx.DOBFS(int(s), read=False)
+                ms.append(st.total_ms)
+            out["dobfs_ms"] = ms
+        elif args.only == "batch":
+            for _ in range(max(1, args.warmup)):
+                ctx.bfs_batch64(srcs, want_depth=False)
+            _, st = ctx.bfs_batch64(srcs, want_depth=False)
+            out["batch_ms"] = st.total_ms
+            out["levels"] = [(st.level_kind[i], st.level_active[i]) for i in range(st.num_levels - 1)]
+            out["sweeps"] = st.sweeps
+        else:
+            for mode in (L.DMM_PR_FP32_KAHAN, L.DMM_PR_FP32_SERIAL, L.DMM_PR_FP64):
+                _, pst = ctx.PageRankPull(20, 0.0, mode, read=False)
+                out[f"pr_mode{mode}_iter_ms"] = pst.iter_ms
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +212,8 @@ def main():
     ap.add_argument("--no-big", action="store_true", help="skip the Kronecker-24 leg")
     ap.add_argument("--cpu-sources", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["bfs", "batch", "pr"],
+                    help="run one leg alone on Kronecker --big-scale (for profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -193,6 +223,10 @@ def main():
 
     import dist_mem_gapbs_amd as dmm
     from oracle import oracle as orc
+
+    if args.only:
+        only_leg(dmm, args)
+        return
 
     tuned = None
     tp = os.path.join(ROOT, "dist_mem_gapbs_amd", "tuning.json")

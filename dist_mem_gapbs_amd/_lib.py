@@ -29,6 +29,9 @@ DMM_STEP_BU = 1
 DMM_STEP_QB = 2
 DMM_STEP_BQ = 3
 
+DMM_LAYOUT_NATURAL = 0
+DMM_LAYOUT_DEGREE = 1
+
 DMM_PR_FP32_SERIAL = 0
 DMM_PR_FP32_KAHAN = 1
 DMM_PR_FP64 = 2
@@ -133,6 +136,7 @@ SIGNATURES = {
     "dmm_ctx_destroy": (None, [_P]),
     "dmm_graph_upload": (_I32, [_P, _I64, _I64, _I64, _PI64, _PI32, _PI64, _PI32, _I64]),
     "dmm_ctx_device_bytes": (_I32, [_P, _PI64]),
+    "dmm_ctx_set_layout": (_I32, [_P, _I32]),
     "dmm_bfs": (_I32, [_P, _I32, _I32, _I32, _PI32, _PI32, _I32, C.POINTER(BfsStats)]),
     "dmm_bfs_read": (_I32, [_P, _PI32, _PI32]),
     "dmm_step_init": (_I32, [_P, _I32]),

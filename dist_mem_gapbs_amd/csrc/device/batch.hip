@@ -463,11 +463,13 @@ __global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
 // Depth output: per vertex, the level bits of each lane are gathered into
 // eight bit-slice masks (bit j of the depth), then lane s's byte is written
 // to depth[s * nloc + u]; consecutive threads write consecutive bytes.
-__global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int nsrc) {
+// perm (may be NULL) maps the caller's vertex ID v to the internal row.
+__global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int nsrc, const int32_t* perm) {
   const int nlev = a.ctrl->nlevels;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t u = gtid; u < a.nloc; u += gsize) {
+  for (int64_t v = gtid; v < a.nloc; v += gsize) {
+    const int64_t u = perm ? perm[v] : v;
     uint64_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t reached = 0;
     for (int L = 0; L < nlev; L++) {
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int 
         uint32_t d = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) d |= static_cast<uint32_t>((m[j] >> s) & 1ull) << j;
-        out[int64_t(s) * a.nloc + u] = ((reached >> s) & 1ull) ? static_cast<uint8_t>(d) : 255;
+        out[int64_t(s) * a.nloc + v] = ((reached >> s) & 1ull) ? static_cast<uint8_t>(d) : 255;
       }
     }
   }
@@ -653,7 +655,8 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
         if (rc) return rc;
         ctx->bdepth_cap = int64_t(nsrc) * ctx->nloc;
       }
-      hipLaunchKernelGGL(k_bdepth, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, ctx->d_bdepth, nsrc);
+      hipLaunchKernelGGL(k_bdepth, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, ctx->d_bdepth, nsrc,
+                         ctx->relabeled ? static_cast<const int32_t*>(ctx->d_perm) : nullptr);
       DMM_HIP_CHECK(ctx, hipGetLastError());
       DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_bdepth, int64_t(nsrc) * ctx->nloc, hipMemcpyDeviceToHost,
                                         ctx->stream));

@@ -28,7 +28,7 @@ int bfs_finish(dmm_ctx* ctx);
 
 // Whole DOBFS with the level loop decided on the device (no host round
 // trip per level).  Captured once per ctx into a hipGraph.
-int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta,
+int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool with_depth,
                     dmm_bfs_stats* stats);
 
 // Reached vertices / edges of the current parent array (stats only).

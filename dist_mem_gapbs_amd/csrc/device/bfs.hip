@@ -39,6 +39,8 @@
 
 #include <algorithm>
 
+#include "layout.h"
+
 namespace dmm {
 
 namespace {
@@ -632,7 +634,7 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   for (int64_t i = gtid; i < n; i += gsize) p[i] = v;
 }
 
-StepArgs make_args(dmm_ctx* ctx) {
+StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   StepArgs a;
   a.out_off = ctx->d_out_off;
   a.out_nbr = ctx->d_out_nbr;
@@ -645,7 +647,7 @@ StepArgs make_args(dmm_ctx* ctx) {
   a.wbegin = ctx->vbegin >> 6;
   a.wend = (ctx->vend + 63) >> 6;
   a.parent = ctx->d_parent;
-  a.depth = ctx->d_depth;
+  a.depth = with_depth ? ctx->d_depth : nullptr;
   a.queue = ctx->d_queue;
   a.qoff = ctx->d_qoff;
   a.bm0 = ctx->d_bm[0];
@@ -802,6 +804,8 @@ int bfs_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size) {
 int bfs_finish(dmm_ctx* ctx) {
   StepArgs a = make_args(ctx);
   hipLaunchKernelGGL(k_finish, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, 0);
+  if (ctx->relabeled) launch_map_parent(ctx, true, nullptr);
+  ctx->depth_valid = true;
   DMM_HIP_CHECK(ctx, hipGetLastError());
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DMM_OK;
@@ -823,8 +827,8 @@ int bfs_reach_stats(dmm_ctx* ctx, int64_t* reached, int64_t* reached_edges) {
 
 // ---- device-resident loop ------------------------------------------------
 
-static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, hipGraphExec_t* out) {
-  StepArgs a = make_args(ctx);
+static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth, hipGraphExec_t* out) {
+  StepArgs a = make_args(ctx, with_depth);
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
   hipGraph_t g = nullptr;
   DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -835,6 +839,8 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, hipGraphExec_t*
   }
   for (int i = 0; i < nsteps; i++) launch_pair<-1>(ctx, a, prm, blocks);
   hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, 1);
+  // Results back to the caller's IDs (a no-op until the loop is done).
+  if (ctx->relabeled) launch_map_parent(ctx, with_depth, &ctx->d_ctrl->done);
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -843,36 +849,39 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, hipGraphExec_t*
   return DMM_OK;
 }
 
-int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, dmm_bfs_stats* stats) {
+int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool with_depth,
+                    dmm_bfs_stats* stats) {
   constexpr int kCont = 6;
+  const int d = with_depth ? 1 : 0;
   const int want = std::max(8, ctx->bfs_chain_len);
-  if (!ctx->bfs_graph || ctx->bfs_graph_steps != want) {
-    if (ctx->bfs_graph) (void)hipGraphExecDestroy(ctx->bfs_graph);
-    ctx->bfs_graph = nullptr;
-    int rc = build_chain(ctx, want, true, &ctx->bfs_graph);
+  if (!ctx->bfs_graph[d] || ctx->bfs_graph_steps[d] != want) {
+    if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
+    ctx->bfs_graph[d] = nullptr;
+    int rc = build_chain(ctx, want, true, with_depth, &ctx->bfs_graph[d]);
     if (rc) return rc;
-    ctx->bfs_graph_steps = want;
+    ctx->bfs_graph_steps[d] = want;
   }
-  if (!ctx->bfs_cont) {
-    int rc = build_chain(ctx, kCont, false, &ctx->bfs_cont);
+  if (!ctx->bfs_cont[d]) {
+    int rc = build_chain(ctx, kCont, false, with_depth, &ctx->bfs_cont[d]);
     if (rc) return rc;
   }
   int rc = write_params(ctx, source, alpha, beta);
   if (rc) return rc;
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
-  DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_graph, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_graph[d], ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
   rc = read_ctrl_head(ctx);
   if (rc) return rc;
   int rounds = 0;
   while (ctx->h_ctrl->done != 2) {
     if (++rounds > (1 << 20)) return set_err(ctx, DMM_ERR_STATE, "device BFS loop did not terminate");
-    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_cont, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_cont[d], ctx->stream));
     DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
     rc = read_ctrl_head(ctx);
     if (rc) return rc;
   }
   ctx->bfs_valid = true;
+  ctx->depth_valid = with_depth;
   // Grow the first chain toward the phase count seen, so a typical BFS
   // finishes in one graph launch.
   if (rounds > 0) ctx->bfs_chain_len = std::min(128, want + rounds * kCont);

@@ -13,6 +13,7 @@
 #include "batch.h"
 #include "bfs.h"
 #include "common.h"
+#include "layout.h"
 #include "pr.h"
 
 namespace dmm {
@@ -57,22 +58,30 @@ double now_ms() {
       .count();
 }
 
-void free_graph_state(dmm_ctx* ctx) {
-  if (ctx->bfs_graph) (void)hipGraphExecDestroy(ctx->bfs_graph);
-  if (ctx->bfs_cont) (void)hipGraphExecDestroy(ctx->bfs_cont);
+void drop_graphs(dmm_ctx* ctx) {
+  for (int d = 0; d < 2; d++) {
+    if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
+    if (ctx->bfs_cont[d]) (void)hipGraphExecDestroy(ctx->bfs_cont[d]);
+    ctx->bfs_graph[d] = ctx->bfs_cont[d] = nullptr;
+    ctx->bfs_graph_steps[d] = 0;
+  }
   if (ctx->pr_graph) (void)hipGraphExecDestroy(ctx->pr_graph);
   if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
   if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
-  ctx->bfs_graph = ctx->bfs_cont = ctx->pr_graph = ctx->batch_graph = ctx->batch_cont = nullptr;
+  ctx->pr_graph = ctx->batch_graph = ctx->batch_cont = nullptr;
   ctx->pr_graph_key = -1;
   ctx->batch_graph_key = -1;
-  ctx->bfs_graph_steps = 0;
+}
+
+void free_graph_state(dmm_ctx* ctx) {
+  drop_graphs(ctx);
   void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
                   ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],
                   ctx->d_score,   ctx->d_contrib[0], ctx->d_contrib[1], ctx->d_score64,
                   ctx->d_contrib64[0], ctx->d_contrib64[1], ctx->d_err_part, ctx->d_heavy,
                   ctx->d_gcontrib, ctx->d_seen, ctx->d_levels, ctx->d_bact, ctx->d_baoff,
-                  ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial};
+                  ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
+                  ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -92,6 +101,13 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bact = nullptr;
   ctx->d_baoff = nullptr;
   ctx->d_bdepth = nullptr;
+  ctx->d_perm = ctx->d_inv = ctx->d_parent_out = ctx->d_depth_out = nullptr;
+  ctx->d_score_out = nullptr;
+  ctx->d_score64_out = nullptr;
+  ctx->relabeled = false;
+  ctx->h_perm.clear();
+  ctx->h_perm.shrink_to_fit();
+  ctx->depth_valid = false;
   ctx->level_cap = 0;
   ctx->bdepth_cap = 0;
   ctx->has_graph = false;
@@ -106,6 +122,17 @@ int need_graph(dmm_ctx* ctx) {
   if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
   return DMM_OK;
 }
+
+// The single-step API exposes the device state as is: natural layout only.
+int need_natural(dmm_ctx* ctx) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (ctx->relabeled)
+    return set_err(ctx, DMM_ERR_STATE, "the step API needs DMM_LAYOUT_NATURAL (dmm_ctx_set_layout before upload)");
+  return DMM_OK;
+}
+
+int32_t to_internal(const dmm_ctx* ctx, int32_t v) { return ctx->relabeled ? ctx->h_perm[v] : v; }
 
 // Validates one CSR block: monotone offsets starting at 0 and IDs in range.
 int check_csr(dmm_ctx* ctx, int64_t rows, const int64_t* off, const int32_t* nbr, int64_t n,
@@ -128,11 +155,17 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
   const int64_t nl = ctx->nloc;
   if (mode == DMM_PR_FP64) {
     if (!ctx->d_score64) {
+      if (ctx->relabeled &&
+          (rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score64_out, sizeof(double) * nl, "FP64 scores (original IDs)")))
+        return rc;
       if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score64, sizeof(double) * nl, "FP64 scores"))) return rc;
       if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib64[0], sizeof(double) * nl, "FP64 contrib"))) return rc;
       if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib64[1], sizeof(double) * nl, "FP64 contrib"))) return rc;
     }
   } else if (!ctx->d_score) {
+    if (ctx->relabeled &&
+        (rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score_out, sizeof(float) * nl, "scores (original IDs)")))
+      return rc;
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_score, sizeof(float) * nl, "scores"))) return rc;
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[0], sizeof(float) * nl, "contrib"))) return rc;
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
@@ -299,6 +332,14 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
   delete ctx;
 }
 
+int dmm_ctx_set_layout(dmm_ctx* ctx, int32_t layout) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (layout != DMM_LAYOUT_NATURAL && layout != DMM_LAYOUT_DEGREE)
+    return set_err(ctx, DMM_ERR_INVALID, "unknown layout %d", layout);
+  ctx->layout = layout;
+  return DMM_OK;
+}
+
 int dmm_ctx_device_bytes(const dmm_ctx* ctx, int64_t* bytes) {
   if (!ctx || !bytes) return set_err(nullptr, DMM_ERR_INVALID, "NULL argument");
   *bytes = ctx->dev_bytes;
@@ -338,20 +379,48 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
   // the block width itself (every PE's width except the last PE's remainder).
   ctx->beta_width = rows;
   auto A = [&](void** p, size_t bytes, const char* what) { return dmm::dev_alloc(ctx, p, bytes, what); };
-  if ((rc = A((void**)&ctx->d_out_off, sizeof(int64_t) * (rows + 1), "out offsets")) ||
-      (rc = A((void**)&ctx->d_out_nbr, sizeof(int32_t) * ctx->m_out, "out neighbours")))
-    return rc;
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_off, out_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_nbr, out_neighs, sizeof(int32_t) * ctx->m_out, hipMemcpyHostToDevice));
-  if (directed) {
-    if ((rc = A((void**)&ctx->d_in_off, sizeof(int64_t) * (rows + 1), "in offsets")) ||
-        (rc = A((void**)&ctx->d_in_nbr, sizeof(int32_t) * ctx->m_in, "in neighbours")))
+  // Degree-ordered internal numbering for whole-graph uploads.
+  ctx->relabeled = ctx->layout == DMM_LAYOUT_DEGREE && rows == num_nodes;
+  std::vector<int64_t> new_out_off, new_in_off;
+  const int64_t* up_out_off = out_offsets;  // offsets as held on the device
+  const int64_t* up_in_off = in_offsets;
+  if (ctx->relabeled) {
+    std::vector<int32_t> inv;
+    dmm::degree_order(rows, out_offsets, &ctx->h_perm, &inv);
+    if ((rc = A((void**)&ctx->d_perm, sizeof(int32_t) * rows, "layout permutation")) ||
+        (rc = A((void**)&ctx->d_inv, sizeof(int32_t) * rows, "layout inverse")) ||
+        (rc = A((void**)&ctx->d_parent_out, sizeof(int32_t) * rows, "parent (original IDs)")) ||
+        (rc = A((void**)&ctx->d_depth_out, sizeof(int32_t) * rows, "depth (original IDs)")))
       return rc;
-    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_off, in_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
-    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_nbr, in_neighs, sizeof(int32_t) * ctx->m_in, hipMemcpyHostToDevice));
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_perm, ctx->h_perm.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_inv, inv.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+    if ((rc = dmm::relabel_upload(ctx, out_offsets, out_neighs, inv, &new_out_off, &ctx->d_out_off, &ctx->d_out_nbr)))
+      return rc;
+    up_out_off = new_out_off.data();
+    if (directed) {
+      if ((rc = dmm::relabel_upload(ctx, in_offsets, in_neighs, inv, &new_in_off, &ctx->d_in_off, &ctx->d_in_nbr)))
+        return rc;
+      up_in_off = new_in_off.data();
+    } else {
+      ctx->d_in_off = ctx->d_out_off;
+      ctx->d_in_nbr = ctx->d_out_nbr;
+    }
   } else {
-    ctx->d_in_off = ctx->d_out_off;
-    ctx->d_in_nbr = ctx->d_out_nbr;
+    if ((rc = A((void**)&ctx->d_out_off, sizeof(int64_t) * (rows + 1), "out offsets")) ||
+        (rc = A((void**)&ctx->d_out_nbr, sizeof(int32_t) * ctx->m_out, "out neighbours")))
+      return rc;
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_off, out_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+    DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_out_nbr, out_neighs, sizeof(int32_t) * ctx->m_out, hipMemcpyHostToDevice));
+    if (directed) {
+      if ((rc = A((void**)&ctx->d_in_off, sizeof(int64_t) * (rows + 1), "in offsets")) ||
+          (rc = A((void**)&ctx->d_in_nbr, sizeof(int32_t) * ctx->m_in, "in neighbours")))
+        return rc;
+      DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_off, in_offsets, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+      DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_in_nbr, in_neighs, sizeof(int32_t) * ctx->m_in, hipMemcpyHostToDevice));
+    } else {
+      ctx->d_in_off = ctx->d_out_off;
+      ctx->d_in_nbr = ctx->d_out_nbr;
+    }
   }
   // BFS state.
   if ((rc = A((void**)&ctx->d_parent, sizeof(int32_t) * rows, "parent")) ||
@@ -364,7 +433,7 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
     return rc;
   // PageRank heavy-vertex list: rows with in-degree >= threshold get a whole
   // wavefront each.
-  const int64_t* ioff = directed ? in_offsets : out_offsets;
+  const int64_t* ioff = directed ? up_in_off : up_out_off;
   std::vector<int32_t> heavy;
   for (int64_t r = 0; r < rows; r++)
     if (ioff[r + 1] - ioff[r] >= ctx->heavy_threshold) heavy.push_back(static_cast<int32_t>(r));
@@ -421,15 +490,16 @@ int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* 
     return set_err(ctx, DMM_ERR_STATE, "dmm_bfs needs the whole graph; drive a PE block with the step API");
   const double t0 = now_ms();
   if (stats) std::memset(stats, 0, sizeof(*stats));
+  const int32_t isrc = to_internal(ctx, source);
   if (!trace) {
-    rc = dmm::bfs_device_loop(ctx, source, alpha, beta, stats);
+    rc = dmm::bfs_device_loop(ctx, isrc, alpha, beta, depth != nullptr, stats);
     if (rc) return rc;
   } else {
     // Host-driven replica of src/bfs.cc:142-188, one kernel per step, each
     // bracketed by events for the per-step times.
     std::vector<hipEvent_t>& ev = ctx->step_events;
     DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
-    if ((rc = dmm::bfs_init(ctx, source))) return rc;
+    if ((rc = dmm::bfs_init(ctx, isrc))) return rc;
     int64_t edges_to_check = ctx->ned;
     int64_t scout_count = ctx->h_ctrl->scout;
     int64_t qsize = (ctx->h_ctrl->qe - ctx->h_ctrl->qb) + ctx->h_ctrl->window_zero;
@@ -500,18 +570,20 @@ int dmm_bfs_read(dmm_ctx* ctx, int32_t* parent, int32_t* depth) {
   int rc = need_graph(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "no BFS result on the ctx");
+  if (depth && !ctx->depth_valid)
+    return set_err(ctx, DMM_ERR_STATE, "the last BFS did not track depths (pass a depth buffer to dmm_bfs)");
+  const int32_t* dp = ctx->relabeled ? ctx->d_parent_out : ctx->d_parent;
+  const int32_t* dd = ctx->relabeled ? ctx->d_depth_out : ctx->d_depth;
   if (parent)
-    DMM_HIP_CHECK(ctx, hipMemcpyAsync(parent, ctx->d_parent, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost,
-                                      ctx->stream));
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(parent, dp, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost, ctx->stream));
   if (depth)
-    DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_depth, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost,
-                                      ctx->stream));
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, dd, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DMM_OK;
 }
 
 int dmm_step_init(dmm_ctx* ctx, int32_t source) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (source < 0 || source >= ctx->n)
     return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
@@ -519,42 +591,42 @@ int dmm_step_init(dmm_ctx* ctx, int32_t source) {
 }
 
 int dmm_step_td(dmm_ctx* ctx, int64_t* scout_count, int64_t* queue_size) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   return dmm::bfs_td_step(ctx, scout_count, queue_size);
 }
 
 int dmm_step_queue_to_bitmap(dmm_ctx* ctx, int64_t* awake_count) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   return dmm::bfs_queue_to_bitmap(ctx, awake_count);
 }
 
 int dmm_step_bu(dmm_ctx* ctx, int64_t* awake_count) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   return dmm::bfs_bu_step(ctx, awake_count);
 }
 
 int dmm_step_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   return dmm::bfs_bitmap_to_queue(ctx, queue_size);
 }
 
 int dmm_step_finish(dmm_ctx* ctx) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   return dmm::bfs_finish(ctx);
 }
 
 int dmm_state_get_parent(dmm_ctx* ctx, int32_t* parent) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!parent) return set_err(ctx, DMM_ERR_INVALID, "parent is NULL");
   DMM_HIP_CHECK(ctx, hipMemcpy(parent, ctx->d_parent, sizeof(int32_t) * ctx->nloc, hipMemcpyDeviceToHost));
@@ -562,7 +634,7 @@ int dmm_state_get_parent(dmm_ctx* ctx, int32_t* parent) {
 }
 
 int dmm_state_set_parent(dmm_ctx* ctx, const int32_t* parent) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!parent) return set_err(ctx, DMM_ERR_INVALID, "parent is NULL");
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -571,7 +643,7 @@ int dmm_state_set_parent(dmm_ctx* ctx, const int32_t* parent) {
 }
 
 int dmm_state_get_queue(dmm_ctx* ctx, int32_t* queue, int64_t capacity, int64_t* size) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
@@ -594,7 +666,7 @@ __global__ void k_set_window(dmm::BfsCtrl* c, long long size, long long edges, l
 }
 
 int dmm_state_set_queue(dmm_ctx* ctx, const int32_t* queue, int64_t size) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
   if (size < 0 || size > ctx->nloc) return set_err(ctx, DMM_ERR_CAPACITY, "queue size %lld", (long long)size);
@@ -636,7 +708,7 @@ int dmm_state_set_queue(dmm_ctx* ctx, const int32_t* queue, int64_t size) {
 }
 
 int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
   DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
@@ -647,7 +719,7 @@ int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words) {
 }
 
 int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -661,7 +733,7 @@ int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words) {
 __global__ void k_set_level(dmm::BfsCtrl* c, int level) { c->level = level; }
 
 int dmm_state_set_level(dmm_ctx* ctx, int32_t level) {
-  int rc = need_graph(ctx);
+  int rc = need_natural(ctx);
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_level, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl, level);
   DMM_HIP_CHECK(ctx, hipGetLastError());
@@ -686,7 +758,9 @@ int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t*
   if ((rc = ensure_batch_state(ctx))) return rc;
   const double t0 = now_ms();
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  rc = dmm::batch_run(ctx, sources, nsrc, depth, params, stats);
+  int32_t isrc[64];
+  for (int i = 0; i < nsrc; i++) isrc[i] = to_internal(ctx, sources[i]);
+  rc = dmm::batch_run(ctx, isrc, nsrc, depth, params, stats);
   if (rc) return rc;
   if (stats) stats->host_ms = now_ms() - t0;
   return DMM_OK;
@@ -713,9 +787,11 @@ int dmm_pr_pull(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, v
   if ((rc = dmm::pr_run(ctx, max_iters, epsilon, mode, st))) return rc;
   if (scores) {
     if (mode == DMM_PR_FP64)
-      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->d_score64, sizeof(double) * ctx->nloc, hipMemcpyDeviceToHost));
+      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->relabeled ? ctx->d_score64_out : ctx->d_score64,
+                                   sizeof(double) * ctx->nloc, hipMemcpyDeviceToHost));
     else
-      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->d_score, sizeof(float) * ctx->nloc, hipMemcpyDeviceToHost));
+      DMM_HIP_CHECK(ctx, hipMemcpy(scores, ctx->relabeled ? ctx->d_score_out : ctx->d_score,
+                                   sizeof(float) * ctx->nloc, hipMemcpyDeviceToHost));
   }
   st->host_ms = now_ms() - t0;
   return DMM_OK;
@@ -745,13 +821,7 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
   ctx->tuning.batch = t->batch;
   if (t->grid_per_cu > 0) ctx->tuning.grid_per_cu = std::min(32, t->grid_per_cu);
   // Graphs captured with the old shape are rebuilt on next use.
-  if (ctx->bfs_graph) (void)hipGraphExecDestroy(ctx->bfs_graph);
-  if (ctx->bfs_cont) (void)hipGraphExecDestroy(ctx->bfs_cont);
-  if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
-  if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
-  ctx->bfs_graph = ctx->bfs_cont = ctx->batch_graph = ctx->batch_cont = nullptr;
-  ctx->bfs_graph_steps = 0;
-  ctx->batch_graph_key = -1;
+  drop_graphs(ctx);
   return DMM_OK;
 }
 

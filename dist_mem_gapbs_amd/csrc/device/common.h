@@ -89,6 +89,16 @@ struct dmm_ctx {
   int64_t fixed_bytes = 0;  // ... of it by the ctx itself (not the graph)
 
   // Graph (one PE's block).
+  int layout = DMM_LAYOUT_DEGREE;  // requested for the next upload
+  bool relabeled = false;          // this graph uses internal IDs
+  int32_t* d_perm = nullptr;       // original -> internal ID
+  int32_t* d_inv = nullptr;        // internal -> original ID
+  std::vector<int32_t> h_perm;     // host copy of perm (source mapping)
+  int32_t* d_parent_out = nullptr; // results in original IDs
+  int32_t* d_depth_out = nullptr;
+  float* d_score_out = nullptr;
+  double* d_score64_out = nullptr;
+  bool depth_valid = false;        // the last BFS tracked depths
   int64_t n = 0, vbegin = 0, vend = 0, nloc = 0, words = 0;
   int64_t m_out = 0, m_in = 0, ned = 0;
   bool directed = false, has_graph = false;
@@ -115,9 +125,10 @@ struct dmm_ctx {
   int64_t qb = 0, qe = 0, qin = 0, q_edges = 0, q_zero = 0;
   int level = 0, front = 0;
   bool bfs_valid = false;
-  hipGraphExec_t bfs_graph = nullptr;  // init + bfs_chain_len steps + finish
-  hipGraphExec_t bfs_cont = nullptr;   // 4 more steps + finish
-  int bfs_graph_steps = 0;
+  // Captured device loops, [0] without and [1] with depth tracking.
+  hipGraphExec_t bfs_graph[2] = {nullptr, nullptr};  // init + steps + finish
+  hipGraphExec_t bfs_cont[2] = {nullptr, nullptr};   // more steps + finish
+  int bfs_graph_steps[2] = {0, 0};
   int bfs_chain_len = 16;
 
   // PageRank state.

@@ -26,6 +26,8 @@
 
 #include <algorithm>
 
+#include "layout.h"
+
 namespace dmm {
 
 namespace {
@@ -518,6 +520,13 @@ int pr_run(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, dmm_pr
       hipLaunchKernelGGL(k_init_f32, dim3(ib), dim3(kThreads), 0, ctx->stream, a, ctx->d_score,
                          ctx->d_contrib[0]);
     for (int it = 0; it < max_iters; it++) launch_iteration(ctx, a, mode, it, nullptr);
+    // Scores back to the caller's IDs.
+    if (ctx->relabeled) {
+      if (mode == DMM_PR_FP64)
+        launch_map_scores_f64(ctx);
+      else
+        launch_map_scores_f32(ctx);
+    }
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "PR capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&ctx->pr_graph, g, nullptr, nullptr, 0);

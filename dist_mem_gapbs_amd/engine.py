@@ -79,6 +79,11 @@ class Context:
         self.n, self.vbegin, self.vend = num_nodes, vbegin, vend
         self.words = (num_nodes + 63) // 64
 
+    def set_layout(self, layout):
+        """L.DMM_LAYOUT_DEGREE (default) or L.DMM_LAYOUT_NATURAL; applies to
+        the next upload.  The step API needs the natural layout."""
+        self._check(self._lib.dmm_ctx_set_layout(self._h, layout))
+
     def device_bytes(self):
         b = C.c_int64()
         self._check(self._lib.dmm_ctx_device_bytes(self._h, C.byref(b)))

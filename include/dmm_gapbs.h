@@ -143,6 +143,18 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin,
                      const int32_t* out_neighs, const int64_t* in_offsets,
                      const int32_t* in_neighs, int64_t num_edges_directed);
 
+/* Internal vertex numbering of whole-graph uploads.  DMM_LAYOUT_DEGREE
+ * (the default) renumbers vertices by non-increasing out-degree inside the
+ * ctx so the most-gathered per-vertex data stays in L2; every result is
+ * returned in the caller's IDs.  DMM_LAYOUT_NATURAL keeps the caller's IDs
+ * and is required by the single-step API (dmm_step_*, dmm_state_*), whose
+ * state is exposed as is.  Takes effect at the next dmm_graph_upload; PE
+ * blocks (vbegin > 0 or vend < N) always use the natural layout.  No
+ * counterpart in the reference. */
+#define DMM_LAYOUT_NATURAL 0
+#define DMM_LAYOUT_DEGREE 1
+int dmm_ctx_set_layout(dmm_ctx* ctx, int32_t layout);
+
 /* Device memory currently held by the ctx, in bytes. */
 int dmm_ctx_device_bytes(const dmm_ctx* ctx, int64_t* bytes);
 

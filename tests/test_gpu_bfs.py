@@ -121,8 +121,54 @@ def test_directed_graph(ctx, dmm, orc):
         check_bfs(ctx, orc, og, s)
 
 
-def test_step_api_matches_oracle_steps(ctx, dmm, orc, graph):
+@pytest.fixture
+def natural(ctx, dmm):
+    """The step API exposes device state in the caller's IDs: natural layout."""
+    ctx.set_layout(dmm._lib.DMM_LAYOUT_NATURAL)
+    yield ctx
+    ctx.set_layout(dmm._lib.DMM_LAYOUT_DEGREE)
+
+
+def test_step_api_rejects_degree_layout(ctx, dmm, graph):
+    ctx.set_layout(dmm._lib.DMM_LAYOUT_DEGREE)
+    ctx.upload(graph(12, 16, False))
+    with pytest.raises(dmm.DmmError) as e:
+        ctx.step_init(0)
+    assert e.value.code == -4
+
+
+def test_layouts_agree(ctx, dmm, orc, graph):
+    g = graph(14, 16, False)
+    og = orc.CSR.of(g)
+    srcs = dmm.pick_sources(g, 4)
+    res = {}
+    for layout in (dmm._lib.DMM_LAYOUT_NATURAL, dmm._lib.DMM_LAYOUT_DEGREE):
+        ctx.set_layout(layout)
+        ctx.upload(g)
+        res[layout] = [ctx.DOBFS(int(s))[1] for s in srcs]
+        for s, d in zip(srcs, res[layout]):
+            assert np.array_equal(d, orc.bfs_depth(og, int(s)))
+    ctx.set_layout(dmm._lib.DMM_LAYOUT_DEGREE)
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
+def test_dobfs_without_depth_then_read(ctx, dmm, orc, graph):
+    g = graph(13, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    s = int(dmm.pick_sources(g, 1)[0])
+    parent, depth, _ = ctx.DOBFS(s, want_depth=False)
+    assert depth is None and orc.bfs_verify(og, s, parent)[0]
+    p2, _ = ctx.bfs_read(parent=True, depth=False)
+    assert np.array_equal(parent, p2)
+    with pytest.raises(dmm.DmmError):
+        ctx.bfs_read(parent=False, depth=True)
+
+
+def test_step_api_matches_oracle_steps(natural, dmm, orc, graph):
     """InitParent, TDStep, QueueToBitmap, BUStep, BitmapToQueue one by one."""
+    ctx = natural
     hg = graph(13, 16, False)
     g = orc.CSR.of(hg)
     ctx.upload(hg)
@@ -175,7 +221,8 @@ def test_step_api_matches_oracle_steps(ctx, dmm, orc, graph):
     ctx.step_finish()
 
 
-def test_set_queue_and_bitmap_roundtrip(ctx, dmm, orc, graph):
+def test_set_queue_and_bitmap_roundtrip(natural, dmm, orc, graph):
+    ctx = natural
     hg = graph(12, 16, False)
     g = orc.CSR.of(hg)
     ctx.upload(hg)

@@ -117,3 +117,16 @@ def test_pr_pe_block_iteration(ctx, dmm, orc, graph):
     got, err, _ = ctx.pr_block_iteration(contrib, s1[b:e], MODES["serial"])
     assert np.array_equal(got, s2[b:e])
     assert err == pytest.approx(float(np.abs(s2[b:e].astype(np.float64) - s1[b:e]).sum()), rel=1e-9)
+
+
+def test_pr_layouts_bit_identical(ctx, dmm, orc, graph):
+    """The degree-ordered layout keeps each row's neighbour order, so the
+    reference-order FP32 sums are unchanged."""
+    g = graph(15, 16, False)
+    out = {}
+    for layout in (dmm._lib.DMM_LAYOUT_NATURAL, dmm._lib.DMM_LAYOUT_DEGREE):
+        ctx.set_layout(layout)
+        ctx.upload(g)
+        out[layout] = ctx.PageRankPull(20, 1e-4, MODES["serial"])[0]
+    ctx.set_layout(dmm._lib.DMM_LAYOUT_DEGREE)
+    assert np.array_equal(out[0], out[1])
