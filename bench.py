@@ -43,6 +43,8 @@ def log(*a):
 @contextlib.contextmanager
 def quiet_stdout():
     """Silences C-level stdout (the reference kernels print per-step lines)."""
+    import ctypes
+    libc = ctypes.CDLL(None)
     sys.stdout.flush()
     fd = os.dup(1)
     dn = os.open(os.devnull, os.O_WRONLY)
@@ -51,13 +53,15 @@ def quiet_stdout():
         yield
     finally:
         sys.stdout.flush()
+        libc.fflush(None)
         os.dup2(fd, 1)
         os.close(fd)
         os.close(dn)
 
 
 def bu_roofline(ctx, sources, nverts):
-    """Per-step trace: bytes and device time of the bottom-up step kernel."""
+    """Per-step trace: algorithmic bytes and device time of the bottom-up
+    sweep kernel (HIP events on the ctx stream around that launch alone)."""
     import dist_mem_gapbs_amd._lib as L
     tot_b, tot_ms, launches = 0, 0.0, 0
     for s in sources:
@@ -66,7 +70,7 @@ def bu_roofline(ctx, sources, nverts):
             if st.step_kind[i] == L.DMM_STEP_BU:
                 u, e = st.step_in[i], st.step_edges[i]
                 tot_b += 20 * u + 4 * e + e // 8 + nverts // 8
-                tot_ms += st.step_ms[i]
+                tot_ms += st.step_kernel_ms[i]
                 launches += 1
     return tot_b, tot_ms, launches
 
@@ -81,30 +85,37 @@ def load_traffic(kernel, workload):
         return None
 
 
-def cpu_baseline(og, sources, m_comp):
+def cpu_baseline(og, sources, m_comp, seconds):
+    """Times the CPU DOBFS on repeated passes over the sources until about
+    `seconds` of work is done (bounded sample of the same workload)."""
     from oracle import oracle as orc
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if orc.ref_available():
         rg = orc.RefGraph(og)
-        with quiet_stdout():
-            rg.dobfs(int(sources[0]))  # warm
-            t0 = time.perf_counter()
-            for s in sources:
-                rg.dobfs(int(s))
-            dt = time.perf_counter() - t0
-        rg.close()
+        run = rg.dobfs
         kind = "reference"
     else:
+        rg = None
         cores = 1
-        t0 = time.perf_counter()
-        for s in sources:
-            orc.dobfs(og, int(s))
-        dt = time.perf_counter() - t0
+        run = lambda s: orc.dobfs(og, s)
         kind = "port"
-    edges = sum(m_comp[int(s)] for s in sources)
+    trials, edges, dt = 0, 0, 0.0
+    with quiet_stdout():
+        run(int(sources[0]))  # warm
+        t0 = time.perf_counter()
+        while dt < seconds and trials < 64 * 1000:
+            s = int(sources[trials % len(sources)])
+            run(s)
+            edges += m_comp[s]
+            trials += 1
+            dt = time.perf_counter() - t0
+    if rg is not None:
+        rg.close()
+    what = ("the reference DOBFS (gapbs-verify bfs.cc, OpenMP)" if kind == "reference"
+            else "the oracle's scalar DOBFS")
     return {"value": edges / dt / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind,
-            "sample": f"{len(sources)} DOBFS trials (same Kronecker-22 graph and sources), "
-                      f"{dt:.2f} s"}
+            "sample": f"{trials} DOBFS trials of {what} cycling over the same {len(sources)} sources "
+                      f"on the same Kronecker-22 graph, {dt:.1f} s"}
 
 
 def kron24_leg(dmm, args, tuned):
@@ -210,8 +221,12 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--big-scale", type=int, default=24)
     ap.add_argument("--no-big", action="store_true", help="skip the Kronecker-24 leg")
-    ap.add_argument("--cpu-sources", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="length of the bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one elementwise kernel that reads and writes 256 MiB "
+                         "(scale reference for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--only", choices=["bfs", "batch", "pr"],
                     help="run one leg alone on Kronecker --big-scale (for profiling)")
     args = ap.parse_args()
@@ -223,6 +238,14 @@ def main():
 
     import dist_mem_gapbs_amd as dmm
     from oracle import oracle as orc
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 26, dtype=torch.float32, device="cuda:0")
+        b = a * 2.0  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        log(f"calibration copy: {a.numel() * 4 >> 10} KiB read, {b.numel() * 4 >> 10} KiB written")
+        del a, b
 
     if args.only:
         only_leg(dmm, args)
@@ -283,7 +306,7 @@ def main():
     og = orc.CSR.of(g)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(og, srcs[:args.cpu_sources], m_comp)
+        cpu = cpu_baseline(og, srcs, m_comp, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
     del og
     g.free()

@@ -47,6 +47,7 @@ class BfsStats(C.Structure):
         ("step_edges", C.c_int64 * DMM_MAX_STEPS),
         ("step_scout", C.c_int64 * DMM_MAX_STEPS),
         ("step_ms", C.c_float * DMM_MAX_STEPS),
+        ("step_kernel_ms", C.c_float * DMM_MAX_STEPS),
         ("alg_bytes", C.c_int64),
         ("bu_alg_bytes", C.c_int64),
         ("reached", C.c_int64),

@@ -697,8 +697,14 @@ void mark_end(dmm_ctx* ctx) {
 }
 
 template <int kPhase>
-void launch_pair(dmm_ctx* ctx, const StepArgs& a, const BfsParams* prm, int blocks) {
+void launch_pair(dmm_ctx* ctx, const StepArgs& a, const BfsParams* prm, int blocks, bool timed = false) {
+  timed = timed && ctx->trace_kev_begin && ctx->trace_kev_end;
+  if (timed) (void)hipEventRecord(ctx->trace_kev_begin, ctx->stream);
   hipLaunchKernelGGL(k_step<kPhase>, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm);
+  if (timed) {
+    (void)hipEventRecord(ctx->trace_kev_end, ctx->stream);
+    ctx->trace_kev_used = true;
+  }
   hipLaunchKernelGGL(k_post<kPhase>, dim3(1), dim3(kThreads), 0, ctx->stream, a, prm, blocks);
 }
 
@@ -737,7 +743,7 @@ int bfs_td_step(dmm_ctx* ctx, int64_t* scout, int64_t* queue_size) {
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
   const int g = grid_blocks(ctx);
   mark_begin(ctx);
-  launch_pair<kPhaseTD>(ctx, a, prm, g);
+  launch_pair<kPhaseTD>(ctx, a, prm, g, true);
   launch_pair<kPhaseCount>(ctx, a, prm, g);
   launch_pair<kPhaseWrite>(ctx, a, prm, g);
   mark_end(ctx);
@@ -759,7 +765,7 @@ int bfs_bu_step(dmm_ctx* ctx, int64_t* awake) {
     launch_fill_u64(ctx->stream, ctx->d_bm[scratch], ctx->words, 0);
   }
   mark_begin(ctx);
-  launch_pair<kPhaseBU>(ctx, a, prm, grid_blocks(ctx));
+  launch_pair<kPhaseBU>(ctx, a, prm, grid_blocks(ctx), true);
   mark_end(ctx);
   DMM_HIP_CHECK(ctx, hipGetLastError());
   int rc = read_ctrl_head(ctx);

@@ -171,7 +171,9 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
   }
   if (!ctx->d_err_part) {
-    const int64_t parts = dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx) + dmm::pr_super_blocks(ctx);
+    const int64_t parts =
+        std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx), dmm::pr_fused_blocks(ctx)) +
+        dmm::pr_super_blocks(ctx);
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
     ctx->err_part_cap = parts;
   }
@@ -283,7 +285,7 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
     return bail(set_err(nullptr, DMM_ERR_HIP, "stream creation failed"));
   if (hipEventCreate(&ctx->ev_a) != hipSuccess || hipEventCreate(&ctx->ev_b) != hipSuccess)
     return bail(set_err(nullptr, DMM_ERR_HIP, "event creation failed"));
-  ctx->step_events.resize(2 * DMM_MAX_STEPS + 2, nullptr);
+  ctx->step_events.resize(4 * DMM_MAX_STEPS + 4, nullptr);
   for (auto& e : ctx->step_events)
     if (hipEventCreate(&e) != hipSuccess) return bail(set_err(nullptr, DMM_ERR_HIP, "event creation failed"));
   int rc;
@@ -504,19 +506,23 @@ int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* 
     int64_t scout_count = ctx->h_ctrl->scout;
     int64_t qsize = (ctx->h_ctrl->qe - ctx->h_ctrl->qb) + ctx->h_ctrl->window_zero;
     int si = 0;
-    std::vector<std::pair<int, int>> step_ev;  // (begin, end) event index per step
+    std::vector<int> step_ev;  // 2 * (first of 4 event indices) + kernel timed
     // Each step's kernels are bracketed by a pair of events recorded inside
-    // the step functions (not the host-side readback that follows them).
+    // the step functions (not the host-side readback that follows them),
+    // and its traversal kernel alone by a second pair.
     auto run = [&](auto fn) -> int {
-      int b = si++;
-      int e = si++;
-      const bool have = e < static_cast<int>(ev.size());
+      const int b = si;
+      si += 4;
+      const bool have = b + 3 < static_cast<int>(ev.size());
       ctx->trace_ev_begin = have ? ev[b] : nullptr;
-      ctx->trace_ev_end = have ? ev[e] : nullptr;
+      ctx->trace_ev_end = have ? ev[b + 1] : nullptr;
+      ctx->trace_kev_begin = have ? ev[b + 2] : nullptr;
+      ctx->trace_kev_end = have ? ev[b + 3] : nullptr;
+      ctx->trace_kev_used = false;
       int r = fn();
-      ctx->trace_ev_begin = ctx->trace_ev_end = nullptr;
+      ctx->trace_ev_begin = ctx->trace_ev_end = ctx->trace_kev_begin = ctx->trace_kev_end = nullptr;
       if (r) return r;
-      step_ev.push_back({b, e});
+      step_ev.push_back(have ? 2 * b + (ctx->trace_kev_used ? 1 : 0) : -2);
       return DMM_OK;
     };
     while (qsize != 0) {
@@ -546,10 +552,14 @@ int dmm_bfs(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, int32_t* 
       DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
       stats->total_ms = ms;
       for (size_t i = 0; i < step_ev.size() && i < DMM_MAX_STEPS; i++) {
-        float sm = 0;
-        if (step_ev[i].second < static_cast<int>(ev.size()))
-          DMM_HIP_CHECK(ctx, hipEventElapsedTime(&sm, ev[step_ev[i].first], ev[step_ev[i].second]));
+        float sm = 0, km = 0;
+        const int b = step_ev[i] >> 1;
+        if (b >= 0) {
+          DMM_HIP_CHECK(ctx, hipEventElapsedTime(&sm, ev[b], ev[b + 1]));
+          if (step_ev[i] & 1) DMM_HIP_CHECK(ctx, hipEventElapsedTime(&km, ev[b + 2], ev[b + 3]));
+        }
         stats->step_ms[i] = sm;
+        stats->step_kernel_ms[i] = km;
       }
     }
   }

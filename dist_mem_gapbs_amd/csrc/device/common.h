@@ -82,6 +82,9 @@ struct dmm_ctx {
   std::vector<hipEvent_t> step_events;  // per-step timing in trace mode
   hipEvent_t trace_ev_begin = nullptr;  // set while a traced step runs
   hipEvent_t trace_ev_end = nullptr;
+  hipEvent_t trace_kev_begin = nullptr;  // around the traversal kernel alone
+  hipEvent_t trace_kev_end = nullptr;
+  bool trace_kev_used = false;
   std::string err;
   std::string arch;
   int cu_count = 0;

@@ -26,6 +26,7 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 int pr_light_blocks(dmm_ctx* ctx);
 int pr_heavy_blocks(dmm_ctx* ctx);
 int pr_super_blocks(dmm_ctx* ctx);
+int pr_fused_blocks(dmm_ctx* ctx);  // persistent KAHAN / FP64 kernel
 // Size of the PrParams block (bytes) the ctx must hold at d_prparams.
 constexpr size_t kPrParamsBytes = 64;
 

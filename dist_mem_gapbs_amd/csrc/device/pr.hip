@@ -34,6 +34,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kGroup = 8;         // lanes per light vertex (KAHAN / FP64)
+constexpr int kFusedThreads = 1024;
+constexpr int kHubBytes = 128 * 1024;  // LDS copy of the first contributions
 
 struct PrParams {
   double eps;
@@ -53,9 +55,11 @@ struct PrArgs {
   const PrSuper* supers;          // split rows and their segment slots
   double2* partial;               // per-segment partial sums
   int64_t nloc, n_heavy, n_items, n_super;
+  int64_t n_gather;               // entries of the gathered contribution vector
   int heavy_threshold;
   double* err_part;               // [light | heavy-or-items | super] blocks
   int light_blocks, heavy_blocks, super_blocks;
+  int main_blocks;                // error partials written before the combine
   PrCtrl* ctrl;
   double* err_trace;              // per-iteration error
   const PrParams* prm;
@@ -84,13 +88,14 @@ struct Comp {
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
+template <int kBlock = kThreads>
 __device__ __forceinline__ double block_err_sum(double x, double* red) {
   double s = wave_sum(x);
   if (lane_id() == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   double r = 0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kThreads / 64; w++) r += red[w];
+    for (int w = 0; w < kBlock / 64; w++) r += red[w];
   __syncthreads();
   return r;
 }
@@ -167,98 +172,136 @@ __global__ __launch_bounds__(kThreads) void k_serial_heavy(PrArgs a, const float
 
 // ---- FP32_KAHAN -------------------------------------------------------------
 
-__global__ __launch_bounds__(kThreads) void k_kahan_light(PrArgs a, const float* __restrict__ cin,
-                                                          float* __restrict__ cout,
-                                                          float* __restrict__ score) {
-  __shared__ double red[kThreads / 64];
-  if (a.ctrl->pad) return;
-  const int sub = threadIdx.x & (kGroup - 1);
-  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
-  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
-  double err = 0;
-  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
-  for (int64_t it = 0; it < iters; it++) {
-    const int64_t u = grp + it * ngrp;
-    Comp acc{0.0f, 0.0f};
-    bool mine = false;
-    if (u < a.nloc) {
-      const int64_t s = a.in_off[u], e = a.in_off[u + 1];
-      if (e - s < a.heavy_threshold) {
-        mine = true;
-        for (int64_t k = s + sub; k < e; k += kGroup) acc.add(cin[a.in_nbr[k]]);
-      }
-    }
-    // Merge the group's partials (fixed butterfly order).
+// Compensated FP32 / plain FP64 sums, one fused launch per iteration.
+//
+// Persistent blocks of kFusedThreads (one per CU).  Each block first copies
+// the contributions of vertices [0, hub_n) -- 128 KB -- into LDS; in the
+// degree-ordered layout these are the highest-degree vertices and receive
+// about half of all gathers on Kronecker graphs, which LDS serves several
+// times faster than L2.  Wavefronts then take the heavy items (rows with
+// in-degree >= heavy_threshold, or 2048-edge segments of split rows), one
+// item per wave with lanes striding its edge range and four gathers in
+// flight, and afterwards the light rows in 8-lane groups.  A split row's
+// segments store (sum, error) pairs for the combine kernel.
+template <typename T>
+struct FusedAcc;
+
+template <>
+struct FusedAcc<float> {
+  Comp c{0.0f, 0.0f};
+  __device__ __forceinline__ void add(float x) { c.add(x); }
+  // Fixed butterfly merge; operands ordered by lane so both partners of a
+  // pair compute the same value.
+  template <int kWidth>
+  __device__ __forceinline__ void reduce(int sub) {
 #pragma unroll
-    for (int d = kGroup / 2; d >= 1; d >>= 1) {
-      const float s2 = __shfl_xor(acc.s, d, kGroup);
-      const float c2 = __shfl_xor(acc.c, d, kGroup);
-      // Order the operands by lane so both partners compute the same pair.
+    for (int d = kWidth / 2; d >= 1; d >>= 1) {
+      const float s2 = __shfl_xor(c.s, d, kWidth);
+      const float c2 = __shfl_xor(c.c, d, kWidth);
       if (sub & d) {
         Comp o{s2, c2};
-        o.merge(acc.s, acc.c);
-        acc = o;
+        o.merge(c.s, c.c);
+        c = o;
       } else {
-        acc.merge(s2, c2);
+        c.merge(s2, c2);
       }
     }
-    if (mine && sub == 0) err += finish_f32(a, u, acc.value(), score, cout);
   }
-  double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+  __device__ __forceinline__ float value() const { return c.value(); }
+  __device__ __forceinline__ double2 partial() const { return make_double2(c.s, c.c); }
+};
+
+template <>
+struct FusedAcc<double> {
+  double s = 0.0;
+  __device__ __forceinline__ void add(double x) { s += x; }
+  template <int kWidth>
+  __device__ __forceinline__ void reduce(int) {
+#pragma unroll
+    for (int d = kWidth / 2; d >= 1; d >>= 1) s += __shfl_xor(s, d, kWidth);
+  }
+  __device__ __forceinline__ double value() const { return s; }
+  __device__ __forceinline__ double2 partial() const { return make_double2(s, 0.0); }
+};
+
+__device__ __forceinline__ double finish_t(const PrArgs& a, int64_t lu, float in, float* score, float* cout) {
+  return finish_f32(a, lu, in, score, cout);
+}
+__device__ __forceinline__ double finish_t(const PrArgs& a, int64_t lu, double in, double* score, double* cout) {
+  return finish_f64(a, lu, in, score, cout);
 }
 
+template <typename T>
+__device__ __forceinline__ T hub_gather(const T* hub, int hub_n, const T* __restrict__ cin, int32_t u) {
+  T x;
+  if (u < hub_n)
+    x = hub[u];
+  else
+    x = cin[u];
+  return x;
+}
 
-// Heavy rows (in-degree in [64, split)) and the segments of split rows, one
-// wavefront per item: lanes stride the item's edge range (coalesced ID
-// loads, four gathers in flight), then a fixed butterfly merges the
-// 64 compensated partials.  A whole row is finished in place; a segment
-// stores its (sum, error) pair for the combine kernel.
-__global__ __launch_bounds__(kThreads) void k_kahan_items(PrArgs a, const float* __restrict__ cin,
-                                                          float* __restrict__ cout,
-                                                          float* __restrict__ score) {
-  __shared__ double red[kThreads / 64];
-  if (a.ctrl->pad) return;
-  const int lane = lane_id();
-  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+template <typename T>
+__global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __restrict__ cin,
+                                                         T* __restrict__ cout, T* __restrict__ score) {
+  __shared__ T hub[kHubBytes / sizeof(T)];
+  __shared__ double red[kFusedThreads / 64];
+  if (a.ctrl->pad) return;  // converged
+  const int hub_n = static_cast<int>(min<int64_t>(kHubBytes / sizeof(T), a.n_gather));
+  for (int j = threadIdx.x; j < hub_n; j += kFusedThreads) hub[j] = cin[j];
+  __syncthreads();
+
   double err = 0;
+  const int lane = lane_id();
+  // Heavy items, one per wavefront.
+  const int64_t wave = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kFusedThreads) >> 6;
   for (int64_t h = wave; h < a.n_items; h += nwaves) {
     const PrItem it = a.items[h];
-    Comp acc{0.0f, 0.0f};
+    FusedAcc<T> acc;
     int64_t k = it.b + lane;
     for (; k + 192 < it.e; k += 256) {
-      const float x0 = cin[a.in_nbr[k]];
-      const float x1 = cin[a.in_nbr[k + 64]];
-      const float x2 = cin[a.in_nbr[k + 128]];
-      const float x3 = cin[a.in_nbr[k + 192]];
+      const int32_t u0 = a.in_nbr[k], u1 = a.in_nbr[k + 64], u2 = a.in_nbr[k + 128], u3 = a.in_nbr[k + 192];
+      const T x0 = hub_gather(hub, hub_n, cin, u0);
+      const T x1 = hub_gather(hub, hub_n, cin, u1);
+      const T x2 = hub_gather(hub, hub_n, cin, u2);
+      const T x3 = hub_gather(hub, hub_n, cin, u3);
       acc.add(x0);
       acc.add(x1);
       acc.add(x2);
       acc.add(x3);
     }
-    for (; k < it.e; k += 64) acc.add(cin[a.in_nbr[k]]);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      const float s2 = __shfl_xor(acc.s, d, 64);
-      const float c2 = __shfl_xor(acc.c, d, 64);
-      if (lane & d) {
-        Comp o{s2, c2};
-        o.merge(acc.s, acc.c);
-        acc = o;
-      } else {
-        acc.merge(s2, c2);
-      }
-    }
+    for (; k < it.e; k += 64) acc.add(hub_gather(hub, hub_n, cin, a.in_nbr[k]));
+    acc.template reduce<64>(lane);
     if (lane == 0) {
       if (it.slot < 0)
-        err += finish_f32(a, it.v, acc.value(), score, cout);
+        err += finish_t(a, it.v, acc.value(), score, cout);
       else
-        a.partial[it.slot] = make_double2(acc.s, acc.c);
+        a.partial[it.slot] = acc.partial();
     }
   }
-  double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+  // Light rows, one per 8-lane group; every lane of a wavefront runs the
+  // same number of rounds so the group shuffles stay convergent.
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kFusedThreads) / kGroup;
+  const int64_t rounds = (a.nloc + ngrp - 1) / ngrp;
+  for (int64_t r = 0; r < rounds; r++) {
+    const int64_t u = grp + r * ngrp;
+    FusedAcc<T> acc;
+    bool mine = false;
+    if (u < a.nloc) {
+      const int64_t s = a.in_off[u], e = a.in_off[u + 1];
+      if (e - s < a.heavy_threshold) {
+        mine = true;
+        for (int64_t k = s + sub; k < e; k += kGroup) acc.add(hub_gather(hub, hub_n, cin, a.in_nbr[k]));
+      }
+    }
+    acc.template reduce<kGroup>(sub);
+    if (mine && sub == 0) err += finish_t(a, u, acc.value(), score, cout);
+  }
+  const double t = block_err_sum<kFusedThreads>(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
 }
 
 // Merges each split row's segment partials in segment order and finishes it.
@@ -279,32 +322,7 @@ __global__ __launch_bounds__(kThreads) void k_kahan_super(PrArgs a, float* __res
     err = finish_f32(a, sp.v, acc.value(), score, cout);
   }
   double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[a.light_blocks + a.heavy_blocks + blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(kThreads) void k_f64_items(PrArgs a, const double* __restrict__ cin,
-                                                        double* __restrict__ cout,
-                                                        double* __restrict__ score) {
-  __shared__ double red[kThreads / 64];
-  if (a.ctrl->pad) return;
-  const int lane = lane_id();
-  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
-  double err = 0;
-  for (int64_t h = wave; h < a.n_items; h += nwaves) {
-    const PrItem it = a.items[h];
-    double acc = 0;
-    for (int64_t k = it.b + lane; k < it.e; k += 64) acc += cin[a.in_nbr[k]];
-    acc = wave_sum(acc);
-    if (lane == 0) {
-      if (it.slot < 0)
-        err += finish_f64(a, it.v, acc, score, cout);
-      else
-        a.partial[it.slot] = make_double2(acc, 0.0);
-    }
-  }
-  double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[a.light_blocks + blockIdx.x] = t;
+  if (threadIdx.x == 0) a.err_part[a.main_blocks + blockIdx.x] = t;
 }
 
 __global__ __launch_bounds__(kThreads) void k_f64_super(PrArgs a, double* __restrict__ cout,
@@ -320,40 +338,8 @@ __global__ __launch_bounds__(kThreads) void k_f64_super(PrArgs a, double* __rest
     err = finish_f64(a, sp.v, acc, score, cout);
   }
   double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[a.light_blocks + a.heavy_blocks + blockIdx.x] = t;
+  if (threadIdx.x == 0) a.err_part[a.main_blocks + blockIdx.x] = t;
 }
-
-// ---- FP64 -----------------------------------------------------------------
-
-__global__ __launch_bounds__(kThreads) void k_f64_light(PrArgs a, const double* __restrict__ cin,
-                                                        double* __restrict__ cout,
-                                                        double* __restrict__ score) {
-  __shared__ double red[kThreads / 64];
-  if (a.ctrl->pad) return;
-  const int sub = threadIdx.x & (kGroup - 1);
-  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
-  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
-  double err = 0;
-  const int64_t iters = (a.nloc + ngrp - 1) / ngrp;
-  for (int64_t it = 0; it < iters; it++) {
-    const int64_t u = grp + it * ngrp;
-    double acc = 0;
-    bool mine = false;
-    if (u < a.nloc) {
-      const int64_t s = a.in_off[u], e = a.in_off[u + 1];
-      if (e - s < a.heavy_threshold) {
-        mine = true;
-        for (int64_t k = s + sub; k < e; k += kGroup) acc += cin[a.in_nbr[k]];
-      }
-    }
-#pragma unroll
-    for (int d = kGroup / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kGroup);
-    if (mine && sub == 0) err += finish_f64(a, u, acc, score, cout);
-  }
-  double t = block_err_sum(err, red);
-  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
-}
-
 
 // ---- iteration bookkeeping --------------------------------------------------
 
@@ -365,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(PrArgs a) {
   __shared__ double red[kThreads / 64];
   PrCtrl* c = a.ctrl;
   if (c->pad) return;
-  const int np = a.light_blocks + a.heavy_blocks + a.super_blocks;
+  const int np = a.main_blocks + a.super_blocks;
   double x = 0;
   for (int i = threadIdx.x; i < np; i += kThreads) x += a.err_part[i];
   double t = block_err_sum(x, red);
@@ -411,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_init_f64(PrArgs a, double* score, 
 // The serial mode has no combine step: clear the combine blocks' partials.
 __global__ void k_zero_err(PrArgs a) {
   for (int i = threadIdx.x; i < a.super_blocks; i += blockDim.x)
-    a.err_part[a.light_blocks + a.heavy_blocks + i] = 0.0;
+    a.err_part[a.main_blocks + i] = 0.0;
 }
 
 __global__ void k_reset_ctrl(PrCtrl* c) {
@@ -438,32 +424,36 @@ PrArgs make_args(dmm_ctx* ctx) {
   a.light_blocks = pr_light_blocks(ctx);
   a.heavy_blocks = pr_heavy_blocks(ctx);
   a.super_blocks = pr_super_blocks(ctx);
+  a.main_blocks = a.light_blocks + a.heavy_blocks;
+  a.n_gather = ctx->n;
   a.ctrl = ctx->d_prctrl;
   a.err_trace = ctx->d_err_trace;
   a.prm = reinterpret_cast<const PrParams*>(ctx->d_prparams);
   return a;
 }
 
-void launch_iteration(dmm_ctx* ctx, const PrArgs& a, int mode, int it, const void* gcontrib) {
-  const int lb = a.light_blocks, hb = a.heavy_blocks;
+void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcontrib) {
   hipStream_t s = ctx->stream;
   const int sb = a.super_blocks;
-  if (mode == DMM_PR_FP64) {
-    const double* cin = gcontrib ? static_cast<const double*>(gcontrib) : ctx->d_contrib64[it & 1];
-    double* cout = ctx->d_contrib64[(it + 1) & 1];
-    hipLaunchKernelGGL(k_f64_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
-    hipLaunchKernelGGL(k_f64_items, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score64);
-    hipLaunchKernelGGL(k_f64_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score64);
-  } else {
+  if (mode == DMM_PR_FP32_SERIAL) {
     const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
     float* cout = ctx->d_contrib[(it + 1) & 1];
-    if (mode == DMM_PR_FP32_SERIAL) {
-      hipLaunchKernelGGL(k_serial_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_serial_heavy, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(kThreads), 0, s, a);
+    a.main_blocks = a.light_blocks + a.heavy_blocks;
+    hipLaunchKernelGGL(k_serial_light, dim3(a.light_blocks), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+    hipLaunchKernelGGL(k_serial_heavy, dim3(a.heavy_blocks), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(kThreads), 0, s, a);
+  } else {
+    const int fb = pr_fused_blocks(ctx);
+    a.main_blocks = fb;
+    if (mode == DMM_PR_FP64) {
+      const double* cin = gcontrib ? static_cast<const double*>(gcontrib) : ctx->d_contrib64[it & 1];
+      double* cout = ctx->d_contrib64[(it + 1) & 1];
+      hipLaunchKernelGGL(k_fused<double>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, ctx->d_score64);
+      hipLaunchKernelGGL(k_f64_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score64);
     } else {
-      hipLaunchKernelGGL(k_kahan_light, dim3(lb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_kahan_items, dim3(hb), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+      const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
+      float* cout = ctx->d_contrib[(it + 1) & 1];
+      hipLaunchKernelGGL(k_fused<float>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, ctx->d_score);
       hipLaunchKernelGGL(k_kahan_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score);
     }
   }
@@ -481,6 +471,12 @@ int pr_heavy_blocks(dmm_ctx* ctx) {
   const int64_t work = std::max(ctx->n_heavy, ctx->n_items);
   return static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8, ceil_div(work * 64, kThreads))));
+}
+
+int pr_fused_blocks(dmm_ctx* ctx) {
+  const int64_t lanes = std::max(ctx->nloc * kGroup, ctx->n_items * 64);
+  return static_cast<int>(
+      std::max<int64_t>(1, std::min<int64_t>(std::max(1, ctx->cu_count), ceil_div(lanes, kFusedThreads))));
 }
 
 int pr_super_blocks(dmm_ctx* ctx) {

@@ -229,7 +229,7 @@ def stats_dict(st):
     names = {L.DMM_STEP_TD: "td", L.DMM_STEP_BU: "bu", L.DMM_STEP_QB: "e", L.DMM_STEP_BQ: "c"}
     return {
         "steps": [(names.get(st.step_kind[i], "?"), st.step_in[i], st.step_out[i], st.step_edges[i],
-                   st.step_scout[i], st.step_ms[i]) for i in range(n)],
+                   st.step_scout[i], st.step_ms[i], st.step_kernel_ms[i]) for i in range(n)],
         "alg_bytes": st.alg_bytes, "bu_alg_bytes": st.bu_alg_bytes, "reached": st.reached,
         "reached_edges": st.reached_edges, "total_ms": st.total_ms, "bu_ms": st.bu_ms,
         "host_ms": st.host_ms,

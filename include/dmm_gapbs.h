@@ -177,6 +177,12 @@ typedef struct dmm_bfs_stats {
   int64_t step_edges[DMM_MAX_STEPS];     /* edges examined                */
   int64_t step_scout[DMM_MAX_STEPS];     /* TD: scout_count returned      */
   float step_ms[DMM_MAX_STEPS];          /* device time per step (trace=1) */
+  float step_kernel_ms[DMM_MAX_STEPS];   /* ... of the step's traversal
+                                            kernel alone: the TD expand or
+                                            BU sweep launch, without the
+                                            partial-sum reduction and queue
+                                            compaction (trace=1; 0 for
+                                            QB/BQ)                         */
   int64_t alg_bytes;                     /* algorithmic HBM bytes, whole BFS */
   int64_t bu_alg_bytes;                  /* ... of the bottom-up steps      */
   int64_t reached;                       /* vertices with parent >= 0       */

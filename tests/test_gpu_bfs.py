@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import csr_from_pairs
+import dist_mem_gapbs_amd._lib as L
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +62,11 @@ def test_dobfs_trace_mode_equals_device_loop(ctx, dmm, orc, graph):
         b = check_bfs(ctx, orc, og, s, trace=False)
         assert trace_of(a) == trace_of(b)
         assert all(a.step_ms[i] > 0 for i in range(a.num_steps))
+        for i in range(a.num_steps):
+            if a.step_kind[i] in (L.DMM_STEP_TD, L.DMM_STEP_BU):
+                assert 0 < a.step_kernel_ms[i] <= a.step_ms[i]
+            else:
+                assert a.step_kernel_ms[i] == 0
 
 
 @pytest.mark.parametrize("alpha,beta", [(1, 18), (10 ** 9, 18), (2, 1), (15, 10 ** 6)])
