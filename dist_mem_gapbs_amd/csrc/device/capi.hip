@@ -440,6 +440,26 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
   for (int64_t r = 0; r < rows; r++)
     if (ioff[r + 1] - ioff[r] >= ctx->heavy_threshold) heavy.push_back(static_cast<int32_t>(r));
   ctx->n_heavy = static_cast<int64_t>(heavy.size());
+  // When rows come in non-increasing in-degree order (the degree-ordered
+  // layout of an undirected graph) the light rows form one range after the
+  // heavy prefix and the rows without in-edges form the tail.
+  {
+    int64_t lb = 0;
+    while (lb < rows && ioff[lb + 1] - ioff[lb] >= ctx->heavy_threshold) lb++;
+    int64_t le = lb;
+    while (le < rows && ioff[le + 1] - ioff[le] > 0 && ioff[le + 1] - ioff[le] < ctx->heavy_threshold) le++;
+    bool ok = true;
+    for (int64_t r = le; r < rows && ok; r++) ok = ioff[r + 1] == ioff[r];
+    if (ok) {
+      ctx->pr_light_begin = lb;
+      ctx->pr_light_end = le;
+      ctx->pr_zero_begin = le;
+    } else {
+      ctx->pr_light_begin = 0;
+      ctx->pr_light_end = rows;
+      ctx->pr_zero_begin = rows;
+    }
+  }
   if ((rc = A((void**)&ctx->d_heavy, sizeof(int32_t) * std::max<size_t>(1, heavy.size()), "heavy list"))) return rc;
   if (!heavy.empty())
     DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_heavy, heavy.data(), sizeof(int32_t) * heavy.size(), hipMemcpyHostToDevice));

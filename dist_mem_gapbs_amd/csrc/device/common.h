@@ -140,6 +140,8 @@ struct dmm_ctx {
   double* d_score64 = nullptr;
   double* d_contrib64[2] = {nullptr, nullptr};
   double* d_err_part = nullptr;
+  // Rows of the light loop and the in-degree-0 tail (see the upload).
+  int64_t pr_light_begin = 0, pr_light_end = 0, pr_zero_begin = 0;
   int64_t err_part_cap = 0;
   dmm::PrCtrl* d_prctrl = nullptr;
   dmm::PrCtrl* h_prctrl = nullptr;

@@ -33,7 +33,8 @@ namespace dmm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGroup = 8;         // lanes per light vertex (KAHAN / FP64)
+constexpr int kGroup = 4;         // lanes per light vertex (KAHAN / FP64)
+constexpr int kUnroll = 8;        // gathers in flight per lane on heavy items
 constexpr int kFusedThreads = 1024;
 constexpr int kHubBytes = 128 * 1024;  // LDS copy of the first contributions
 
@@ -56,6 +57,8 @@ struct PrArgs {
   double2* partial;               // per-segment partial sums
   int64_t nloc, n_heavy, n_items, n_super;
   int64_t n_gather;               // entries of the gathered contribution vector
+  int64_t light_begin, light_end; // light-row range (heavy rows inside are skipped)
+  int64_t zero_begin;             // rows [zero_begin, nloc) have no in-edges
   int heavy_threshold;
   double* err_part;               // [light | heavy-or-items | super] blocks
   int light_blocks, heavy_blocks, super_blocks;
@@ -180,9 +183,10 @@ __global__ __launch_bounds__(kThreads) void k_serial_heavy(PrArgs a, const float
 // about half of all gathers on Kronecker graphs, which LDS serves several
 // times faster than L2.  Wavefronts then take the heavy items (rows with
 // in-degree >= heavy_threshold, or 2048-edge segments of split rows), one
-// item per wave with lanes striding its edge range and four gathers in
-// flight, and afterwards the light rows in 8-lane groups.  A split row's
-// segments store (sum, error) pairs for the combine kernel.
+// item per wave with lanes striding its edge range and kUnroll gathers in
+// flight, afterwards the light rows in kGroup-lane groups, and last the rows
+// without in-edges, one per thread.  A split row's segments store
+// (sum, error) pairs for the combine kernel.
 template <typename T>
 struct FusedAcc;
 
@@ -260,16 +264,15 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
     const PrItem it = a.items[h];
     FusedAcc<T> acc;
     int64_t k = it.b + lane;
-    for (; k + 192 < it.e; k += 256) {
-      const int32_t u0 = a.in_nbr[k], u1 = a.in_nbr[k + 64], u2 = a.in_nbr[k + 128], u3 = a.in_nbr[k + 192];
-      const T x0 = hub_gather(hub, hub_n, cin, u0);
-      const T x1 = hub_gather(hub, hub_n, cin, u1);
-      const T x2 = hub_gather(hub, hub_n, cin, u2);
-      const T x3 = hub_gather(hub, hub_n, cin, u3);
-      acc.add(x0);
-      acc.add(x1);
-      acc.add(x2);
-      acc.add(x3);
+    for (; k + 64 * (kUnroll - 1) < it.e; k += 64 * kUnroll) {
+      int32_t u[kUnroll];
+      T x[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) u[j] = a.in_nbr[k + 64 * j];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, u[j]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) acc.add(x[j]);
     }
     for (; k < it.e; k += 64) acc.add(hub_gather(hub, hub_n, cin, a.in_nbr[k]));
     acc.template reduce<64>(lane);
@@ -285,12 +288,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
   const int sub = threadIdx.x & (kGroup - 1);
   const int64_t grp = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kFusedThreads) / kGroup;
-  const int64_t rounds = (a.nloc + ngrp - 1) / ngrp;
+  const int64_t rounds = (a.light_end - a.light_begin + ngrp - 1) / ngrp;
   for (int64_t r = 0; r < rounds; r++) {
-    const int64_t u = grp + r * ngrp;
+    const int64_t u = a.light_begin + grp + r * ngrp;
     FusedAcc<T> acc;
     bool mine = false;
-    if (u < a.nloc) {
+    if (u < a.light_end) {
       const int64_t s = a.in_off[u], e = a.in_off[u + 1];
       if (e - s < a.heavy_threshold) {
         mine = true;
@@ -300,6 +303,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
     acc.template reduce<kGroup>(sub);
     if (mine && sub == 0) err += finish_t(a, u, acc.value(), score, cout);
   }
+  // Rows without in-edges: the empty sum.
+  for (int64_t u = a.zero_begin + int64_t(blockIdx.x) * kFusedThreads + threadIdx.x; u < a.nloc;
+       u += int64_t(gridDim.x) * kFusedThreads)
+    err += finish_t(a, u, T(0), score, cout);
   const double t = block_err_sum<kFusedThreads>(err, red);
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
 }
@@ -426,6 +433,9 @@ PrArgs make_args(dmm_ctx* ctx) {
   a.super_blocks = pr_super_blocks(ctx);
   a.main_blocks = a.light_blocks + a.heavy_blocks;
   a.n_gather = ctx->n;
+  a.light_begin = ctx->pr_light_begin;
+  a.light_end = ctx->pr_light_end;
+  a.zero_begin = ctx->pr_zero_begin;
   a.ctrl = ctx->d_prctrl;
   a.err_trace = ctx->d_err_trace;
   a.prm = reinterpret_cast<const PrParams*>(ctx->d_prparams);
