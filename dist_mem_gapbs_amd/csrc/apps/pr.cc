@@ -52,7 +52,7 @@ void PrintTopScores(const Graph& g, const std::vector<ScoreT>& scores) {
 }
 
 // Verification: the push-direction residual check of gapbs-verify/pr.cc:
-// 336-352 (one serial push iteration from the scores must change them by
+// 77-93 (one serial push iteration from the scores must change them by
 // less than the tolerance).  Like the reference's DMM build (src/pr.cc:
 // 87-103) the scores are also appended to pr_output.txt.
 bool PRVerifier(const Graph& g, const std::vector<ScoreT>& scores, double target_error) {

@@ -292,7 +292,7 @@ int32_t orc_pr_pull_f64(int64_t n, const int64_t* out_offs, const int64_t* in_of
   return iter;
 }
 
-/* gapbs-verify/pr.cc:336-352. */
+/* gapbs-verify/pr.cc:77-93. */
 double orc_pr_residual_f32(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,
                            const float* scores) {
   const float kDamp = 0.85f;

@@ -73,7 +73,7 @@ int32_t orc_pr_pull_f64(int64_t n, const int64_t* out_offs, const int64_t* in_of
                         const int32_t* in_neigh, int32_t max_iters, double epsilon,
                         int32_t exact_constants, double* scores, double* err_trace);
 
-/* Residual verifier (the push-direction check, gapbs-verify/pr.cc:336-352):
+/* Residual verifier (the push-direction check, gapbs-verify/pr.cc:77-93):
  * one serial push iteration from `scores`; returns sum_n |base + d *
  * incoming[n] - scores[n]| (FP32 terms, FP64 sum). */
 double orc_pr_residual_f32(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,
