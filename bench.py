@@ -185,6 +185,89 @@ def kron24_leg(dmm, args, tuned):
     return out
 
 
+def pe_share_leg(dmm, args):
+    """One PE's share at real shape: Kronecker scale --pe-scale in the
+    vertex-block layout for np = 8, PE 3 (src/partition.h:79-106).  The
+    inputs a multi-PE run would receive -- the all-gathered 64-lane frontier
+    words of every level and the full contribution vector -- come from a
+    whole-graph run on the same GPU; only the block kernels are timed, and
+    every block result is compared with the whole-graph result restricted
+    to the block."""
+    import dist_mem_gapbs_amd._lib as L
+    out = {"leg": "pe_share", "scale": args.pe_scale, "npes": 8, "pe": 3}
+    t0 = time.time()
+    g = dmm.build_graph(args.pe_scale, 16, uniform=False)
+    n = g.num_nodes
+    log(f"kron{args.pe_scale}: n={n} m={g.num_edges_directed} built in {time.time() - t0:.1f}s")
+    srcs = dmm.pick_sources(g, 64)
+    part = dmm.partition(n, 8, 3)
+    b, e = part["start"], part["end"]
+    oo = np.asarray(g.out_offsets)
+    deg = np.diff(oo)
+    out.update({"num_nodes": n, "num_edges_directed": int(g.num_edges_directed), "block": [b, e],
+                "block_edges": int(oo[e] - oo[b])})
+    # Whole-graph inputs.
+    with dmm.Context(0) as full:
+        t0 = time.time()
+        full.upload(g)
+        log(f"whole graph uploaded in {time.time() - t0:.1f}s")
+        _, bst = full.bfs_batch64(srcs, want_depth=False)
+        out["whole_graph_batch_ms"] = bst.total_ms
+        levels = []
+        while True:
+            try:
+                levels.append(full.batch_read_level(len(levels)))
+            except L.DmmError:
+                break
+        log(f"whole-graph batched BFS: {len(levels)} levels, {bst.total_ms:.1f} ms")
+        s1, _ = full.PageRankPull(1, 0.0, L.DMM_PR_FP32_SERIAL)
+        s2, _ = full.PageRankPull(2, 0.0, L.DMM_PR_FP32_SERIAL)
+    with np.errstate(divide="ignore"):
+        contrib = (s1 / deg.astype(np.float32)).astype(np.float32)
+    with dmm.Context(0) as blk:
+        blk.set_layout(L.DMM_LAYOUT_NATURAL)
+        blk.upload(num_nodes=n, out_offsets=oo[b:e + 1] - oo[b],
+                   out_neighs=np.asarray(g.out_neighs)[oo[b]:oo[e]], vbegin=b, vend=e,
+                   num_edges_directed=g.num_edges_directed)
+        seen = np.zeros(e - b, np.uint64)
+        lv_out, ok = [], True
+        tot_ms, tot_b = 0.0, 0
+        for lv in range(len(levels) - 1):
+            seen |= levels[lv][b:e]
+            nxt, st = blk.batch_block_level(levels[lv], seen)
+            same = bool(np.array_equal(nxt, levels[lv + 1][b:e]))
+            ok = ok and same
+            lv_out.append({"level": lv, "kernel_ms": st.kernel_ms, "edges_examined": st.edges_examined,
+                           "alg_bytes": st.alg_bytes,
+                           "gbps": st.alg_bytes / (st.kernel_ms / 1e3) / 1e9 if st.kernel_ms > 0 else None,
+                           "equals_whole_graph": same})
+            tot_ms += st.kernel_ms
+            tot_b += st.alg_bytes
+            log(f"block level {lv}: {st.kernel_ms:.3f} ms, equal={same}")
+        out["batch_levels"] = lv_out
+        out["batch_block_kernel_ms"] = tot_ms
+        out["batch_block_gbps"] = tot_b / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else None
+        out["batch_equals_whole_graph"] = ok
+        # PageRank: the block's iteration from the whole-graph contributions.
+        got, err, _ = blk.pr_block_iteration(contrib, s1[b:e], L.DMM_PR_FP32_SERIAL)
+        out["pr_serial_block_equals_whole_graph"] = bool(np.array_equal(got, s2[b:e]))
+        ms = []
+        sc = s1[b:e].copy()
+        for _ in range(20):
+            sc, err, kms = blk.pr_block_iteration(contrib, sc, L.DMM_PR_FP32_KAHAN)
+            ms.append(kms)
+        m_loc = int(oo[e] - oo[b])
+        alg = 8 * m_loc + 20 * (e - b) + 8
+        out["pr_kahan_block_iter_ms"] = float(np.median(ms))
+        out["pr_kahan_block_gbps"] = alg / (np.median(ms) / 1e3) / 1e9
+        out["pr_kahan_block_hbm_frac"] = alg / (np.median(ms) / 1e3) / HBM_PEAK
+        out["pr_kahan_block_max_abs_diff_vs_serial"] = float(
+            np.abs(blk.pr_block_iteration(contrib, s1[b:e], L.DMM_PR_FP32_KAHAN)[0].astype(np.float64)
+                   - s2[b:e]).max())
+    g.free()
+    print(json.dumps(out), flush=True)
+
+
 def only_leg(dmm, args):
     """One leg on Kronecker --big-scale, printed as JSON (profiling runs)."""
     import dist_mem_gapbs_amd._lib as L
@@ -227,6 +310,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise kernel that reads and writes 256 MiB "
                          "(scale reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pe-share", action="store_true",
+                    help="run only the one-PE-share leg (Kronecker --pe-scale, PE 3 of 8)")
+    ap.add_argument("--pe-scale", type=int, default=27)
     ap.add_argument("--only", choices=["bfs", "batch", "pr"],
                     help="run one leg alone on Kronecker --big-scale (for profiling)")
     args = ap.parse_args()
@@ -247,6 +333,9 @@ def main():
         log(f"calibration copy: {a.numel() * 4 >> 10} KiB read, {b.numel() * 4 >> 10} KiB written")
         del a, b
 
+    if args.pe_share:
+        pe_share_leg(dmm, args)
+        return
     if args.only:
         only_leg(dmm, args)
         return

@@ -92,6 +92,14 @@ class PrStats(C.Structure):
     ]
 
 
+class BlockLevelStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("edges_examined", C.c_int64),
+        ("alg_bytes", C.c_int64),
+    ]
+
+
 class Tuning(C.Structure):
     _fields_ = [
         ("alpha", C.c_int32),
@@ -154,6 +162,8 @@ SIGNATURES = {
     "dmm_state_set_bitmap": (_I32, [_P, _I32, _PU64]),
     "dmm_state_set_level": (_I32, [_P, _I32]),
     "dmm_bfs_batch64": (_I32, [_P, _PI32, _I32, _PU8, C.POINTER(BatchParams), C.POINTER(BatchStats)]),
+    "dmm_batch_read_level": (_I32, [_P, _I32, _PU64]),
+    "dmm_batch_block_level": (_I32, [_P, _PU64, _PU64, _PU64, C.POINTER(BlockLevelStats)]),
     "dmm_pr_pull": (_I32, [_P, _I32, C.c_double, _I32, _P, C.POINTER(PrStats)]),
     "dmm_pr_block_iteration": (_I32, [_P, _PF32, _PF32, _I32, _PD, _PD]),
     "dmm_autotune": (_I32, [_P, _PI32, _I32, _I32, C.POINTER(Tuning)]),

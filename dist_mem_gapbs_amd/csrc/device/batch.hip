@@ -189,6 +189,67 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
 // group can stop as soon as every needed lane is found.  Partials: edges
 // scanned, vertices of the next frontier (out-degree >= 1) and their
 // out-degree sum.
+// Pull over one row's in-edges [b, e) by an 8-lane group: the first round
+// reads kGroup edges (rows whose needed lanes are all found there stop
+// early), later rounds keep kSweepUnroll frontier-word loads in flight per
+// lane; the group ORs them and the scan stops once every needed lane is
+// found.  nz (may be NULL) is a
+// non-zero bitmap of the frontier words (bit v), with a summary bitmap nz2
+// (bit w set iff nz word w != 0, so 256 KB for 2^27 vertices, resident in
+// L2); both are probed before a word load, so sparse frontiers cost one L2
+// probe per edge.
+// Returns the end of the examined range; *found = the needed lanes found.
+constexpr int kSweepUnroll = 4;
+__device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr, int64_t b, int64_t e,
+                                             int sub, uint64_t need, const uint64_t* __restrict__ front,
+                                             const uint64_t* __restrict__ nz, const uint64_t* __restrict__ nz2,
+                                             uint64_t* found) {
+  uint64_t acc = 0;
+  int64_t stop = e;
+  int64_t k0 = b;
+  {
+    const int64_t k = k0 + sub;
+    int32_t v = k < e ? in_nbr[k] : -1;
+    if (nz && v >= 0 && !((nz2[v >> 12] >> ((v >> 6) & 63)) & 1ull)) v = -1;
+    if (nz && v >= 0 && !((nz[v >> 6] >> (v & 63)) & 1ull)) v = -1;
+    if (v >= 0) acc |= front[v];
+#pragma unroll
+    for (int d = kGroup / 2; d >= 1; d >>= 1) acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
+    k0 += kGroup;
+    if ((acc & need) == need) {
+      *found = acc & need;
+      return min<int64_t>(k0, e);
+    }
+  }
+  for (; k0 < e; k0 += kGroup * kSweepUnroll) {
+    int32_t v[kSweepUnroll];
+#pragma unroll
+    for (int j = 0; j < kSweepUnroll; j++) {
+      const int64_t k = k0 + j * kGroup + sub;
+      v[j] = k < e ? in_nbr[k] : -1;
+    }
+    if (nz) {
+#pragma unroll
+      for (int j = 0; j < kSweepUnroll; j++)
+        if (v[j] >= 0 && !((nz2[v[j] >> 12] >> ((v[j] >> 6) & 63)) & 1ull)) v[j] = -1;
+#pragma unroll
+      for (int j = 0; j < kSweepUnroll; j++)
+        if (v[j] >= 0 && !((nz[v[j] >> 6] >> (v[j] & 63)) & 1ull)) v[j] = -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kSweepUnroll; j++)
+      if (v[j] >= 0) acc |= front[v[j]];
+#pragma unroll
+    for (int d = kGroup / 2; d >= 1; d >>= 1) acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
+    if ((acc & need) == need) {
+      stop = min<int64_t>(k0 + kGroup * kSweepUnroll, e);
+      break;
+    }
+  }
+  *found = acc & need;
+  return stop;
+}
+
 __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
   const uint64_t* __restrict__ Fl = a.levels + level * a.nloc;
   uint64_t* __restrict__ Fn = a.levels + (level + 1) * a.nloc;
@@ -204,21 +265,8 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
     const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
     if (need && e > b) {
-      uint64_t acc = 0;
-      int64_t stop = e;
-      for (int64_t k0 = b; k0 < e; k0 += kGroup) {
-        const int64_t k = k0 + sub;
-        if (k < e) acc |= Fl[in_nbr[k]];
-#pragma unroll
-        for (int d = kGroup / 2; d >= 1; d >>= 1)
-          acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
-        if ((acc & need) == need) {
-          stop = min<int64_t>(k0 + kGroup, e);
-          break;
-        }
-      }
+      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, Fl, nullptr, nullptr, &nxt);
       scanned += stop - b;
-      nxt = acc & need;
     }
     if (sub == 0) {
       Fn[u] = nxt;
@@ -250,22 +298,23 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
 // per workgroup in ascending order; listed = nonzero word and out-degree >= 1.
 template <bool kWrite>
 __device__ void count_or_write(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
+  // Workgroup b owns the contiguous range [v0, v1) and walks it in tiles
+  // of kThreads consecutive vertices (coalesced); WRITE orders entries by
+  // a block scan per tile on top of the workgroup's prefix from k_bpost.
   const uint64_t* Fl = a.levels + level * a.nloc;
   const int64_t vpb = (a.nloc + gridDim.x - 1) / gridDim.x;
   const int64_t v0 = min(a.nloc, int64_t(blockIdx.x) * vpb), v1 = min(a.nloc, v0 + vpb);
-  const int64_t vpt = (vpb + kThreads - 1) / kThreads;
-  const int64_t t0 = min(v1, v0 + threadIdx.x * vpt), t1 = min(v1, t0 + vpt);
-  int64_t cnt = 0, degs = 0;
-  for (int64_t u = t0; u < t1; u++) {
-    if (Fl[u]) {
-      const int64_t d = a.out_off[u + 1] - a.out_off[u];
-      if (d > 0) {
-        cnt++;
-        degs += d;
+  if (!kWrite) {
+    int64_t cnt = 0, degs = 0;
+    for (int64_t u = v0 + threadIdx.x; u < v1; u += kThreads) {
+      if (Fl[u]) {
+        const int64_t d = a.out_off[u + 1] - a.out_off[u];
+        if (d > 0) {
+          cnt++;
+          degs += d;
+        }
       }
     }
-  }
-  if (!kWrite) {
     const int64_t sc = block_sum<kThreads>(cnt, L.red[0]);
     const int64_t sd = block_sum<kThreads>(degs, L.red[1]);
     if (threadIdx.x == 0) {
@@ -280,19 +329,22 @@ __device__ void count_or_write(const BArgs& a, BCtrl* c, SmallLds& L, int level)
   const int cur = c->cur;
   int32_t* oq = cur ? a.act0 : a.act1;
   int64_t* oqoff = cur ? a.aoff0 : a.aoff1;
-  int64_t tc, td;
-  int64_t pos = a.part[blockIdx.x * kPartWidth + 0] + block_excl_scan<kThreads>(cnt, L.scan_c, &tc);
-  int64_t run = a.part[blockIdx.x * kPartWidth + 1] + block_excl_scan<kThreads>(degs, L.scan_d, &td);
-  for (int64_t u = t0; u < t1; u++) {
-    if (Fl[u]) {
-      const int64_t d = a.out_off[u + 1] - a.out_off[u];
-      if (d > 0) {
-        oq[pos] = static_cast<int32_t>(u + a.vbegin);
-        oqoff[pos] = run;
-        pos++;
-        run += d;
-      }
+  int64_t pos = a.part[blockIdx.x * kPartWidth + 0];
+  int64_t run = a.part[blockIdx.x * kPartWidth + 1];
+  for (int64_t t = v0; t < v1; t += kThreads) {
+    const int64_t u = t + threadIdx.x;
+    int64_t d = 0;
+    if (u < v1 && Fl[u]) d = a.out_off[u + 1] - a.out_off[u];
+    const int64_t f = d > 0 ? 1 : 0;
+    int64_t tc, td;
+    const int64_t ec = block_excl_scan<kThreads>(f, L.scan_c, &tc);
+    const int64_t ed = block_excl_scan<kThreads>(d, L.scan_d, &td);
+    if (f) {
+      oq[pos + ec] = static_cast<int32_t>(u + a.vbegin);
+      oqoff[pos + ec] = run + ed;
     }
+    pos += tc;
+    run += td;
   }
 }
 
@@ -538,6 +590,68 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   return a;
 }
 
+// One bottom-up level over a PE block against the all-gathered frontier
+// words of every vertex (front, global IDs): the sweep of bu_phase with the
+// block's own seen words (lanes reached at the current level or before).
+__global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restrict__ in_off,
+                                                         const int32_t* __restrict__ in_nbr, int64_t nloc,
+                                                         const uint64_t* __restrict__ front,
+                                                         const uint64_t* __restrict__ fnz,
+                                                         const uint64_t* __restrict__ fnz2,
+                                                         const uint64_t* __restrict__ seen,
+                                                         uint64_t* __restrict__ next, int64_t* part) {
+  __shared__ int64_t red[kThreads / 64];
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
+  int64_t scanned = 0;
+  for (int64_t u = grp; u < nloc; u += ngrp) {
+    const uint64_t need = ~seen[u];
+    const int64_t b = in_off[u], e = in_off[u + 1];
+    uint64_t nxt = 0;
+    if (need && e > b) {
+      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, front, fnz, fnz2, &nxt);
+      scanned += stop - b;
+    }
+    if (sub == 0) next[u] = nxt;
+  }
+  if (sub != 0) scanned = 0;
+  const int64_t s = block_sum<kThreads>(scanned, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Non-zero bitmap of n words (bit v set iff w[v] != 0): one wavefront per
+// 64 words.  Applied twice it gives the frontier bitmap and its summary.
+__global__ __launch_bounds__(kThreads) void k_nz_build(const uint64_t* __restrict__ w, int64_t n,
+                                                      uint64_t* __restrict__ nz) {
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  const int64_t nw = (n + 63) >> 6;
+  for (int64_t c = wave; c < nw; c += nwaves) {
+    const int64_t v = c * 64 + lane;
+    const uint64_t m = __ballot(v < n && w[v] != 0);
+    if (lane == 0) nz[c] = m;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_block_sum(const int64_t* part, int n, int64_t* out) {
+  __shared__ int64_t red[kThreads / 64];
+  int64_t x = 0;
+  for (int i = threadIdx.x; i < n; i += kThreads) x += part[i];
+  const int64_t s = block_sum<kThreads>(x, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// Level words of the last whole-graph run in the caller's vertex IDs.
+__global__ __launch_bounds__(kThreads) void k_level_out(const uint64_t* __restrict__ lw,
+                                                       const int32_t* __restrict__ perm, int64_t n,
+                                                       uint64_t* __restrict__ out) {
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t v = gtid; v < n; v += gsize) out[v] = lw[perm ? perm[v] : v];
+}
+
 int ensure_levels(dmm_ctx* ctx, int64_t cap) {
   if (ctx->d_levels && ctx->level_cap >= cap) return DMM_OK;
   if (ctx->d_levels) dev_free(ctx->d_levels);
@@ -662,6 +776,8 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
                                         ctx->stream));
       DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
+    ctx->batch_valid = true;
+    ctx->batch_last_level = h->level;
     if (stats) {
       int64_t* out = ctx->d_scratch64;
       hipLaunchKernelGGL(k_breach, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
@@ -674,6 +790,78 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
     return DMM_OK;
   }
   return set_err(ctx, DMM_ERR_CAPACITY, "batched BFS level storage could not be grown");
+}
+
+int batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words) {
+  if (!ctx->batch_valid) return set_err(ctx, DMM_ERR_STATE, "no batched BFS result on the ctx");
+  if (level < 0 || level > ctx->batch_last_level || level >= ctx->level_cap)
+    return set_err(ctx, DMM_ERR_INVALID, "level %d outside [0, %d]", level, ctx->batch_last_level);
+  if (!ctx->d_words_tmp) {
+    int rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_words_tmp), sizeof(uint64_t) * ctx->nloc,
+                       "level word staging");
+    if (rc) return rc;
+  }
+  const int blocks = static_cast<int>(
+      std::max<int64_t>(1, std::min<int64_t>(int64_t(std::max(1, ctx->cu_count)) * 8, ceil_div(ctx->nloc, kThreads))));
+  hipLaunchKernelGGL(k_level_out, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+                     ctx->d_levels + int64_t(level) * ctx->nloc,
+                     ctx->relabeled ? static_cast<const int32_t*>(ctx->d_perm) : nullptr, ctx->nloc,
+                     ctx->d_words_tmp);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(words, ctx->d_words_tmp, sizeof(uint64_t) * ctx->nloc, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
+}
+
+int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen, uint64_t* next,
+                      dmm_block_level_stats* stats) {
+  int rc;
+  if (!ctx->d_gfront &&
+      (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_gfront), sizeof(uint64_t) * ctx->n, "global frontier words")))
+    return rc;
+  if (!ctx->d_bseen &&
+      (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bseen), sizeof(uint64_t) * ctx->nloc, "block seen words")))
+    return rc;
+  if (!ctx->d_bnext &&
+      (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bnext), sizeof(uint64_t) * ctx->nloc, "block next words")))
+    return rc;
+  // Frontier bitmap (ceil(n/64) words) followed by its summary.
+  if (!ctx->d_gnz &&
+      (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_gnz),
+                      sizeof(uint64_t) * ((ctx->n + 63) / 64 + ((ctx->n + 63) / 64 + 63) / 64), "global frontier bitmaps")))
+    return rc;
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_gfront, front, sizeof(uint64_t) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bseen, seen, sizeof(uint64_t) * ctx->nloc, hipMemcpyHostToDevice, ctx->stream));
+  const int blocks = static_cast<int>(std::min<int64_t>(
+      int64_t(std::max(1, ctx->cu_count)) * std::max(1, std::min(kMaxGridPerCU, ctx->tuning.grid_per_cu)),
+      ctx->part_cap));
+  // The timed region starts after the host copies have landed.
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+  const int64_t nzw = (ctx->n + 63) / 64;
+  hipLaunchKernelGGL(k_nz_build, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gfront, ctx->n, ctx->d_gnz);
+  hipLaunchKernelGGL(k_nz_build, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gnz, nzw, ctx->d_gnz + nzw);
+  hipLaunchKernelGGL(k_block_sweep, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_in_off, ctx->d_in_nbr,
+                     ctx->nloc, ctx->d_gfront, ctx->d_gnz, ctx->d_gnz + nzw, ctx->d_bseen, ctx->d_bnext,
+                     ctx->d_part);
+  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_part, blocks, ctx->d_scratch64);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  int64_t scanned = 0;
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(next, ctx->d_bnext, sizeof(uint64_t) * ctx->nloc, hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(&scanned, ctx->d_scratch64, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+    stats->kernel_ms = ms;
+    stats->edges_examined = scanned;
+    // seen read, offsets, next write (24 B per block vertex) + 12 B per
+    // examined in-edge (4 B ID + 8 B frontier word).
+    stats->alg_bytes = 24 * ctx->nloc + 8 + 12 * scanned;
+  }
+  return DMM_OK;
 }
 
 }  // namespace dmm

@@ -81,7 +81,8 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_contrib64[0], ctx->d_contrib64[1], ctx->d_err_part, ctx->d_heavy,
                   ctx->d_gcontrib, ctx->d_seen, ctx->d_levels, ctx->d_bact, ctx->d_baoff,
                   ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
-                  ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out};
+                  ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
+                  ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -104,6 +105,9 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_perm = ctx->d_inv = ctx->d_parent_out = ctx->d_depth_out = nullptr;
   ctx->d_score_out = nullptr;
   ctx->d_score64_out = nullptr;
+  ctx->d_words_tmp = ctx->d_gfront = ctx->d_bseen = ctx->d_bnext = ctx->d_gnz = nullptr;
+  ctx->batch_valid = false;
+  ctx->batch_last_level = -1;
   ctx->relabeled = false;
   ctx->h_perm.clear();
   ctx->h_perm.shrink_to_fit();
@@ -790,10 +794,31 @@ int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t*
   if (stats) std::memset(stats, 0, sizeof(*stats));
   int32_t isrc[64];
   for (int i = 0; i < nsrc; i++) isrc[i] = to_internal(ctx, sources[i]);
+  ctx->batch_valid = false;
   rc = dmm::batch_run(ctx, isrc, nsrc, depth, params, stats);
   if (rc) return rc;
   if (stats) stats->host_ms = now_ms() - t0;
   return DMM_OK;
+}
+
+int dmm_batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!words) return set_err(ctx, DMM_ERR_INVALID, "words is NULL");
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return dmm::batch_read_level(ctx, level, words);
+}
+
+int dmm_batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen, uint64_t* next,
+                          dmm_block_level_stats* stats) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!front || !seen || !next) return set_err(ctx, DMM_ERR_INVALID, "NULL word buffer");
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (ctx->relabeled)
+    return set_err(ctx, DMM_ERR_STATE, "dmm_batch_block_level works on the natural layout (dmm_ctx_set_layout)");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return dmm::batch_block_level(ctx, front, seen, next, stats);
 }
 
 // ---------------------------------------------------------------------------

@@ -176,6 +176,13 @@ struct dmm_ctx {
   int64_t bdepth_cap = 0;
   hipGraphExec_t batch_graph = nullptr;
   hipGraphExec_t batch_cont = nullptr;
+  bool batch_valid = false;         // a finished run's level words are held
+  int batch_last_level = -1;
+  uint64_t* d_words_tmp = nullptr;  // nloc words (level read-out)
+  uint64_t* d_gfront = nullptr;     // n words: all-gathered frontier (PE block)
+  uint64_t* d_bseen = nullptr;      // nloc words
+  uint64_t* d_bnext = nullptr;      // nloc words
+  uint64_t* d_gnz = nullptr;        // non-zero bitmap of d_gfront + its summary
   int batch_graph_key = -1;
 
   dmm::Tuning tuning;

@@ -588,6 +588,8 @@ int pr_block_iteration(dmm_ctx* ctx, const float* gcontrib_host, float* scores_i
                                     hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_reset_ctrl, dim3(1), dim3(1), 0, ctx->stream, ctx->d_prctrl);
   PrArgs a = make_args(ctx);
+  // The timed region starts after the host copies have landed.
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
   launch_iteration(ctx, a, mode, 0, ctx->d_gcontrib);
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));

@@ -188,6 +188,28 @@ class Context:
                                               C.byref(st)))
         return depth, st
 
+    def batch_read_level(self, level):
+        """Level words (uint64[nloc], caller's IDs) of the last bfs_batch64 run."""
+        w = np.empty(self.nloc, np.uint64)
+        self._check(self._lib.dmm_batch_read_level(self._h, int(level), L.ptr(w, C.c_uint64)))
+        return w
+
+    def batch_block_level(self, front, seen):
+        """One bottom-up batched level over the uploaded PE block.
+
+        front: uint64[num_nodes] all-gathered frontier words; seen:
+        uint64[nloc] lanes reached at or before the current level.  Returns
+        (next uint64[nloc], BlockLevelStats)."""
+        f = np.ascontiguousarray(front, dtype=np.uint64)
+        sn = np.ascontiguousarray(seen, dtype=np.uint64)
+        if f.shape[0] != self.n or sn.shape[0] != self.nloc:
+            raise ValueError("front must have num_nodes words and seen one word per block vertex")
+        nxt = np.empty(self.nloc, np.uint64)
+        st = L.BlockLevelStats()
+        self._check(self._lib.dmm_batch_block_level(self._h, L.ptr(f, C.c_uint64), L.ptr(sn, C.c_uint64),
+                                                    L.ptr(nxt, C.c_uint64), C.byref(st)))
+        return nxt, st
+
     # ---- PageRank ----------------------------------------------------------------------
     def PageRankPull(self, max_iters=20, epsilon=0.0, mode=L.DMM_PR_FP32_SERIAL, read=True):
         """Returns (scores, stats); scores float32 (FP32 modes) or float64."""

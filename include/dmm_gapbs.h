@@ -275,6 +275,35 @@ int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc,
                     uint8_t* depth, const dmm_batch_params* params,
                     dmm_batch_stats* stats);
 
+/* Level words of the last dmm_bfs_batch64 run on this ctx, in the caller's
+ * vertex IDs: words[v] (uint64[block rows]) has lane s set iff vertex v is
+ * at depth `level` from source s.  Valid levels are 0 .. the last level
+ * the run reached.  No reference counterpart: state access for the
+ * PE-share runs (the all-gathered frontier of dmm_batch_block_level). */
+int dmm_batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words);
+
+typedef struct dmm_block_level_stats {
+  double kernel_ms;        /* device time of the level: frontier bitmap
+                              build + sweep                               */
+  int64_t edges_examined;  /* in-edges read before every lane was found   */
+  int64_t alg_bytes;       /* 24 B per block vertex + 12 B per examined edge */
+} dmm_block_level_stats;
+
+/* One bottom-up level of the batched BFS over the uploaded PE block: the
+ * pull sweep of dmm_bfs_batch64 with the frontier all-gathered from every
+ * PE, as one PE of the vertex-block layout (src/partition.h:69-161) would
+ * run it (the per-PE BU step of src/bfs.cc:57-78, 64 sources at once).
+ *   front  uint64[num_nodes]  lanes whose frontier holds v at the current
+ *                             level, for every vertex (global IDs)
+ *   seen   uint64[rows]       lanes that reached each block vertex at the
+ *                             current level or before
+ *   next   uint64[rows]       out: lanes reaching each block vertex at the
+ *                             next level (OR of front over its in-
+ *                             neighbours, minus seen)
+ * Requires the natural layout (dmm_ctx_set_layout before the upload). */
+int dmm_batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
+                          uint64_t* next, dmm_block_level_stats* stats);
+
 /* ======================================================================== */
 /* PageRank (pull)                                                           */
 /* ======================================================================== */

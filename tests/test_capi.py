@@ -38,16 +38,16 @@ def test_ctypes_table_matches_header(dmm):
 def test_struct_sizes_match_header(dmm):
     # Compile a probe against the header and compare sizeof with ctypes.
     from dist_mem_gapbs_amd import _lib as L
-    src = ("#include \"dmm_gapbs.h\"\n#include <stdio.h>\nint main(){printf(\"%zu %zu %zu %zu %zu\\n\","
+    src = ("#include \"dmm_gapbs.h\"\n#include <stdio.h>\nint main(){printf(\"%zu %zu %zu %zu %zu %zu\\n\","
            "sizeof(dmm_bfs_stats),sizeof(dmm_batch_stats),sizeof(dmm_pr_stats),sizeof(dmm_tuning),"
-           "sizeof(dmm_batch_params));return 0;}\n")
+           "sizeof(dmm_batch_params),sizeof(dmm_block_level_stats));return 0;}\n")
     tmp = "/tmp/dmm_sizeof_probe"
     with open(tmp + ".c", "w") as f:
         f.write(src)
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", tmp, tmp + ".c"], check=True)
     got = [int(x) for x in subprocess.run([tmp], capture_output=True, text=True).stdout.split()]
     exp = [C.sizeof(L.BfsStats), C.sizeof(L.BatchStats), C.sizeof(L.PrStats), C.sizeof(L.Tuning),
-           C.sizeof(L.BatchParams)]
+           C.sizeof(L.BatchParams), C.sizeof(L.BlockLevelStats)]
     assert got == exp
 
 

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import csr_from_pairs
+import dist_mem_gapbs_amd._lib as L
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +110,45 @@ def test_uneven_batch_disjoint_union(ctx, dmm, orc):
     srcs = np.array(srcs, np.int32)
     depth, st = ctx.bfs_batch64(srcs)
     assert np.array_equal(depth, expected(orc, og, srcs))
+
+
+def level_words(depth, level):
+    """uint64 words with lane s set where depth[s, v] == level."""
+    w = np.zeros(depth.shape[1], np.uint64)
+    for s in range(depth.shape[0]):
+        w |= (depth[s] == level).astype(np.uint64) << np.uint64(s)
+    return w
+
+
+def test_block_level_equals_full_graph(ctx, dmm, orc, graph):
+    """One PE's block (src/partition.h, np = 8, pe = 3): fed the all-gathered
+    level words of a whole-graph batched run, each bottom-up block level
+    equals the whole-graph run's next level restricted to the block."""
+    hg = graph(14, 16, False)
+    g = orc.CSR.of(hg)
+    n = g.n
+    srcs = dmm.pick_sources(hg, 64)
+    ctx.upload(hg)
+    depth, _ = ctx.bfs_batch64(srcs)
+    levels = []
+    while True:
+        try:
+            levels.append(ctx.batch_read_level(len(levels)))
+        except L.DmmError as ex:
+            assert ex.code == L.DMM_ERR_INVALID
+            break
+    assert len(levels) >= 3
+    for lv, w in enumerate(levels):
+        assert np.array_equal(w, level_words(depth, lv)), lv
+    assert not levels[-1].any()
+    p = dmm.partition(n, 8, 3)
+    b, e = p["start"], p["end"]
+    ctx.upload(num_nodes=n, out_offsets=g.oo[b:e + 1] - g.oo[b], out_neighs=g.on[g.oo[b]:g.oo[e]],
+               vbegin=b, vend=e, num_edges_directed=g.ned)
+    seen = np.zeros(e - b, np.uint64)
+    for lv in range(len(levels) - 1):
+        seen |= levels[lv][b:e]
+        nxt, st = ctx.batch_block_level(levels[lv], seen)
+        assert np.array_equal(nxt, levels[lv + 1][b:e]), lv
+        assert st.kernel_ms > 0
+        assert 0 <= st.edges_examined <= g.oo[e] - g.oo[b]
