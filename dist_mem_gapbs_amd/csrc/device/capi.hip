@@ -82,7 +82,10 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_gcontrib, ctx->d_seen, ctx->d_levels, ctx->d_bact, ctx->d_baoff,
                   ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
                   ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
-                  ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz};
+                  ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
+                  ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
+                  ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
+                  ctx->d_piece_part};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -108,6 +111,13 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_words_tmp = ctx->d_gfront = ctx->d_bseen = ctx->d_bnext = ctx->d_gnz = nullptr;
   ctx->batch_valid = false;
   ctx->batch_last_level = -1;
+  ctx->tiles_ready = false;
+  ctx->d_tl_off = ctx->d_tl_poff = ctx->d_twin = nullptr;
+  ctx->d_tl_nbr = ctx->d_tl_pslot = nullptr;
+  ctx->d_titems = ctx->d_tsupers = ctx->d_tpartial = ctx->d_tchunks = ctx->d_piece_part = nullptr;
+  ctx->d_tsrc = nullptr;
+  ctx->n_titems = ctx->n_tsuper = ctx->n_chunks = ctx->n_pieces = ctx->tile_edges = ctx->list_edges = 0;
+  ctx->n_windows = 0;
   ctx->relabeled = false;
   ctx->h_perm.clear();
   ctx->h_perm.shrink_to_fit();
@@ -174,10 +184,17 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[0], sizeof(float) * nl, "contrib"))) return rc;
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
   }
+  // Compensated mode on a degree-ordered whole graph: window-tiled layout.
+  if (mode == DMM_PR_FP32_KAHAN && ctx->relabeled && ctx->nloc == ctx->n && !ctx->tiles_ready) {
+    if ((rc = dmm::pr_tiles_build(ctx))) return rc;
+    if (ctx->d_err_part) dmm::dev_free(ctx->d_err_part);  // resized below
+    ctx->d_err_part = nullptr;
+    drop_graphs(ctx);
+  }
   if (!ctx->d_err_part) {
     const int64_t parts =
         std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx), dmm::pr_fused_blocks(ctx)) +
-        dmm::pr_super_blocks(ctx);
+        dmm::pr_light_blocks(ctx) + std::max<int64_t>(dmm::pr_super_blocks(ctx), dmm::ceil_div(ctx->n_tsuper, 256) + 1);
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
     ctx->err_part_cap = parts;
   }

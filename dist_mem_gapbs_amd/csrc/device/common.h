@@ -79,6 +79,7 @@ struct dmm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
+
   std::vector<hipEvent_t> step_events;  // per-step timing in trace mode
   hipEvent_t trace_ev_begin = nullptr;  // set while a traced step runs
   hipEvent_t trace_ev_end = nullptr;
@@ -140,6 +141,25 @@ struct dmm_ctx {
   double* d_score64 = nullptr;
   double* d_contrib64[2] = {nullptr, nullptr};
   double* d_err_part = nullptr;
+  // Window-tiled compensated PageRank (pr_tiles.hip).
+  bool tiles_ready = false;
+  int64_t* d_tl_off = nullptr;    // per row: hub + short-run in-edges
+  int32_t* d_tl_nbr = nullptr;
+  int64_t* d_tl_poff = nullptr;   // per row: its piece slots
+  int32_t* d_tl_pslot = nullptr;
+  void* d_titems = nullptr;       // TItem
+  int64_t n_titems = 0;
+  void* d_tsupers = nullptr;      // PrSuper of split rows
+  int64_t n_tsuper = 0;
+  void* d_tpartial = nullptr;     // double2 per split-row segment
+  uint16_t* d_tsrc = nullptr;     // 64 per chunk
+  void* d_tchunks = nullptr;      // TileChunk
+  int64_t n_chunks = 0;
+  int64_t* d_twin = nullptr;      // first chunk of each window (+ end)
+  int32_t n_windows = 0;
+  void* d_piece_part = nullptr;   // float2 (sum, error) per piece
+  int64_t n_pieces = 0;
+  int64_t tile_edges = 0, list_edges = 0;
   // Rows of the light loop and the in-degree-0 tail (see the upload).
   int64_t pr_light_begin = 0, pr_light_end = 0, pr_zero_begin = 0;
   int64_t err_part_cap = 0;

@@ -213,6 +213,7 @@ struct FusedAcc<float> {
   }
   __device__ __forceinline__ float value() const { return c.value(); }
   __device__ __forceinline__ double2 partial() const { return make_double2(c.s, c.c); }
+  __device__ __forceinline__ void merge(float2 p) { c.merge(p.x, p.y); }
 };
 
 template <>
@@ -235,6 +236,26 @@ __device__ __forceinline__ double finish_t(const PrArgs& a, int64_t lu, double i
   return finish_f64(a, lu, in, score, cout);
 }
 
+// Block-wide copy of `bytes` (a multiple of 16) from src into LDS with all
+// 16-byte loads of a thread issued before its stores (one round trip
+// instead of one per element); entries at or beyond cnt are not read.
+template <typename T, int kBytes>
+__device__ __forceinline__ void lds_fill(T* dst, const T* __restrict__ src, int cnt) {
+  constexpr int kVec = 16 / sizeof(T);
+  constexpr int kPer = kBytes / 16 / kFusedThreads;
+  static_assert(kBytes % (16 * kFusedThreads) == 0, "fill size");
+  if (cnt * static_cast<int>(sizeof(T)) >= kBytes) {
+    float4 v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) v[i] = reinterpret_cast<const float4*>(src)[threadIdx.x + i * kFusedThreads];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) reinterpret_cast<float4*>(dst)[threadIdx.x + i * kFusedThreads] = v[i];
+  } else {
+    for (int j = threadIdx.x; j < cnt; j += kFusedThreads) dst[j] = src[j];
+  }
+  (void)kVec;
+}
+
 template <typename T>
 __device__ __forceinline__ T hub_gather(const T* hub, int hub_n, const T* __restrict__ cin, int32_t u) {
   T x;
@@ -252,7 +273,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
   __shared__ double red[kFusedThreads / 64];
   if (a.ctrl->pad) return;  // converged
   const int hub_n = static_cast<int>(min<int64_t>(kHubBytes / sizeof(T), a.n_gather));
-  for (int j = threadIdx.x; j < hub_n; j += kFusedThreads) hub[j] = cin[j];
+  lds_fill<T, kHubBytes>(hub, cin, hub_n);
   __syncthreads();
 
   double err = 0;
@@ -309,6 +330,185 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
     err += finish_t(a, u, T(0), score, cout);
   const double t = block_err_sum<kFusedThreads>(err, red);
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+}
+
+// ---- FP32_KAHAN, window-tiled (degree-ordered whole graphs) ----------------
+
+// Pieces of every window w >= 1: persistent blocks, each owning a contiguous
+// range of chunks (window-major), copy one window of contributions into LDS
+// at a time; a wavefront reduces kTileIlp chunks at once (independent
+// chains), each with a segmented inclusive scan over its 64 lanes built
+// from DPP lane moves (row_shr 1/2/4/8 inside 16-lane rows, then
+// row_bcast 15 and 31 across rows), restarted at every piece head.  A piece
+// holds at most 64 FP32 values; they accumulate in FP64 (relative error
+// <= 64 * 2^-53) and the piece enters its row's compensated FP32 sum as the
+// unevaluated pair (hi, lo) = (float(s), float(s - hi)).
+constexpr int kTileIlp = 4;
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), kCtrl, kRowMask, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kCtrl, kRowMask, 0xf, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Inclusive scan of v over lanes [seg, lane] (seg = the lane's piece head).
+__device__ __forceinline__ double seg_scan(double v, int lane, int seg) {
+  double t;
+  t = dpp_move<0x111, 0xf>(v);  // row_shr:1
+  if ((lane & 15) >= 1 && lane - 1 >= seg) v += t;
+  t = dpp_move<0x112, 0xf>(v);  // row_shr:2
+  if ((lane & 15) >= 2 && lane - 2 >= seg) v += t;
+  t = dpp_move<0x114, 0xf>(v);  // row_shr:4
+  if ((lane & 15) >= 4 && lane - 4 >= seg) v += t;
+  t = dpp_move<0x118, 0xf>(v);  // row_shr:8
+  if ((lane & 15) >= 8 && lane - 8 >= seg) v += t;
+  t = dpp_move<0x142, 0xa>(v);  // row_bcast:15 -> rows 1 and 3 get lanes 15 / 47
+  if (((lane >> 4) & 1) && (lane & ~15) - 1 >= seg) v += t;
+  t = dpp_move<0x143, 0xc>(v);  // row_bcast:31 -> rows 2 and 3 get lane 31
+  if (lane >= 32 && 31 >= seg) v += t;
+  return v;
+}
+__global__ __launch_bounds__(kFusedThreads) void k_tile(const TileChunk* __restrict__ chunks, int64_t n_chunks,
+                                                        const int64_t* __restrict__ win_start,
+                                                        const uint16_t* __restrict__ src,
+                                                        const float* __restrict__ cin, int64_t n_gather,
+                                                        float2* __restrict__ piece_part, const PrCtrl* ctrl) {
+  __shared__ float win[kTileWindow];
+  if (ctrl->pad) return;  // converged
+  const int64_t per = (n_chunks + gridDim.x - 1) / gridDim.x;
+  const int64_t c0 = min(n_chunks, int64_t(blockIdx.x) * per), c1 = min(n_chunks, c0 + per);
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t below = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  for (int64_t c = c0; c < c1;) {
+    const int32_t w = chunks[c].window;
+    const int64_t wend = min(c1, win_start[w + 1]);
+    const int64_t base = int64_t(w) * kTileWindow;
+    const int cnt = static_cast<int>(min<int64_t>(kTileWindow, n_gather - base));
+    __syncthreads();  // the previous window's readers are done
+    lds_fill<float, kTileWindow * 4>(win, cin + base, cnt);
+    __syncthreads();
+    constexpr int kWaves = kFusedThreads / 64;
+    for (int64_t cb = c + wave; cb < wend; cb += kWaves * kTileIlp) {
+      TileChunk ch[kTileIlp];
+      uint16_t e[kTileIlp];
+#pragma unroll
+      for (int j = 0; j < kTileIlp; j++) {
+        const int64_t cc = cb + j * kWaves;
+        if (cc < wend) {
+          ch[j] = chunks[cc];
+          e[j] = src[cc * 64 + lane];
+        } else {
+          ch[j] = TileChunk{0, 0, 1ull, 0ull};
+          e[j] = 0;
+        }
+      }
+      double acc[kTileIlp];
+#pragma unroll
+      for (int j = 0; j < kTileIlp; j++) {
+        const bool valid = (ch[j].valid >> lane) & 1ull;
+        const float xv = win[valid ? e[j] : 0];
+        const int seg = 63 - __clzll(static_cast<long long>(ch[j].head & below));
+        acc[j] = seg_scan(valid ? static_cast<double>(xv) : 0.0, lane, seg);
+      }
+#pragma unroll
+      for (int j = 0; j < kTileIlp; j++) {
+        const uint64_t tails = ch[j].valid & ((ch[j].head >> 1) | ~(ch[j].valid >> 1));
+        if ((tails >> lane) & 1ull) {
+          const int rank = __popcll(tails & ((1ull << lane) - 1));
+          const float hi = static_cast<float>(acc[j]);
+          const float lo = static_cast<float>(acc[j] - static_cast<double>(hi));
+          piece_part[ch[j].first_slot + rank] = make_float2(hi, lo);
+        }
+      }
+    }
+    c = wend;
+  }
+}
+
+struct TiledArgs {
+  const int64_t* off;      // row lists (hub + short runs)
+  const int32_t* nbr;
+  const int64_t* poff;     // row piece slots
+  const int32_t* pslot;
+  const float2* ppart;     // piece partials of this iteration
+  const TItem* items;
+  int64_t n_items;
+};
+
+// Row pass of the tiled mode: k_fused over the row lists, and every row
+// (or split-row segment) merges its pieces' partials, lanes striding the
+// slots in window order, before its fixed butterfly.
+__global__ __launch_bounds__(kFusedThreads) void k_fused_tiled(PrArgs a, TiledArgs t,
+                                                               const float* __restrict__ cin,
+                                                               float* __restrict__ cout,
+                                                               float* __restrict__ score) {
+  __shared__ float hub[kHubBytes / sizeof(float)];
+  __shared__ double red[kFusedThreads / 64];
+  if (a.ctrl->pad) return;
+  const int hub_n = static_cast<int>(min<int64_t>(kHubBytes / sizeof(float), a.n_gather));
+  lds_fill<float, kHubBytes>(hub, cin, hub_n);
+  __syncthreads();
+  double err = 0;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kFusedThreads) >> 6;
+  for (int64_t h = wave; h < t.n_items; h += nwaves) {
+    const TItem it = t.items[h];
+    FusedAcc<float> acc;
+    int64_t k = it.b + lane;
+    for (; k + 64 * (kUnroll - 1) < it.e; k += 64 * kUnroll) {
+      int32_t u[kUnroll];
+      float x[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) u[j] = t.nbr[k + 64 * j];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, u[j]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) acc.add(x[j]);
+    }
+    for (; k < it.e; k += 64) acc.add(hub_gather(hub, hub_n, cin, t.nbr[k]));
+    for (int64_t p = it.pb + lane; p < it.pe; p += 64) acc.merge(t.ppart[t.pslot[p]]);
+    acc.reduce<64>(lane);
+    if (lane == 0) {
+      if (it.slot < 0)
+        err += finish_f32(a, it.v, acc.value(), score, cout);
+      else
+        a.partial[it.slot] = acc.partial();
+    }
+  }
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kFusedThreads) / kGroup;
+  const int64_t rounds = (a.zero_begin + ngrp - 1) / ngrp;
+  for (int64_t r = 0; r < rounds; r++) {
+    const int64_t u = grp + r * ngrp;
+    FusedAcc<float> acc;
+    bool mine = false;
+    if (u < a.zero_begin) {
+      const int64_t s = t.off[u], e = t.off[u + 1];
+      if (e - s < a.heavy_threshold) {
+        mine = true;
+        // The row's list edges, then its pieces, as one strided range.
+        const int64_t pb = t.poff[u], np = t.poff[u + 1] - pb, nl = e - s;
+        for (int64_t k = sub; k < nl + np; k += kGroup) {
+          if (k < nl)
+            acc.add(hub_gather(hub, hub_n, cin, t.nbr[s + k]));
+          else
+            acc.merge(t.ppart[t.pslot[pb + k - nl]]);
+        }
+      }
+    }
+    acc.reduce<kGroup>(sub);
+    if (mine && sub == 0) err += finish_f32(a, u, acc.value(), score, cout);
+  }
+  for (int64_t u = a.zero_begin + int64_t(blockIdx.x) * kFusedThreads + threadIdx.x; u < a.nloc;
+       u += int64_t(gridDim.x) * kFusedThreads)
+    err += finish_f32(a, u, 0.0f, score, cout);
+  const double tt = block_err_sum<kFusedThreads>(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = tt;
 }
 
 // Merges each split row's segment partials in segment order and finishes it.
@@ -460,6 +660,26 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       double* cout = ctx->d_contrib64[(it + 1) & 1];
       hipLaunchKernelGGL(k_fused<double>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, ctx->d_score64);
       hipLaunchKernelGGL(k_f64_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score64);
+    } else if (ctx->tiles_ready && !gcontrib) {
+      const float* cin = ctx->d_contrib[it & 1];
+      float* cout = ctx->d_contrib[(it + 1) & 1];
+      TiledArgs t;
+      t.off = ctx->d_tl_off;
+      t.nbr = ctx->d_tl_nbr;
+      t.poff = ctx->d_tl_poff;
+      t.pslot = ctx->d_tl_pslot;
+      t.ppart = static_cast<const float2*>(ctx->d_piece_part);
+      t.items = static_cast<const TItem*>(ctx->d_titems);
+      t.n_items = ctx->n_titems;
+      a.supers = static_cast<const PrSuper*>(ctx->d_tsupers);
+      a.partial = static_cast<double2*>(ctx->d_tpartial);
+      a.n_super = ctx->n_tsuper;
+      a.super_blocks = static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_tsuper, kThreads)));
+      hipLaunchKernelGGL(k_tile, dim3(std::max(1, ctx->cu_count)), dim3(kFusedThreads), 0, s,
+                         static_cast<const TileChunk*>(ctx->d_tchunks), ctx->n_chunks, ctx->d_twin, ctx->d_tsrc,
+                         cin, a.n_gather, static_cast<float2*>(ctx->d_piece_part), a.ctrl);
+      hipLaunchKernelGGL(k_fused_tiled, dim3(fb), dim3(kFusedThreads), 0, s, a, t, cin, cout, ctx->d_score);
+      hipLaunchKernelGGL(k_kahan_super, dim3(a.super_blocks), dim3(kThreads), 0, s, a, cout, ctx->d_score);
     } else {
       const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
       float* cout = ctx->d_contrib[(it + 1) & 1];

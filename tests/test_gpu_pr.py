@@ -130,3 +130,29 @@ def test_pr_layouts_bit_identical(ctx, dmm, orc, graph):
         out[layout] = ctx.PageRankPull(20, 1e-4, MODES["serial"])[0]
     ctx.set_layout(dmm._lib.DMM_LAYOUT_DEGREE)
     assert np.array_equal(out[0], out[1])
+
+
+def test_pr_kahan_tiled_and_untiled_layouts(dmm, orc, graph):
+    """The compensated mode runs the window-tiled kernels on a degree-ordered
+    whole graph (scale 18: several tiled source windows) and the untiled
+    fused kernel on the natural layout; both stay within the FP64 oracle's
+    tolerance, are run-to-run deterministic, and agree with each other to
+    compensated-sum accuracy."""
+    import dist_mem_gapbs_amd._lib as L
+    g = graph(18, 16, False)
+    og = orc.CSR.of(g)
+    ref64, _, _ = orc.pr_f64(og, 20, 0.0)
+    out = {}
+    for layout in (L.DMM_LAYOUT_DEGREE, L.DMM_LAYOUT_NATURAL):
+        with dmm.Context(0) as c:
+            c.set_layout(layout)
+            c.upload(g)
+            s1, st = c.PageRankPull(20, 0.0, MODES["kahan"])
+            s2, _ = c.PageRankPull(20, 0.0, MODES["kahan"])
+            assert np.array_equal(s1, s2)
+            d = np.abs(s1.astype(np.float64) - ref64)
+            assert d.max() < 1e-6
+            assert (d / ref64).max() < 4e-7
+            out[layout] = s1
+    a, b = out[L.DMM_LAYOUT_DEGREE].astype(np.float64), out[L.DMM_LAYOUT_NATURAL].astype(np.float64)
+    assert (np.abs(a - b) / ref64).max() < 4e-7
