@@ -18,7 +18,9 @@ Also reported on the same line:
   kron24        Kronecker scale 24: plain DOBFS GTEPS, batched 64-source BFS
                 aggregate GTEPS and its ratio to plain, PageRankPull
                 edges/s/iter and HBM fraction (FP32 compensated, FP32
-                reference-order, FP64), and the FP32 error versus FP64.
+                reference-order, FP64), and the FP32 error versus FP64;
+                the same plain/batched pair on a uniform random graph of
+                scale 24.
 Synthetic graphs from the reference generator (no datasets).
 """
 import argparse
@@ -118,7 +120,27 @@ def cpu_baseline(og, sources, m_comp, seconds):
                       f"on the same Kronecker-22 graph, {dt:.1f} s"}
 
 
-def kron24_leg(dmm, args, tuned):
+def load_tuning():
+    p = os.path.join(ROOT, "dist_mem_gapbs_amd", "tuning.json")
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        return json.load(f)
+
+
+def apply_tuning(ctx, tuned):
+    """Installs the tuned launch shapes and batch settings (tuning.json);
+    alpha / beta stay at the workload's 15 / 18."""
+    if not tuned:
+        return
+    t = ctx.get_tuning()
+    t.grid_per_cu = tuned.get("grid_per_cu", t.grid_per_cu)
+    t.batch.td_div = tuned.get("td_div", t.batch.td_div)
+    t.batch.grid_per_cu = tuned.get("batch_grid_per_cu", t.batch.grid_per_cu)
+    ctx.set_tuning(t)
+
+
+def kron24_leg(dmm, args, tuned_all):
     import dist_mem_gapbs_amd._lib as L
     out = {}
     t0 = time.time()
@@ -128,32 +150,9 @@ def kron24_leg(dmm, args, tuned):
     deg = np.diff(g.out_offsets)
     with dmm.Context(0) as ctx:
         ctx.upload(g)
-        if tuned:
-            t = ctx.get_tuning()
-            t.alpha, t.beta = tuned.get("alpha", 15), tuned.get("beta", 18)
-            t.batch.td_div = tuned.get("td_div", t.batch.td_div)
-            t.grid_per_cu = tuned.get("grid_per_cu", t.grid_per_cu)
-            ctx.set_tuning(t)
-        # plain DOBFS on 16 sources (alpha/beta = 15/18)
-        mc = {}
-        for s in srcs[:16]:
-            _, _, st = ctx.DOBFS(int(s), read=False)
-            mc[int(s)] = st.reached_edges // 2
-        t0 = time.perf_counter()
-        for s in srcs[:16]:
-            ctx.DOBFS(int(s), read=False, with_stats=False)
-        dt = time.perf_counter() - t0
-        plain = sum(mc[int(s)] for s in srcs[:16]) / dt / 1e9
-        # batched 64 lanes
-        ctx.bfs_batch64(srcs, want_depth=False)
-        best = None
-        for _ in range(3):
-            t0 = time.perf_counter()
-            _, bst = ctx.bfs_batch64(srcs, want_depth=False)
-            wall = time.perf_counter() - t0
-            best = (wall, bst) if best is None or wall < best[0] else best
-        wall, bst = best
-        batched = bst.reached_edges / 2 / wall / 1e9
+        apply_tuning(ctx, tuned_all.get(f"kron-{args.big_scale}-16"))
+        # plain DOBFS on 16 sources (alpha/beta = 15/18), batched 64 lanes
+        plain, batched, wall, bst = plain_and_batched(ctx, srcs)
         out["plain_gteps"] = plain
         out["batched_gteps"] = batched
         out["batched_over_plain"] = batched / plain
@@ -182,7 +181,40 @@ def kron24_leg(dmm, args, tuned):
         out["pagerank"] = pr
     del deg
     g.free()
+    # Uniform random graph of the same scale (batched-BFS configuration).
+    g = dmm.build_graph(args.big_scale, 16, uniform=True)
+    srcs = dmm.pick_sources(g, 64)
+    with dmm.Context(0) as ctx:
+        ctx.upload(g)
+        apply_tuning(ctx, tuned_all.get(f"uniform-{args.big_scale}-16"))
+        plain, batched, _, _ = plain_and_batched(ctx, srcs)
+    out[f"uniform{args.big_scale}"] = {"plain_gteps": plain, "batched_gteps": batched,
+                                      "batched_over_plain": batched / plain}
+    g.free()
     return out
+
+
+def plain_and_batched(ctx, srcs):
+    """Plain DOBFS over 16 sources and batched 64-source BFS (best of 3),
+    aggregate GTEPS over the wall clock."""
+    mc = {}
+    for s in srcs[:16]:
+        _, _, st = ctx.DOBFS(int(s), read=False)
+        mc[int(s)] = st.reached_edges // 2
+    t0 = time.perf_counter()
+    for s in srcs[:16]:
+        ctx.DOBFS(int(s), read=False, with_stats=False)
+    dt = time.perf_counter() - t0
+    plain = sum(mc[int(s)] for s in srcs[:16]) / dt / 1e9
+    ctx.bfs_batch64(srcs, want_depth=False)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, bst = ctx.bfs_batch64(srcs, want_depth=False)
+        wall = time.perf_counter() - t0
+        best = (wall, bst) if best is None or wall < best[0] else best
+    wall, bst = best
+    return plain, bst.reached_edges / 2 / wall / 1e9, wall, bst
 
 
 def pe_share_leg(dmm, args):
@@ -276,6 +308,7 @@ def only_leg(dmm, args):
     out = {"leg": args.only, "scale": args.big_scale}
     with dmm.Context(0) as ctx:
         ctx.upload(g)
+        apply_tuning(ctx, load_tuning().get(f"kron-{args.big_scale}-16"))
         if args.only == "bfs":
             ms = []
             for s in srcs[:args.steps]:
@@ -340,13 +373,7 @@ def main():
         only_leg(dmm, args)
         return
 
-    tuned = None
-    tp = os.path.join(ROOT, "dist_mem_gapbs_amd", "tuning.json")
-    if os.path.exists(tp):
-        with open(tp) as f:
-            tuned_all = json.load(f)
-    else:
-        tuned_all = {}
+    tuned_all = load_tuning()
 
     t0 = time.time()
     g = dmm.build_graph(args.scale, 16, uniform=False)
@@ -356,7 +383,7 @@ def main():
     ned = g.num_edges_directed
     with dmm.Context(0) as ctx:
         ctx.upload(g)
-        tuned = tuned_all.get(f"kron-{args.scale}-16")
+        apply_tuning(ctx, tuned_all.get(f"kron-{args.scale}-16"))
         # m_comp per source (Graph500: reached out-edges / 2) and warm-up.
         m_comp = {}
         for s in srcs:
@@ -402,7 +429,7 @@ def main():
 
     extra = None
     if not args.no_big:
-        extra = kron24_leg(dmm, args, tuned_all.get(f"kron-{args.big_scale}-16"))
+        extra = kron24_leg(dmm, args, tuned_all)
 
     line = {
         "metric": "BFS GTEPS & PR edges/s/iter, batched/FP32 vs plain GPU, % HBM peak",
@@ -412,7 +439,8 @@ def main():
         "config": {"workload": f"plain DOBFS, Kronecker scale {args.scale} edge factor 16, "
                                f"64 sources, alpha 15 beta 18, one source per launch",
                    "num_nodes": n, "num_edges_directed": ned,
-                   "parallelism": "1 GPU = 1 PE"},
+                   "parallelism": "1 GPU = 1 PE",
+                   "tuning": tuned_all.get(f"kron-{args.scale}-16")},
         "device_gteps": dev_gteps,
         "roofline": roof,
         "cpu_baseline": cpu,

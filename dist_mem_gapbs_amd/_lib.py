@@ -61,7 +61,7 @@ class BfsStats(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [
         ("td_div", C.c_int32),
-        ("block", C.c_int32),
+        ("grid_per_cu", C.c_int32),
         ("graph_levels", C.c_int32),
         ("reserved", C.c_int32),
     ]

@@ -1,7 +1,7 @@
 // Autotuner: searches the direction-switch thresholds and launch shapes on
 // the ctx's own graph and sample sources, timing the device-resident paths.
 //   plain DOBFS : alpha x beta x workgroups-per-CU
-//   batched BFS : top-down divisor x workgroups-per-CU
+//   batched BFS : top-down divisor x its own workgroups-per-CU
 // The best values are installed on the ctx.  (The reference fixes alpha =
 // 15, beta = 18 in the DOBFS signature, src/bfs.cc:142.)
 #include <algorithm>
@@ -51,7 +51,7 @@ extern "C" int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, 
   best.plain_ms = 1e30;
   const int alphas[] = {5, 10, 15, 20, 30, 60};
   const int betas[] = {9, 18, 36, 72};
-  const int gpcs[] = {4, 8, 16};
+  const int gpcs[] = {2, 4, 8, 16};
   // Workgroups per CU first at the reference thresholds, then alpha x beta.
   for (int g : gpcs) {
     dmm_tuning t = base;
@@ -78,19 +78,23 @@ extern "C" int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, 
       }
     }
   }
+  // Batched BFS: top-down divisor x its own workgroups per CU.
   const int divs[] = {5, 10, 20, 40, 80, 160};
   best.batch_ms = 1e30;
-  for (int d : divs) {
-    dmm_tuning t = best;
-    t.batch.td_div = d;
-    if ((rc = dmm_set_tuning(ctx, &t))) return rc;
-    double ms;
-    if ((rc = time_batch(&ms))) return rc;
-    if (ms < best.batch_ms) {
-      const double pm = best.plain_ms;
-      best = t;
-      best.plain_ms = pm;
-      best.batch_ms = ms;
+  for (int g : gpcs) {
+    for (int d : divs) {
+      dmm_tuning t = best;
+      t.batch.td_div = d;
+      t.batch.grid_per_cu = g;
+      if ((rc = dmm_set_tuning(ctx, &t))) return rc;
+      double ms;
+      if ((rc = time_batch(&ms))) return rc;
+      if (ms < best.batch_ms) {
+        const double pm = best.plain_ms;
+        best = t;
+        best.plain_ms = pm;
+        best.batch_ms = ms;
+      }
     }
   }
   if ((rc = dmm_set_tuning(ctx, &best))) return rc;

@@ -38,6 +38,7 @@ namespace dmm {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
 constexpr int kItems = 4;
 constexpr int kEPB = kThreads * kItems;
 constexpr int kGroup = 8;  // lanes per vertex in the bottom-up sweep
@@ -388,33 +389,33 @@ __device__ void choose_next(const BArgs& a, BCtrl* c, long long cnt, long long d
     c->phase = kBBU;
 }
 
-__global__ __launch_bounds__(kThreads) void k_bpost(BArgs a, int nblocks) {
-  __shared__ int64_t red[kPartWidth][kThreads / 64];
-  __shared__ int64_t sc[kThreads / 64 + 1];
-  __shared__ int64_t sd[kThreads / 64 + 1];
+__global__ __launch_bounds__(kPostThreads) void k_bpost(BArgs a, int nblocks) {
+  __shared__ int64_t red[kPartWidth][kPostThreads / 64];
+  __shared__ int64_t sc[kPostThreads / 64 + 1];
+  __shared__ int64_t sd[kPostThreads / 64 + 1];
   BCtrl* c = a.ctrl;
   const int phase = c->phase;
   if (phase != kBTD && phase != kBBU && phase != kBCount && phase != kBWrite) return;
   int64_t s0 = 0, s1 = 0, s2 = 0;
   if (phase == kBBU || phase == kBCount) {
-    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+    for (int b = threadIdx.x; b < nblocks; b += kPostThreads) {
       s0 += a.part[b * kPartWidth + 0];
       s1 += a.part[b * kPartWidth + 1];
       s2 += a.part[b * kPartWidth + 2];
     }
-    s0 = block_sum<kThreads>(s0, red[0]);
-    s1 = block_sum<kThreads>(s1, red[1]);
-    s2 = block_sum<kThreads>(s2, red[2]);
+    s0 = block_sum<kPostThreads>(s0, red[0]);
+    s1 = block_sum<kPostThreads>(s1, red[1]);
+    s2 = block_sum<kPostThreads>(s2, red[2]);
   }
   if (phase == kBCount) {
     int64_t carry_c = 0, carry_d = 0;
-    for (int base = 0; base < nblocks; base += kThreads) {
+    for (int base = 0; base < nblocks; base += kPostThreads) {
       const int b = base + threadIdx.x;
       const int64_t vc = b < nblocks ? a.part[b * kPartWidth + 0] : 0;
       const int64_t vd = b < nblocks ? a.part[b * kPartWidth + 1] : 0;
       int64_t tc, td;
-      const int64_t ec = block_excl_scan<kThreads>(vc, sc, &tc);
-      const int64_t ed = block_excl_scan<kThreads>(vd, sd, &td);
+      const int64_t ec = block_excl_scan<kPostThreads>(vc, sc, &tc);
+      const int64_t ed = block_excl_scan<kPostThreads>(vd, sd, &td);
       if (b < nblocks) {
         a.part[b * kPartWidth + 0] = carry_c + ec;
         a.part[b * kPartWidth + 1] = carry_d + ed;
@@ -684,11 +685,11 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bsrc, ctx->h_bsrc, sizeof(int32_t) * 64, hipMemcpyHostToDevice,
                                     ctx->stream));
   for (int attempt = 0; attempt < 4; attempt++) {
-    const int key = p.td_div * 1000 + p.graph_levels * 10 + nsrc * 1000000 + ctx->tuning.grid_per_cu;
+    const int gpc = p.grid_per_cu > 0 ? p.grid_per_cu : ctx->tuning.grid_per_cu;
+    const int64_t key = ((int64_t(p.td_div) * 4096 + p.graph_levels) * 128 + nsrc) * 64 + gpc;
     BArgs a = make_args(ctx, nsrc, p.td_div);
     const int blocks = static_cast<int>(std::min<int64_t>(
-        int64_t(std::max(1, ctx->cu_count)) * std::max(1, std::min(kMaxGridPerCU, ctx->tuning.grid_per_cu)),
-        ctx->part_cap));
+        int64_t(std::max(1, ctx->cu_count)) * std::max(1, std::min(kMaxGridPerCU, gpc)), ctx->part_cap));
     if (!ctx->batch_graph || !ctx->batch_cont || ctx->batch_graph_key != key) {
       if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);
       if (ctx->batch_cont) (void)hipGraphExecDestroy(ctx->batch_cont);
@@ -701,7 +702,7 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
                          static_cast<const int32_t*>(ctx->d_bsrc), nsrc);
       for (int i = 0; i < p.graph_levels; i++) {
         hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
-        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kThreads), 0, ctx->stream, a, blocks);
+        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kPostThreads), 0, ctx->stream, a, blocks);
       }
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
@@ -711,7 +712,7 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
       DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       for (int i = 0; i < 6; i++) {
         hipLaunchKernelGGL(k_bstep, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
-        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kThreads), 0, ctx->stream, a, blocks);
+        hipLaunchKernelGGL(k_bpost, dim3(1), dim3(kPostThreads), 0, ctx->stream, a, blocks);
       }
       e = hipStreamEndCapture(ctx->stream, &g);
       if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));

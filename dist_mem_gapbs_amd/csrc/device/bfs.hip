@@ -46,6 +46,7 @@ namespace dmm {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
 constexpr int kItems = 4;                 // TD edges per thread per chunk
 constexpr int kEPB = kThreads * kItems;   // TD edges per chunk
 
@@ -381,34 +382,37 @@ __global__ __launch_bounds__(kThreads) void k_step(StepArgs a, const BfsParams* 
 
 // One workgroup after every step: reduces the partials, advances the loop
 // state, and (kForced < 0) schedules the next phase.
+// One workgroup of kPostThreads: the per-block partials are summed with
+// every load of a thread issued at once, and the COUNT prefix over the
+// blocks takes ceil(nblocks / kPostThreads) scan rounds.
 template <int kForced>
-__global__ __launch_bounds__(kThreads) void k_post(StepArgs a, const BfsParams* prm, int nblocks) {
-  __shared__ int64_t red[kPartWidth][kThreads / 64];
-  __shared__ int64_t sc[kThreads / 64 + 1];
-  __shared__ int64_t sd[kThreads / 64 + 1];
+__global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsParams* prm, int nblocks) {
+  __shared__ int64_t red[kPartWidth][kPostThreads / 64];
+  __shared__ int64_t sc[kPostThreads / 64 + 1];
+  __shared__ int64_t sd[kPostThreads / 64 + 1];
   BfsCtrl* c = a.ctrl;
   const int phase = kForced >= 0 ? kForced : c->phase;
   if (phase == kPhaseBU || phase == kPhaseCount) {
     int64_t s0 = 0, s1 = 0, s2 = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+    for (int b = threadIdx.x; b < nblocks; b += kPostThreads) {
       s0 += a.part[b * kPartWidth + 0];
       s1 += a.part[b * kPartWidth + 1];
       s2 += a.part[b * kPartWidth + 2];
     }
-    s0 = block_sum<kThreads>(s0, red[0]);
-    s1 = block_sum<kThreads>(s1, red[1]);
-    s2 = block_sum<kThreads>(s2, red[2]);
+    s0 = block_sum<kPostThreads>(s0, red[0]);
+    s1 = block_sum<kPostThreads>(s1, red[1]);
+    s2 = block_sum<kPostThreads>(s2, red[2]);
     if (phase == kPhaseCount) {
       // In-place exclusive scan of the count and degree partials, in
       // workgroup order, for the WRITE phase.
       int64_t carry_c = 0, carry_d = 0;
-      for (int base = 0; base < nblocks; base += kThreads) {
+      for (int base = 0; base < nblocks; base += kPostThreads) {
         const int b = base + threadIdx.x;
         const int64_t vc = b < nblocks ? a.part[b * kPartWidth + 0] : 0;
         const int64_t vd = b < nblocks ? a.part[b * kPartWidth + 1] : 0;
         int64_t tc, td;
-        const int64_t ec = block_excl_scan<kThreads>(vc, sc, &tc);
-        const int64_t ed = block_excl_scan<kThreads>(vd, sd, &td);
+        const int64_t ec = block_excl_scan<kPostThreads>(vc, sc, &tc);
+        const int64_t ed = block_excl_scan<kPostThreads>(vd, sd, &td);
         if (b < nblocks) {
           a.part[b * kPartWidth + 0] = carry_c + ec;
           a.part[b * kPartWidth + 1] = carry_d + ed;
@@ -705,7 +709,7 @@ void launch_pair(dmm_ctx* ctx, const StepArgs& a, const BfsParams* prm, int bloc
     (void)hipEventRecord(ctx->trace_kev_end, ctx->stream);
     ctx->trace_kev_used = true;
   }
-  hipLaunchKernelGGL(k_post<kPhase>, dim3(1), dim3(kThreads), 0, ctx->stream, a, prm, blocks);
+  hipLaunchKernelGGL(k_post<kPhase>, dim3(1), dim3(kPostThreads), 0, ctx->stream, a, prm, blocks);
 }
 
 }  // namespace

@@ -66,10 +66,9 @@ struct PrCtrl {
 
 struct Tuning {
   int alpha = 15, beta = 18;           // src/bfs.cc:142
-  int bu_block = 256;
   int td_edges_per_block = 1024;
   int grid_per_cu = 8;                 // workgroups per CU for grid-stride steps
-  dmm_batch_params batch = {20, 256, 16, 0};
+  dmm_batch_params batch = {20, 0, 16, 0};  // grid_per_cu 0: the one above
   double plain_ms = 0, batch_ms = 0;
 };
 
@@ -203,7 +202,7 @@ struct dmm_ctx {
   uint64_t* d_bseen = nullptr;      // nloc words
   uint64_t* d_bnext = nullptr;      // nloc words
   uint64_t* d_gnz = nullptr;        // non-zero bitmap of d_gfront + its summary
-  int batch_graph_key = -1;
+  int64_t batch_graph_key = -1;
 
   dmm::Tuning tuning;
 };

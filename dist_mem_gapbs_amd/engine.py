@@ -178,7 +178,7 @@ class Context:
         if td_div is not None or graph_levels is not None:
             t = self.get_tuning()
             params = L.BatchParams(td_div if td_div is not None else t.batch.td_div,
-                                   t.batch.block,
+                                   t.batch.grid_per_cu,
                                    graph_levels if graph_levels is not None else t.batch.graph_levels,
                                    0)
         st = L.BatchStats()

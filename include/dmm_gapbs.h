@@ -250,7 +250,8 @@ int dmm_state_set_level(dmm_ctx* ctx, int32_t level);
 
 typedef struct dmm_batch_params {
   int32_t td_div;        /* top-down while active edges < M / td_div       */
-  int32_t block;         /* threads per workgroup for the sweep kernels    */
+  int32_t grid_per_cu;   /* workgroups per CU of the batched kernels
+                            (<= 0: the ctx's grid_per_cu)                 */
   int32_t graph_levels;  /* levels per captured hipGraph chunk             */
   int32_t reserved;
 } dmm_batch_params;
