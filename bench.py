@@ -22,6 +22,14 @@ Also reported on the same line:
                 the same plain/batched pair on a uniform random graph of
                 scale 24.
 Synthetic graphs from the reference generator (no datasets).
+
+Other modes: --only {bfs,batch,pr} runs one Kronecker --big-scale leg (for
+profiling); --pe-share runs one PE's share of Kronecker --pe-scale (PE 3 of
+np = 8), timing the block kernels against whole-graph inputs and checking
+every block result against the whole-graph result; --calibrate-copy adds one
+256 MiB elementwise kernel as the reference for the FETCH_SIZE / WRITE_SIZE
+counters.  Launch shapes and the batched BFS settings come from
+dist_mem_gapbs_amd/tuning.json (python -m dist_mem_gapbs_amd.tune).
 """
 import argparse
 import contextlib

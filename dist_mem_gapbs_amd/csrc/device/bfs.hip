@@ -95,6 +95,9 @@ struct TdLds {
   int64_t base[kEPB + 2];
   int32_t u[kEPB + 2];
   int64_t i0, nv;
+  int64_t sc[kThreads / 64 + 1];
+  int64_t sd[kThreads / 64 + 1];
+  unsigned long long tail;
 };
 
 struct SmallLds {
@@ -113,8 +116,13 @@ union StepLds {
 // covers window edges [k*kEPB, (k+1)*kEPB); the frontier entries it spans
 // are staged in LDS and each edge finds its source by binary search.  A
 // claim (successful CAS of parent[v] from its negative value to u) sets v's
-// bit in the spare bitmap; the count step that follows turns the bits into
-// |F'| and scout_count (the sum of the claimed -parent values, src/bfs.cc:92).
+// bit in the spare bitmap.  The claimed vertices of out-degree >= 1 are also
+// appended to the queue after qin (block scan per chunk, one packed atomic
+// on ctrl->td_tail for the chunk's range), unordered, with their out-degree
+// prefix; the per-block partials carry |F'| and scout_count (the sum of the
+// claimed -parent values, src/bfs.cc:92).  The device loop slides to that
+// queue directly; the host-driven step recounts the bitmap and rewrites the
+// queue in ascending order (COUNT, WRITE) as BitmapToQueue would.
 // ------------------------------------------------------------------------
 __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
   const int64_t qb = c->qb;
@@ -131,6 +139,8 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
   for (int64_t w = gtid; w < a.words; w += gsize) clr[w] = 0;
 
   const int tid = threadIdx.x;
+  const long long qin = c->qin;
+  int64_t blk_cnt = 0, blk_degs = 0, my_zero = 0;  // blk_* valid in thread 0
   for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
     const int64_t e1 = min(T, e0 + kEPB);
     if (tid == 0) {
@@ -167,8 +177,13 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
       L.base[i] = a.out_off[u - a.vbegin] - qo;
     }
     __syncthreads();
+    int32_t cv[kItems];
+    int64_t cd[kItems];
+    int64_t ncl = 0, dsum = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
+      cv[k] = -1;
+      cd[k] = 0;
       const int64_t e = e0 + tid + k * kThreads;
       if (e < e1) {
         const int32_t r = static_cast<int32_t>(e - e0);
@@ -186,13 +201,54 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
           if (atomicCAS(&a.parent[lv], cur, u) == cur) {
             if (a.depth) a.depth[lv] = level + 1;
             atomicOr(reinterpret_cast<unsigned long long*>(&out_bm[v >> 6]), 1ull << (v & 63));
+            // parent = -1 stands for out-degree 0 and 1: read the degree.
+            const int64_t d = a.out_off[lv + 1] - a.out_off[lv];
+            if (d > 0) {
+              cv[k] = v;
+              cd[k] = d;
+              ncl++;
+              dsum += d;
+            } else {
+              my_zero++;
+            }
           }
         }
       }
     }
+    // The chunk's claims of out-degree >= 1 go to the queue after qin.
+    int64_t tc, td;
+    const int64_t ec = block_excl_scan<kThreads>(ncl, L.sc, &tc);
+    const int64_t ed = block_excl_scan<kThreads>(dsum, L.sd, &td);
+    if (tc > 0) {
+      if (tid == 0)
+        L.tail = atomicAdd(&c->td_tail, (static_cast<unsigned long long>(tc) << 34) |
+                                            static_cast<unsigned long long>(td));
+      __syncthreads();
+      const unsigned long long t = L.tail;
+      int64_t pos = qin + static_cast<int64_t>(t >> 34) + ec;
+      int64_t run = static_cast<int64_t>(t & ((1ull << 34) - 1)) + ed;
+#pragma unroll
+      for (int k = 0; k < kItems; k++) {
+        if (cv[k] >= 0) {
+          a.queue[pos] = cv[k];
+          a.qoff[pos] = run;
+          pos++;
+          run += cd[k];
+        }
+      }
+    }
+    blk_cnt += tc;
+    blk_degs += td;
     __syncthreads();
   }
-  if (threadIdx.x < kPartWidth) a.part[blockIdx.x * kPartWidth + threadIdx.x] = 0;
+  const int64_t z = block_sum<kThreads>(my_zero, L.sc);
+  if (threadIdx.x == 0) {
+    int64_t* p = a.part + blockIdx.x * kPartWidth;
+    p[0] = blk_cnt;
+    p[1] = blk_degs;
+    p[2] = z;
+    p[3] = 0;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -392,7 +448,9 @@ __global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsPara
   __shared__ int64_t sd[kPostThreads / 64 + 1];
   BfsCtrl* c = a.ctrl;
   const int phase = kForced >= 0 ? kForced : c->phase;
-  if (phase == kPhaseBU || phase == kPhaseCount) {
+  // The preceding step kernel has finished: its appended range is final.
+  if (threadIdx.x == 0) c->td_tail = 0;
+  if (phase == kPhaseBU || phase == kPhaseCount || (kForced < 0 && phase == kPhaseTD)) {
     int64_t s0 = 0, s1 = 0, s2 = 0;
     for (int b = threadIdx.x; b < nblocks; b += kPostThreads) {
       s0 += a.part[b * kPartWidth + 0];
@@ -422,6 +480,35 @@ __global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsPara
       }
     }
     if (threadIdx.x != 0) return;
+    if (phase == kPhaseTD) {
+      // Device loop: the step appended its claims after qin, so the count
+      // is in the partials and the queue needs no rewrite.
+      c->td_in = (c->qe - c->qb) + c->window_zero;
+      c->td_edges = c->window_edges;
+      c->level = c->level + 1;
+      const int fr = c->front;
+      c->front = roles(role_spare(fr), role_scratch(fr), role_front(fr));
+      c->count_src = 0;
+      c->pend_cnt = s0;
+      c->pend_degs = s1;
+      c->pend_zero = s2;
+      c->qb = c->qe = c->qin;
+      c->window_edges = 0;
+      c->window_zero = 0;
+      c->scout = s1 + s2;  // sum of -parent over the claims
+      trace_step(c, DMM_STEP_TD, c->td_in, s0 + s2, c->td_edges, s1 + s2);
+      decide(c, prm);
+      if (c->phase == kPhaseWrite) {
+        // Top-down again: slide onto the appended claims.
+        c->qb = c->qin;
+        c->qe = c->qin + s0;
+        c->qin = c->qe;
+        c->window_edges = s1;
+        c->window_zero = s2;
+        c->phase = kPhaseTD;
+      }
+      return;
+    }
     if (phase == kPhaseBU) {
       c->level = c->level + 1;
       c->old_awake = c->awake_count;
@@ -532,6 +619,7 @@ __global__ void k_loop_init(StepArgs a, const BfsParams* prm, int decide_now) {
   c->pend_cnt = local ? 1 : 0;
   c->pend_degs = deg;
   c->pend_zero = 0;
+  c->td_tail = 0;
   c->td_in = 0;
   c->td_edges = 0;
   c->level = 0;

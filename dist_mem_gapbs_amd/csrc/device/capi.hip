@@ -194,7 +194,7 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
   if (!ctx->d_err_part) {
     const int64_t parts =
         std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx), dmm::pr_fused_blocks(ctx)) +
-        dmm::pr_light_blocks(ctx) + std::max<int64_t>(dmm::pr_super_blocks(ctx), dmm::ceil_div(ctx->n_tsuper, 256) + 1);
+        std::max<int64_t>(dmm::pr_super_blocks(ctx), dmm::ceil_div(ctx->n_tsuper, 256) + 1);
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
     ctx->err_part_cap = parts;
   }

@@ -31,6 +31,10 @@ struct BfsCtrl {
   long long scout, edges_to_check, awake_count, old_awake;
   // Results of the last frontier count, waiting for the queue write.
   long long pend_cnt, pend_degs, pend_zero;
+  // Top-down claims appended straight to the queue after qin: packed
+  // (count << 34) | out-degree sum, one atomic per workgroup chunk; every
+  // post kernel clears it.
+  unsigned long long td_tail;
   // Inputs of the last top-down step, for its trace record.
   long long td_in, td_edges;
   int level;
