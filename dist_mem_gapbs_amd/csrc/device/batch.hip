@@ -594,6 +594,13 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
 // One bottom-up level over a PE block against the all-gathered frontier
 // words of every vertex (front, global IDs): the sweep of bu_phase with the
 // block's own seen words (lanes reached at the current level or before).
+// Rows with more than kBlockSplit in-edges are only cleared here; their
+// kBlockSplit-edge segments are swept by k_block_segs, so one hub row does
+// not hold a group for its whole length when no lane exits early.
+constexpr int64_t kBlockSplit = 512;
+struct BlockSeg {
+  int64_t row, b, e;
+};
 __global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restrict__ in_off,
                                                          const int32_t* __restrict__ in_nbr, int64_t nloc,
                                                          const uint64_t* __restrict__ front,
@@ -610,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restr
     const uint64_t need = ~seen[u];
     const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
-    if (need && e > b) {
+    if (need && e > b && e - b <= kBlockSplit) {
       const int64_t stop = sweep_row(in_nbr, b, e, sub, need, front, fnz, fnz2, &nxt);
       scanned += stop - b;
     }
@@ -634,6 +641,35 @@ __global__ __launch_bounds__(kThreads) void k_nz_build(const uint64_t* __restric
     const uint64_t m = __ballot(v < n && w[v] != 0);
     if (lane == 0) nz[c] = m;
   }
+}
+
+// Segments of the long rows: the found lanes are ORed into next[row]
+// (cleared by k_block_sweep); partials at part[blockIdx.x].
+__global__ __launch_bounds__(kThreads) void k_block_segs(const BlockSeg* __restrict__ segs, int64_t nsegs,
+                                                        const int32_t* __restrict__ in_nbr,
+                                                        const uint64_t* __restrict__ front,
+                                                        const uint64_t* __restrict__ fnz,
+                                                        const uint64_t* __restrict__ fnz2,
+                                                        const uint64_t* __restrict__ seen,
+                                                        uint64_t* __restrict__ next, int64_t* part) {
+  __shared__ int64_t red[kThreads / 64];
+  const int sub = threadIdx.x & (kGroup - 1);
+  const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
+  const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
+  int64_t scanned = 0;
+  for (int64_t i = grp; i < nsegs; i += ngrp) {
+    const BlockSeg sg = segs[i];
+    const uint64_t need = ~seen[sg.row];
+    if (!need) continue;
+    uint64_t nxt = 0;
+    const int64_t stop = sweep_row(in_nbr, sg.b, sg.e, sub, need, front, fnz, fnz2, &nxt);
+    scanned += stop - sg.b;
+    if (sub == 0 && nxt)
+      atomicOr(reinterpret_cast<unsigned long long*>(&next[sg.row]), static_cast<unsigned long long>(nxt));
+  }
+  if (sub != 0) scanned = 0;
+  const int64_t s = block_sum<kThreads>(scanned, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(kThreads) void k_block_sum(const int64_t* part, int n, int64_t* out) {
@@ -836,7 +872,23 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bseen, seen, sizeof(uint64_t) * ctx->nloc, hipMemcpyHostToDevice, ctx->stream));
   const int blocks = static_cast<int>(std::min<int64_t>(
       int64_t(std::max(1, ctx->cu_count)) * std::max(1, std::min(kMaxGridPerCU, ctx->tuning.grid_per_cu)),
-      ctx->part_cap));
+      ctx->part_cap / 2));
+  // Long rows' segments, built once per upload from the block's offsets.
+  if (!ctx->bsegs_ready) {
+    std::vector<int64_t> off(ctx->nloc + 1);
+    DMM_HIP_CHECK(ctx, hipMemcpy(off.data(), ctx->d_in_off, sizeof(int64_t) * (ctx->nloc + 1), hipMemcpyDeviceToHost));
+    std::vector<BlockSeg> segs;
+    for (int64_t r = 0; r < ctx->nloc; r++)
+      if (off[r + 1] - off[r] > kBlockSplit)
+        for (int64_t x = off[r]; x < off[r + 1]; x += kBlockSplit)
+          segs.push_back({r, x, std::min(off[r + 1], x + kBlockSplit)});
+    if ((rc = dev_alloc(ctx, &ctx->d_bsegs, sizeof(BlockSeg) * std::max<size_t>(1, segs.size()), "block row segments")))
+      return rc;
+    if (!segs.empty())
+      DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_bsegs, segs.data(), sizeof(BlockSeg) * segs.size(), hipMemcpyHostToDevice));
+    ctx->n_bsegs = static_cast<int64_t>(segs.size());
+    ctx->bsegs_ready = true;
+  }
   // The timed region starts after the host copies have landed.
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
@@ -846,8 +898,12 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
   hipLaunchKernelGGL(k_block_sweep, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_in_off, ctx->d_in_nbr,
                      ctx->nloc, ctx->d_gfront, ctx->d_gnz, ctx->d_gnz + nzw, ctx->d_bseen, ctx->d_bnext,
                      ctx->d_part);
+  hipLaunchKernelGGL(k_block_segs, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+                     static_cast<const BlockSeg*>(ctx->d_bsegs), ctx->n_bsegs, ctx->d_in_nbr, ctx->d_gfront,
+                     ctx->d_gnz, ctx->d_gnz + nzw, ctx->d_bseen, ctx->d_bnext, ctx->d_part + blocks);
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
-  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_part, blocks, ctx->d_scratch64);
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_part, 2 * blocks,
+                     ctx->d_scratch64);
   DMM_HIP_CHECK(ctx, hipGetLastError());
   int64_t scanned = 0;
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(next, ctx->d_bnext, sizeof(uint64_t) * ctx->nloc, hipMemcpyDeviceToHost, ctx->stream));

@@ -85,7 +85,7 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
                   ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
-                  ctx->d_piece_part};
+                  ctx->d_piece_part, ctx->d_bsegs};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -112,6 +112,9 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->batch_valid = false;
   ctx->batch_last_level = -1;
   ctx->tiles_ready = false;
+  ctx->d_bsegs = nullptr;
+  ctx->n_bsegs = 0;
+  ctx->bsegs_ready = false;
   ctx->d_tl_off = ctx->d_tl_poff = ctx->d_twin = nullptr;
   ctx->d_tl_nbr = ctx->d_tl_pslot = nullptr;
   ctx->d_titems = ctx->d_tsupers = ctx->d_tpartial = ctx->d_tchunks = ctx->d_piece_part = nullptr;

@@ -206,6 +206,9 @@ struct dmm_ctx {
   uint64_t* d_bseen = nullptr;      // nloc words
   uint64_t* d_bnext = nullptr;      // nloc words
   uint64_t* d_gnz = nullptr;        // non-zero bitmap of d_gfront + its summary
+  void* d_bsegs = nullptr;          // segments of the block's long rows
+  int64_t n_bsegs = 0;
+  bool bsegs_ready = false;
   int64_t batch_graph_key = -1;
 
   dmm::Tuning tuning;
