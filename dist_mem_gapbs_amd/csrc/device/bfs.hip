@@ -449,7 +449,10 @@ __global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsPara
   BfsCtrl* c = a.ctrl;
   const int phase = kForced >= 0 ? kForced : c->phase;
   // The preceding step kernel has finished: its appended range is final.
-  if (threadIdx.x == 0) c->td_tail = 0;
+  if (threadIdx.x == 0) {
+    c->td_tail = 0;
+    if (kForced < 0 && phase >= kPhaseTD && phase <= kPhaseWrite) c->pairs = c->pairs + 1;
+  }
   if (phase == kPhaseBU || phase == kPhaseCount || (kForced < 0 && phase == kPhaseTD)) {
     int64_t s0 = 0, s1 = 0, s2 = 0;
     for (int b = threadIdx.x; b < nblocks; b += kPostThreads) {
@@ -620,6 +623,7 @@ __global__ void k_loop_init(StepArgs a, const BfsParams* prm, int decide_now) {
   c->pend_degs = deg;
   c->pend_zero = 0;
   c->td_tail = 0;
+  c->pairs = 0;
   c->td_in = 0;
   c->td_edges = 0;
   c->level = 0;
@@ -951,7 +955,7 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
                     dmm_bfs_stats* stats) {
   constexpr int kCont = 6;
   const int d = with_depth ? 1 : 0;
-  const int want = std::max(8, ctx->bfs_chain_len);
+  const int want = std::max(4, ctx->bfs_chain_len);
   if (!ctx->bfs_graph[d] || ctx->bfs_graph_steps[d] != want) {
     if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
     ctx->bfs_graph[d] = nullptr;
@@ -980,9 +984,13 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
   }
   ctx->bfs_valid = true;
   ctx->depth_valid = with_depth;
-  // Grow the first chain toward the phase count seen, so a typical BFS
-  // finishes in one graph launch.
-  if (rounds > 0) ctx->bfs_chain_len = std::min(128, want + rounds * kCont);
+  // Size the first chain to the most pairs any of the last searches needed,
+  // so a typical search finishes in one graph launch without trailing
+  // no-op pairs.
+  ctx->bfs_recent[ctx->bfs_recent_pos++ % kRecentSearches] = ctx->h_ctrl->pairs;
+  int most = 0;
+  for (int i = 0; i < kRecentSearches; i++) most = std::max(most, ctx->bfs_recent[i]);
+  ctx->bfs_chain_len = std::min(128, std::max(4, most));
   if (stats) {
     float ms = 0;
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));

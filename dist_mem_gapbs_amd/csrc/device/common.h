@@ -16,11 +16,12 @@ namespace dmm {
 // --------------------------------------------------------------------------
 // Control block shared by the BFS kernels (one per ctx, in HBM).
 // --------------------------------------------------------------------------
-constexpr int kSlots = 32;          // counters spread over slots (contention)
-constexpr int kCountBits = 30;      // packed reservation: count | degsum << 30
-constexpr uint64_t kCountMask = (uint64_t(1) << kCountBits) - 1;
+// Block rows stay below 2^kCountBits and block edges below 2^34, so a
+// queue reservation packs (count << 34) | out-degree sum in 64 bits.
+constexpr int kCountBits = 30;
 constexpr int kMaxGridPerCU = 32;   // upper bound of workgroups per CU
 constexpr int kPartWidth = 4;       // int64 partials per workgroup
+constexpr int kRecentSearches = 32; // searches sizing the device-loop chain
 
 struct BfsCtrl {
   // Sliding-queue window [qb, qe), tail qin; the window's out-degree sum and
@@ -41,6 +42,7 @@ struct BfsCtrl {
   int phase;       // kPhase* below
   int done;        // 1: loop finished, 2: cleanup done
   int step;        // trace records written
+  int pairs;       // (step, post) pairs that did work in the device loop
   int front;       // bitmap roles: front | spare << 2 | scratch << 4
   int count_src;   // 0: count after a TD step, 1: after the BU loop
   int error;       // != 0: a device-side invariant was violated
@@ -137,6 +139,8 @@ struct dmm_ctx {
   hipGraphExec_t bfs_cont[2] = {nullptr, nullptr};   // more steps + finish
   int bfs_graph_steps[2] = {0, 0};
   int bfs_chain_len = 16;
+  int bfs_recent[dmm::kRecentSearches] = {};  // pairs used by recent searches
+  int bfs_recent_pos = 0;
 
   // PageRank state.
   float* d_score = nullptr;
