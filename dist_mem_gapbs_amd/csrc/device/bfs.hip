@@ -47,6 +47,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
+constexpr int kBuProbe = 4;         // in-neighbours probed per round (bottom-up)
 constexpr int kItems = 4;                 // TD edges per thread per chunk
 constexpr int kEPB = kThreads * kItems;   // TD edges per chunk
 
@@ -276,17 +277,31 @@ __device__ void bu_phase(const StepArgs& a, BfsCtrl* c, SmallLds& L, int level) 
       if (parent[lu] < 0) {
         scanned++;
         const int64_t s = in_off[lu], e = in_off[lu + 1];
-        int64_t k = s;
-        for (; k < e; k++) {
-          const int32_t v = in_nbr[k];
-          if ((front[v >> 6] >> (v & 63)) & 1ull) {
-            parent[lu] = v;
-            found = true;
-            break;
-          }
+        // kBuProbe in-neighbours per round: their IDs, then their frontier
+        // bits, all loads in flight at once; the first hit in row order is
+        // the parent (the sequential scan's choice and edge count).
+        int32_t pv = 0;
+        int64_t hit = e;
+        for (int64_t k = s; k < e && hit == e; k += kBuProbe) {
+          int32_t v[kBuProbe];
+#pragma unroll
+          for (int j = 0; j < kBuProbe; j++) v[j] = k + j < e ? in_nbr[k + j] : -1;
+          bool h[kBuProbe];
+#pragma unroll
+          for (int j = 0; j < kBuProbe; j++) h[j] = v[j] >= 0 && ((front[v[j] >> 6] >> (v[j] & 63)) & 1ull);
+#pragma unroll
+          for (int j = kBuProbe - 1; j >= 0; j--)
+            if (h[j]) {
+              hit = k + j;
+              pv = v[j];
+            }
         }
-        edges += found ? k - s + 1 : e - s;
-        if (found && a.depth) a.depth[lu] = level + 1;
+        found = hit < e;
+        if (found) {
+          parent[lu] = pv;
+          if (a.depth) a.depth[lu] = level + 1;
+        }
+        edges += found ? hit - s + 1 : e - s;
       }
     }
     const unsigned long long m = __ballot(found);
