@@ -77,7 +77,8 @@ struct BfsParams {
   int beta;
   int pad;
   long long num_edges_directed;
-  long long pad2;
+  long long init_end;  // InitParent writes parents of rows [0, init_end)
+  long long scan_end;  // rows the bottom-up step scans
 };
 
 __device__ __forceinline__ uint64_t* bm_sel(const StepArgs& a, int i) {
@@ -265,7 +266,8 @@ __device__ void bu_phase(const StepArgs& a, BfsCtrl* c, SmallLds& L, int level) 
   const int lane = lane_id();
   const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
-  const int64_t nchunks = (a.nloc + 63) >> 6;
+  // Rows from scan_end on have no in-edges: never found, not scanned.
+  const int64_t nchunks = (c->scan_end + 63) >> 6;
   const int64_t* __restrict__ in_off = a.in_off;
   const int32_t* __restrict__ in_nbr = a.in_nbr;
   int32_t* __restrict__ parent = a.parent;
@@ -322,10 +324,14 @@ __device__ void bu_phase(const StepArgs& a, BfsCtrl* c, SmallLds& L, int level) 
     L.red[2][w] = scanned;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
-    int64_t s = 0;
-    for (int i = 0; i < kThreads / 64; i++) s += L.red[threadIdx.x][i];
-    a.part[blockIdx.x * kPartWidth + threadIdx.x] = s;
+  if (threadIdx.x == 0) {
+    int64_t s[3] = {0, 0, 0};
+    for (int i = 0; i < kThreads / 64; i++)
+      for (int k = 0; k < 3; k++) s[k] += L.red[k][i];
+    int64_t* p = a.part + blockIdx.x * kPartWidth;
+    p[0] = s[0];
+    p[1] = s[1];
+    p[2] = s[2];
   }
 }
 
@@ -401,9 +407,11 @@ __device__ void count_or_write(const StepArgs& a, BfsCtrl* c, SmallLds& L) {
   }
 }
 
-__device__ void trace_step(BfsCtrl* c, int kind, long long in, long long out, long long edges,
+// Appends one step record: the counters go to the head `h` (the post
+// kernel's LDS copy, or the control block itself), the record to c's trace.
+__device__ void trace_step(BfsHead& h, BfsCtrl* c, int kind, long long in, long long out, long long edges,
                            long long scout) {
-  const int s = c->step;
+  const int s = h.step;
   if (s < DMM_MAX_STEPS) {
     c->step_kind[s] = kind;
     c->step_in[s] = in;
@@ -411,27 +419,27 @@ __device__ void trace_step(BfsCtrl* c, int kind, long long in, long long out, lo
     c->step_edges[s] = edges;
     c->step_scout[s] = scout;
   }
-  c->step = s + 1;
+  h.step = s + 1;
 }
 
 // Loop head of src/bfs.cc:160-161 / 178-180 once the next frontier is
 // counted: stop, enter the BU branch (QueueToBitmap is implicit -- the front
 // bitmap already holds the frontier; the "e" step is recorded), or continue
 // top-down after writing the counted frontier as the queue window.
-__device__ void decide(BfsCtrl* c, const BfsParams* prm) {
-  const long long F = c->pend_cnt + c->pend_zero;
+__device__ void decide(BfsHead& h, BfsCtrl* c, const BfsParams* prm) {
+  const long long F = h.pend_cnt + h.pend_zero;
   if (F == 0) {
-    c->phase = kPhaseDone;
-    c->done = 1;
+    h.phase = kPhaseDone;
+    h.done = 1;
     return;
   }
-  if (c->scout > c->edges_to_check / prm->alpha) {
-    trace_step(c, DMM_STEP_QB, F, F, 0, 0);
-    c->awake_count = F;  // src/bfs.cc:165
-    c->phase = kPhaseBU;
+  if (h.scout > h.edges_to_check / prm->alpha) {
+    trace_step(h, c, DMM_STEP_QB, F, F, 0, 0);
+    h.awake_count = F;  // src/bfs.cc:165
+    h.phase = kPhaseBU;
   } else {
-    c->edges_to_check -= c->scout;  // src/bfs.cc:180
-    c->phase = kPhaseWrite;
+    h.edges_to_check -= h.scout;  // src/bfs.cc:180
+    h.phase = kPhaseWrite;
   }
 }
 
@@ -451,168 +459,190 @@ __global__ __launch_bounds__(kThreads) void k_step(StepArgs a, const BfsParams* 
   (void)prm;
 }
 
-// One workgroup after every step: reduces the partials, advances the loop
-// state, and (kForced < 0) schedules the next phase.
-// One workgroup of kPostThreads: the per-block partials are summed with
-// every load of a thread issued at once, and the COUNT prefix over the
-// blocks takes ceil(nblocks / kPostThreads) scan rounds.
+// The loop-state update after a step (one thread), on the head copy h with
+// the step's totals s0..s2 (meaning per phase: see the step functions).
 template <int kForced>
-__global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsParams* prm, int nblocks) {
-  __shared__ int64_t red[kPartWidth][kPostThreads / 64];
-  __shared__ int64_t sc[kPostThreads / 64 + 1];
-  __shared__ int64_t sd[kPostThreads / 64 + 1];
-  BfsCtrl* c = a.ctrl;
-  const int phase = kForced >= 0 ? kForced : c->phase;
-  // The preceding step kernel has finished: its appended range is final.
-  if (threadIdx.x == 0) {
-    c->td_tail = 0;
-    if (kForced < 0 && phase >= kPhaseTD && phase <= kPhaseWrite) c->pairs = c->pairs + 1;
-  }
-  if (phase == kPhaseBU || phase == kPhaseCount || (kForced < 0 && phase == kPhaseTD)) {
-    int64_t s0 = 0, s1 = 0, s2 = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kPostThreads) {
-      s0 += a.part[b * kPartWidth + 0];
-      s1 += a.part[b * kPartWidth + 1];
-      s2 += a.part[b * kPartWidth + 2];
+__device__ void post_update(BfsHead& h, BfsCtrl* c, const StepArgs& a, const BfsParams* prm, int phase,
+                            int64_t s0, int64_t s1, int64_t s2) {
+  // The preceding step has finished: its appended range is final.
+  h.td_tail = 0;
+  if (kForced < 0 && phase >= kPhaseTD && phase <= kPhaseWrite) h.pairs = h.pairs + 1;
+  if (phase == kPhaseTD && kForced < 0) {
+    // Device loop: the step appended its claims after qin, so the count is
+    // in the partials and the queue needs no rewrite.
+    h.td_in = (h.qe - h.qb) + h.window_zero;
+    h.td_edges = h.window_edges;
+    h.level = h.level + 1;
+    const int fr = h.front;
+    h.front = roles(role_spare(fr), role_scratch(fr), role_front(fr));
+    h.count_src = 0;
+    h.pend_cnt = s0;
+    h.pend_degs = s1;
+    h.pend_zero = s2;
+    h.qb = h.qe = h.qin;
+    h.window_edges = 0;
+    h.window_zero = 0;
+    h.scout = s1 + s2;  // sum of -parent over the claims
+    trace_step(h, c, DMM_STEP_TD, h.td_in, s0 + s2, h.td_edges, s1 + s2);
+    decide(h, c, prm);
+    if (h.phase == kPhaseWrite) {
+      // Top-down again: slide onto the appended claims.
+      h.qb = h.qin;
+      h.qe = h.qin + s0;
+      h.qin = h.qe;
+      h.window_edges = s1;
+      h.window_zero = s2;
+      h.phase = kPhaseTD;
     }
-    s0 = block_sum<kPostThreads>(s0, red[0]);
-    s1 = block_sum<kPostThreads>(s1, red[1]);
-    s2 = block_sum<kPostThreads>(s2, red[2]);
-    if (phase == kPhaseCount) {
-      // In-place exclusive scan of the count and degree partials, in
-      // workgroup order, for the WRITE phase.
-      int64_t carry_c = 0, carry_d = 0;
-      for (int base = 0; base < nblocks; base += kPostThreads) {
-        const int b = base + threadIdx.x;
-        const int64_t vc = b < nblocks ? a.part[b * kPartWidth + 0] : 0;
-        const int64_t vd = b < nblocks ? a.part[b * kPartWidth + 1] : 0;
-        int64_t tc, td;
-        const int64_t ec = block_excl_scan<kPostThreads>(vc, sc, &tc);
-        const int64_t ed = block_excl_scan<kPostThreads>(vd, sd, &td);
-        if (b < nblocks) {
-          a.part[b * kPartWidth + 0] = carry_c + ec;
-          a.part[b * kPartWidth + 1] = carry_d + ed;
-        }
-        carry_c += tc;
-        carry_d += td;
-      }
-    }
-    if (threadIdx.x != 0) return;
-    if (phase == kPhaseTD) {
-      // Device loop: the step appended its claims after qin, so the count
-      // is in the partials and the queue needs no rewrite.
-      c->td_in = (c->qe - c->qb) + c->window_zero;
-      c->td_edges = c->window_edges;
-      c->level = c->level + 1;
-      const int fr = c->front;
-      c->front = roles(role_spare(fr), role_scratch(fr), role_front(fr));
-      c->count_src = 0;
-      c->pend_cnt = s0;
-      c->pend_degs = s1;
-      c->pend_zero = s2;
-      c->qb = c->qe = c->qin;
-      c->window_edges = 0;
-      c->window_zero = 0;
-      c->scout = s1 + s2;  // sum of -parent over the claims
-      trace_step(c, DMM_STEP_TD, c->td_in, s0 + s2, c->td_edges, s1 + s2);
-      decide(c, prm);
-      if (c->phase == kPhaseWrite) {
-        // Top-down again: slide onto the appended claims.
-        c->qb = c->qin;
-        c->qe = c->qin + s0;
-        c->qin = c->qe;
-        c->window_edges = s1;
-        c->window_zero = s2;
-        c->phase = kPhaseTD;
-      }
-      return;
-    }
-    if (phase == kPhaseBU) {
-      c->level = c->level + 1;
-      c->old_awake = c->awake_count;
-      c->awake_count = s0;
-      const int fr = c->front;
-      c->front = roles(role_scratch(fr), role_spare(fr), role_front(fr));  // front.swap(curr)
-      trace_step(c, DMM_STEP_BU, s2, s0, s1, 0);
-      if (kForced >= 0) return;
-      // src/bfs.cc:174
-      if ((c->awake_count >= c->old_awake) || (c->awake_count > a.beta_width / prm->beta)) {
-        c->phase = kPhaseBU;
-      } else {
-        c->phase = kPhaseCount;
-        c->count_src = 1;
-      }
-      return;
-    }
-    // COUNT
-    c->pend_cnt = s0;
-    c->pend_degs = s1;
-    c->pend_zero = s2;
-    // The previous window is consumed (slide_window, src/bfs.cc:166/182).
-    c->qb = c->qe = c->qin;
-    c->window_edges = 0;
-    c->window_zero = 0;
-    if (c->count_src == 0) {
-      c->scout = s1 + s2;  // sum of -parent over the claims
-      trace_step(c, DMM_STEP_TD, c->td_in, s0 + s2, c->td_edges, s1 + s2);
-    } else {
-      c->scout = 1;  // src/bfs.cc:177
-      trace_step(c, DMM_STEP_BQ, s0 + s2, s0 + s2, 0, 0);
-    }
-    if (kForced >= 0) return;
-    decide(c, prm);
     return;
   }
-  if (threadIdx.x != 0) return;
   if (phase == kPhaseTD) {
-    c->td_in = (c->qe - c->qb) + c->window_zero;
-    c->td_edges = c->window_edges;
-    c->level = c->level + 1;
-    const int fr = c->front;
+    h.td_in = (h.qe - h.qb) + h.window_zero;
+    h.td_edges = h.window_edges;
+    h.level = h.level + 1;
+    const int fr = h.front;
     // The spare (now holding the claims) is the front; the cleared scratch
     // is the spare.
-    c->front = roles(role_spare(fr), role_scratch(fr), role_front(fr));
-    c->count_src = 0;
-    if (kForced < 0) c->phase = kPhaseCount;
-  } else if (phase == kPhaseWrite) {
-    c->qb = c->qin;
-    c->qe = c->qin + c->pend_cnt;
-    c->qin = c->qe;
-    c->window_edges = c->pend_degs;
-    c->window_zero = c->pend_zero;
-    if (kForced < 0) c->phase = kPhaseTD;
+    h.front = roles(role_spare(fr), role_scratch(fr), role_front(fr));
+    h.count_src = 0;
+    return;
+  }
+  if (phase == kPhaseBU) {
+    h.level = h.level + 1;
+    h.old_awake = h.awake_count;
+    h.awake_count = s0;
+    const int fr = h.front;
+    h.front = roles(role_scratch(fr), role_spare(fr), role_front(fr));  // front.swap(curr)
+    trace_step(h, c, DMM_STEP_BU, s2, s0, s1, 0);
+    if (kForced >= 0) return;
+    // src/bfs.cc:174
+    if ((h.awake_count >= h.old_awake) || (h.awake_count > a.beta_width / prm->beta)) {
+      h.phase = kPhaseBU;
+    } else {
+      h.phase = kPhaseCount;
+      h.count_src = 1;
+    }
+    return;
+  }
+  if (phase == kPhaseCount) {
+    h.pend_cnt = s0;
+    h.pend_degs = s1;
+    h.pend_zero = s2;
+    // The previous window is consumed (slide_window, src/bfs.cc:166/182).
+    h.qb = h.qe = h.qin;
+    h.window_edges = 0;
+    h.window_zero = 0;
+    if (h.count_src == 0) {
+      h.scout = s1 + s2;  // sum of -parent over the claims
+      trace_step(h, c, DMM_STEP_TD, h.td_in, s0 + s2, h.td_edges, s1 + s2);
+    } else {
+      h.scout = 1;  // src/bfs.cc:177
+      trace_step(h, c, DMM_STEP_BQ, s0 + s2, s0 + s2, 0, 0);
+    }
+    if (kForced >= 0) return;
+    decide(h, c, prm);
+    return;
+  }
+  if (phase == kPhaseWrite) {
+    h.qb = h.qin;
+    h.qe = h.qin + h.pend_cnt;
+    h.qin = h.qe;
+    h.window_edges = h.pend_degs;
+    h.window_zero = h.pend_zero;
+    if (kForced < 0) h.phase = kPhaseTD;
   }
 }
 
-// InitParent (src/bfs.cc:131-140) + clearing of the three bitmaps, with the
-// source's bit set in bitmap 0 (the window [source] as a front).
-__global__ __launch_bounds__(kThreads) void k_init(StepArgs a, const BfsParams* prm) {
-  const int32_t source = prm->source;
-  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t v = gtid; v < a.nloc; v += gsize) {
-    const int64_t d = a.out_off[v + 1] - a.out_off[v];
-    int32_t p = d != 0 ? static_cast<int32_t>(-d) : -1;
-    int32_t dep = -1;
-    if (a.vbegin + v == source) {
-      p = source;
-      dep = 0;
+// LDS of the post logic for a workgroup of kT threads.
+template <int kT>
+struct PostLds {
+  int64_t red[3][kT / 64];
+  int64_t sc[kT / 64 + 1];
+  int64_t sd[kT / 64 + 1];
+  union {
+    BfsHead h;
+    long long w[kHeadWords];
+  } sh;
+};
+
+// The post logic, run by one workgroup of kT threads once every workgroup
+// of the step has finished: reduces the per-block partials, advances the
+// loop state, and (kForced < 0) schedules the next phase.  The control head
+// and the partials are loaded at once, up front; the head is updated in LDS
+// and written back whole.  After a COUNT step the count and degree partials
+// are replaced by their exclusive prefixes in workgroup order, for the
+// WRITE step.
+template <int kT, int kForced>
+__device__ void post_body(const StepArgs& a, const BfsParams* prm, int nblocks, PostLds<kT>& L) {
+  constexpr int kWaves = kT / 64;
+  BfsCtrl* c = a.ctrl;
+  const int tid = threadIdx.x;
+  if (tid < kHeadWords) L.sh.w[tid] = reinterpret_cast<const long long*>(c)[tid];
+  int64_t s0 = 0, s1 = 0, s2 = 0;
+  for (int b = tid; b < nblocks; b += kT) {
+    s0 += a.part[b * kPartWidth + 0];
+    s1 += a.part[b * kPartWidth + 1];
+    s2 += a.part[b * kPartWidth + 2];
+  }
+  const int64_t own0 = s0, own1 = s1;  // block tid's values when nblocks <= kT
+  {
+    const int64_t w0 = wave_sum(s0), w1 = wave_sum(s1), w2 = wave_sum(s2);
+    if (lane_id() == 0) {
+      L.red[0][tid >> 6] = w0;
+      L.red[1][tid >> 6] = w1;
+      L.red[2][tid >> 6] = w2;
     }
-    a.parent[v] = p;
-    if (a.depth) a.depth[v] = dep;
   }
-  for (int64_t w = gtid; w < a.words; w += gsize) {
-    uint64_t x = 0;
-    if (w == (source >> 6)) x = 1ull << (source & 63);
-    a.bm0[w] = x;
-    a.bm1[w] = 0;
-    a.bm2[w] = 0;
+  __syncthreads();
+  const int phase = kForced >= 0 ? kForced : L.sh.h.phase;
+  if (phase < kPhaseTD || phase > kPhaseWrite) {
+    // Idle or done: only the appended-range counter is reset.
+    if (tid == 0) c->td_tail = 0;
+    return;
   }
+  if (phase == kPhaseCount) {
+    int64_t carry_c = 0, carry_d = 0;
+    for (int base = 0; base < nblocks; base += kT) {
+      const int b = base + tid;
+      int64_t vc = 0, vd = 0;
+      if (b < nblocks) {
+        vc = nblocks <= kT ? own0 : a.part[b * kPartWidth + 0];
+        vd = nblocks <= kT ? own1 : a.part[b * kPartWidth + 1];
+      }
+      int64_t tc, td;
+      const int64_t ec = block_excl_scan<kT>(vc, L.sc, &tc);
+      const int64_t ed = block_excl_scan<kT>(vd, L.sd, &td);
+      if (b < nblocks) {
+        a.part[b * kPartWidth + 0] = carry_c + ec;
+        a.part[b * kPartWidth + 1] = carry_d + ed;
+      }
+      carry_c += tc;
+      carry_d += td;
+    }
+  }
+  if (tid == 0) {
+    int64_t t0 = 0, t1 = 0, t2 = 0;
+    for (int w = 0; w < kWaves; w++) {
+      t0 += L.red[0][w];
+      t1 += L.red[1][w];
+      t2 += L.red[2][w];
+    }
+    post_update<kForced>(L.sh.h, c, a, prm, phase, t0, t1, t2);
+  }
+  __syncthreads();
+  if (tid < kHeadWords) reinterpret_cast<long long*>(c)[tid] = L.sh.w[tid];
+}
+
+// One workgroup of kPostThreads after every step.
+template <int kForced>
+__global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsParams* prm, int nblocks) {
+  __shared__ PostLds<kPostThreads> L;
+  post_body<kPostThreads, kForced>(a, prm, nblocks, L);
 }
 
 // Loop state for a fresh DOBFS (src/bfs.cc:150-159).  With decide_now the
 // first phase is scheduled (device loop); otherwise the host drives.
-__global__ void k_loop_init(StepArgs a, const BfsParams* prm, int decide_now) {
+__device__ void loop_init(const StepArgs& a, const BfsParams* prm, int decide_now) {
   BfsCtrl* c = a.ctrl;
   const int32_t source = prm->source;
   const int64_t ls = int64_t(source) - a.vbegin;
@@ -641,6 +671,7 @@ __global__ void k_loop_init(StepArgs a, const BfsParams* prm, int decide_now) {
   c->pairs = 0;
   c->td_in = 0;
   c->td_edges = 0;
+  c->scan_end = prm->scan_end;
   c->level = 0;
   c->done = 0;
   c->step = 0;
@@ -650,8 +681,40 @@ __global__ void k_loop_init(StepArgs a, const BfsParams* prm, int decide_now) {
   c->phase = kPhaseIdle;
   if (decide_now) {
     // The window [source] is already written: decide between TD and BU.
-    decide(c, prm);
+    decide(*c, c, prm);
     if (c->phase == kPhaseWrite) c->phase = kPhaseTD;
+  }
+}
+
+// InitParent (src/bfs.cc:131-140) + clearing of the three bitmaps, with the
+// source's bit set in bitmap 0 (the window [source] as a front), and the
+// loop state (one thread; it touches none of the arrays above).  Parents of
+// rows from init_end on already hold their initial values (rows no search
+// can reach); depths are always written in full.
+__global__ __launch_bounds__(kThreads) void k_init(StepArgs a, const BfsParams* prm, int decide_now) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) loop_init(a, prm, decide_now);
+  const int32_t source = prm->source;
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  const int64_t init_end = prm->init_end;
+  const int64_t end = a.depth ? a.nloc : init_end;
+  for (int64_t v = gtid; v < end; v += gsize) {
+    const int64_t d = a.out_off[v + 1] - a.out_off[v];
+    int32_t p = d != 0 ? static_cast<int32_t>(-d) : -1;
+    int32_t dep = -1;
+    if (a.vbegin + v == source) {
+      p = source;
+      dep = 0;
+    }
+    if (v < init_end) a.parent[v] = p;
+    if (a.depth) a.depth[v] = dep;
+  }
+  for (int64_t w = gtid; w < a.words; w += gsize) {
+    uint64_t x = 0;
+    if (w == (source >> 6)) x = 1ull << (source & 63);
+    a.bm0[w] = x;
+    a.bm1[w] = 0;
+    a.bm2[w] = 0;
   }
 }
 
@@ -679,7 +742,7 @@ __global__ __launch_bounds__(kThreads) void k_queue_to_bitmap(StepArgs a) {
 // slide_window after QueueToBitmap (src/bfs.cc:165-166) with its "e" record.
 __global__ void k_slide_to_empty(BfsCtrl* c) {
   const long long F = (c->qe - c->qb) + c->window_zero;
-  trace_step(c, DMM_STEP_QB, F, F, 0, 0);
+  trace_step(*c, c, DMM_STEP_QB, F, F, 0, 0);
   c->awake_count = F;
   c->qb = c->qe = c->qin;
   c->window_edges = 0;
@@ -778,7 +841,7 @@ int grid_blocks(dmm_ctx* ctx) {
 }
 
 int read_ctrl_head(dmm_ctx* ctx) {
-  const size_t head = offsetof(BfsCtrl, step_kind);
+  const size_t head = sizeof(BfsHead);
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_ctrl, ctx->d_ctrl, head, hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->h_ctrl->error)
@@ -793,7 +856,15 @@ int write_params(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta) {
   p->beta = beta;
   p->pad = 0;
   p->num_edges_directed = ctx->ned;
-  p->pad2 = 0;
+  // Rows from ctx->scan_end on (degree layout) are unreachable except as
+  // the source: a source there is scanned and initialised in full, and
+  // leaves its row's parent to reset at the next search.
+  int64_t scan_end = ctx->relabeled ? ctx->scan_end : ctx->nloc;
+  const bool src_tail = int64_t(source) - ctx->vbegin >= scan_end;
+  if (src_tail) scan_end = ctx->nloc;
+  p->scan_end = scan_end;
+  p->init_end = (ctx->parent_tail_dirty || src_tail) ? ctx->nloc : scan_end;
+  ctx->parent_tail_dirty = src_tail;
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params, sizeof(BfsParams), hipMemcpyHostToDevice,
                                     ctx->stream));
   return DMM_OK;
@@ -840,8 +911,7 @@ int bfs_init(dmm_ctx* ctx, int32_t source) {
   int rc = write_params(ctx, source, ctx->tuning.alpha, ctx->tuning.beta);
   if (rc) return rc;
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
-  hipLaunchKernelGGL(k_init, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, prm);
-  hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, ctx->stream, a, prm, 0);
+  hipLaunchKernelGGL(k_init, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, prm, 0);
   DMM_HIP_CHECK(ctx, hipGetLastError());
   ctx->bfs_valid = true;
   return read_ctrl_head(ctx);
@@ -951,13 +1021,15 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
   DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   const int blocks = grid_blocks(ctx);
   if (with_init) {
-    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm);
-    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, ctx->stream, a, prm, 1);
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, 1);
   }
   for (int i = 0; i < nsteps; i++) launch_pair<-1>(ctx, a, prm, blocks);
-  hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, 1);
-  // Results back to the caller's IDs (a no-op until the loop is done).
-  if (ctx->relabeled) launch_map_parent(ctx, with_depth, &ctx->d_ctrl->done);
+  // Results back to the caller's IDs, unreached as -1 (a no-op until the
+  // loop is done; either kernel marks the cleanup done).
+  if (ctx->relabeled)
+    launch_map_parent(ctx, with_depth, &ctx->d_ctrl->done);
+  else
+    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, 1);
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);

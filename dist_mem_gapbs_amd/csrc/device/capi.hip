@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -448,6 +449,9 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
       ctx->d_in_nbr = ctx->d_out_nbr;
     }
   }
+  ctx->scan_end = rows;
+  if (ctx->relabeled && (rc = dmm::compute_scan_end(ctx, directed ? new_in_off.data() : new_out_off.data())))
+    return rc;
   // BFS state.
   if ((rc = A((void**)&ctx->d_parent, sizeof(int32_t) * rows, "parent")) ||
       (rc = A((void**)&ctx->d_depth, sizeof(int32_t) * rows, "depth")) ||
@@ -700,7 +704,7 @@ int dmm_state_get_queue(dmm_ctx* ctx, int32_t* queue, int64_t capacity, int64_t*
   int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int64_t qb = ctx->h_ctrl->qb, qe = ctx->h_ctrl->qe;
   if (size) *size = qe - qb;
   if (qe - qb > capacity) return set_err(ctx, DMM_ERR_CAPACITY, "queue window of %lld > capacity", (long long)(qe - qb));
@@ -765,7 +769,7 @@ int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words) {
   int rc = need_natural(ctx);
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
   const int idx = which == 0 ? f : t;
   DMM_HIP_CHECK(ctx, hipMemcpy(words, ctx->d_bm[idx], sizeof(uint64_t) * ctx->words, hipMemcpyDeviceToHost));
@@ -777,7 +781,7 @@ int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words) {
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, offsetof(dmm::BfsCtrl, step_kind), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
   const int idx = which == 0 ? f : t;
   DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_bm[idx], words, sizeof(uint64_t) * ctx->words, hipMemcpyHostToDevice));

@@ -23,7 +23,9 @@ constexpr int kMaxGridPerCU = 32;   // upper bound of workgroups per CU
 constexpr int kPartWidth = 4;       // int64 partials per workgroup
 constexpr int kRecentSearches = 32; // searches sizing the device-loop chain
 
-struct BfsCtrl {
+// Loop state of a BFS on the device.  The head (BfsHead) is what the post
+// kernel reads and rewrites after every step; the trace arrays follow it.
+struct BfsHead {
   // Sliding-queue window [qb, qe), tail qin; the window's out-degree sum and
   // its members of out-degree 0 (counted, not stored).
   long long qb, qe, qin;
@@ -38,6 +40,9 @@ struct BfsCtrl {
   unsigned long long td_tail;
   // Inputs of the last top-down step, for its trace record.
   long long td_in, td_edges;
+  // Rows the bottom-up step scans: rows at and above it have no in-edges
+  // and are never claimed (the zero-degree tail of the degree layout).
+  long long scan_end;
   int level;
   int phase;       // kPhase* below
   int done;        // 1: loop finished, 2: cleanup done
@@ -47,6 +52,11 @@ struct BfsCtrl {
   int count_src;   // 0: count after a TD step, 1: after the BU loop
   int error;       // != 0: a device-side invariant was violated
   int pad;
+};
+constexpr int kHeadWords = sizeof(BfsHead) / 8;
+static_assert(sizeof(BfsHead) % 8 == 0 && kHeadWords <= 64, "BfsHead is copied as 8-byte words by one wave");
+
+struct BfsCtrl : BfsHead {
   // Per-step trace.
   int step_kind[DMM_MAX_STEPS];
   long long step_out[DMM_MAX_STEPS];
@@ -105,6 +115,12 @@ struct dmm_ctx {
   std::vector<int32_t> h_perm;     // host copy of perm (source mapping)
   int32_t* d_parent_out = nullptr; // results in original IDs
   int32_t* d_depth_out = nullptr;
+  // Degree layout: internal rows [scan_end, nloc) have no in-edges and are
+  // no out-neighbour, so only a search's source can change their parent.
+  // Their parents keep the InitParent values between searches unless the
+  // last source lay there (parent_tail_dirty).
+  int64_t scan_end = 0;
+  bool parent_tail_dirty = true;
   float* d_score_out = nullptr;
   double* d_score64_out = nullptr;
   bool depth_valid = false;        // the last BFS tracked depths

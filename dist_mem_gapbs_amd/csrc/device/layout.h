@@ -17,9 +17,15 @@ void degree_order(int64_t n, const int64_t* out_off, std::vector<int32_t>* perm,
 int relabel_upload(dmm_ctx* ctx, const int64_t* old_off, const int32_t* old_nbr, const std::vector<int32_t>& inv,
                    std::vector<int64_t>* new_off, int64_t** d_off, int32_t** d_nbr);
 
-// Results back to original IDs (device to device).
-// done_flag (may be NULL): skip unless *done_flag == 2 (device loop finished).
-void launch_map_parent(dmm_ctx* ctx, bool with_depth, const int* done_flag);
+// ctx->scan_end of an uploaded relabeled graph: one past the last internal
+// row with in-edges (in_off: the host copy of the internal in-offsets) or
+// appearing as an out-neighbour.
+int compute_scan_end(dmm_ctx* ctx, const int64_t* in_off);
+
+// Parents (and depths) back to original IDs, -1 where unreached (device to
+// device).  done_flag (may be NULL): skip while *done_flag == 0 (device loop
+// not finished), else set it to 2.
+void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag);
 void launch_map_scores_f32(dmm_ctx* ctx);
 void launch_map_scores_f64(dmm_ctx* ctx);
 

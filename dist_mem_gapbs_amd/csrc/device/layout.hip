@@ -42,21 +42,49 @@ __global__ __launch_bounds__(kThreads) void k_relabel_rows(const int64_t* __rest
   }
 }
 
+// parent_out[v] = inv[parent[perm[v]]], -1 where unreached (the cleanup of
+// gapbs-verify/bfs.cc:168-171); rows from *scan_end on were not reached
+// and are not read.  With done_flag: nothing until the device loop has
+// finished (*done_flag != 0), then marks the cleanup done (2).
 __global__ __launch_bounds__(kThreads) void k_map_parent(const int32_t* __restrict__ parent,
                                                         const int32_t* __restrict__ depth,
                                                         const int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ inv, int64_t n,
+                                                        const long long* scan_end_p,
                                                         int32_t* __restrict__ parent_out,
-                                                        int32_t* __restrict__ depth_out,
-                                                        const int* done_flag) {
-  if (done_flag && *done_flag != 2) return;
+                                                        int32_t* __restrict__ depth_out, int* done_flag) {
+  if (done_flag && *done_flag == 0) return;
+  const int64_t scan_end = *scan_end_p;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
   for (int64_t v = gtid; v < n; v += gsize) {
     const int32_t u = perm[v];
-    const int32_t p = parent[u];
-    parent_out[v] = p >= 0 ? inv[p] : p;
-    if (depth) depth_out[v] = depth[u];
+    int32_t po = -1, dep = -1;
+    if (u < scan_end) {
+      const int32_t p = parent[u];
+      if (p >= 0) po = inv[p];
+      if (depth) dep = depth[u];
+    }
+    parent_out[v] = po;
+    if (depth) depth_out[v] = dep;
+  }
+  if (done_flag && gtid == 0) *done_flag = 2;
+}
+
+// Largest internal out-neighbour ID + 1 (block maximum, one atomic each).
+__global__ __launch_bounds__(kThreads) void k_max_nbr(const int32_t* __restrict__ nbr, int64_t m,
+                                                     unsigned long long* out) {
+  __shared__ unsigned long long red[kThreads / 64];
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  unsigned long long mx = 0;
+  for (int64_t e = gtid; e < m; e += gsize) mx = max(mx, static_cast<unsigned long long>(nbr[e]) + 1);
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, static_cast<unsigned long long>(__shfl_xor(mx, o)));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kThreads / 64; i++) mx = max(mx, red[i]);
+    atomicMax(out, mx);
   }
 }
 
@@ -134,10 +162,31 @@ int relabel_upload(dmm_ctx* ctx, const int64_t* old_off, const int32_t* old_nbr,
   return DMM_OK;
 }
 
-void launch_map_parent(dmm_ctx* ctx, bool with_depth, const int* done_flag) {
+int compute_scan_end(dmm_ctx* ctx, const int64_t* in_off) {
+  int64_t end = 0;
+  for (int64_t r = ctx->nloc; r > 0; r--)
+    if (in_off[r] > in_off[r - 1]) {
+      end = r;
+      break;
+    }
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_scratch64);
+  unsigned long long h = 0;
+  DMM_HIP_CHECK(ctx, hipMemsetAsync(d, 0, sizeof(h), ctx->stream));
+  if (ctx->m_out > 0)
+    hipLaunchKernelGGL(k_max_nbr, dim3(grid_for(ctx, ctx->m_out)), dim3(kThreads), 0, ctx->stream, ctx->d_out_nbr,
+                       ctx->m_out, d);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->scan_end = std::min<int64_t>(ctx->nloc, std::max<int64_t>(end, static_cast<int64_t>(h)));
+  ctx->parent_tail_dirty = true;
+  return DMM_OK;
+}
+
+void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag) {
   hipLaunchKernelGGL(k_map_parent, dim3(grid_for(ctx, ctx->nloc)), dim3(kThreads), 0, ctx->stream, ctx->d_parent,
-                     with_depth ? ctx->d_depth : nullptr, ctx->d_perm, ctx->d_inv, ctx->nloc, ctx->d_parent_out,
-                     ctx->d_depth_out, done_flag);
+                     with_depth ? ctx->d_depth : nullptr, ctx->d_perm, ctx->d_inv, ctx->nloc,
+                     &ctx->d_ctrl->scan_end, ctx->d_parent_out, ctx->d_depth_out, done_flag);
 }
 
 void launch_map_scores_f32(dmm_ctx* ctx) {

@@ -117,6 +117,31 @@ def test_zero_degree_source(ctx, dmm, orc):
     assert trace_of(st) == tr
 
 
+def test_isolated_sources_between_searches(ctx, dmm, orc, graph):
+    # In the degree layout the zero-degree vertices form the tail of the
+    # internal order; the bottom-up step skips it and InitParent rewrites it
+    # only after a search that started there.  Alternate such sources with
+    # ordinary ones, with and without depths, in both loop modes.
+    g = graph(12, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    iso = np.flatnonzero(np.diff(g.out_offsets) == 0)
+    assert len(iso) > 3
+    srcs = dmm.pick_sources(g, 3)
+    order = [srcs[0], iso[0], srcs[1], iso[-1], iso[1], srcs[2], srcs[0]]
+    for want_depth in (True, False):
+        for trace in (False, True):
+            for s in order:
+                parent, depth, st = ctx.DOBFS(int(s), trace=trace, want_depth=want_depth)
+                ref = orc.bfs_depth(og, int(s))
+                if want_depth:
+                    assert np.array_equal(depth, ref), f"depth mismatch from {s}"
+                assert np.array_equal(parent >= 0, ref >= 0), f"reached set differs from {s}"
+                ok, bad = orc.bfs_verify(og, int(s), parent)
+                assert ok, f"parent tree fails the verifier at {bad} (source {s})"
+                assert st.reached == int((ref >= 0).sum())
+
+
 def test_directed_graph(ctx, dmm, orc):
     rng = np.random.default_rng(5)
     pairs = rng.integers(0, 2000, size=(12000, 2))
