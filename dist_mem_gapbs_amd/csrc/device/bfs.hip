@@ -66,6 +66,7 @@ struct StepArgs {
   uint64_t* bm1;
   uint64_t* bm2;
   int64_t* part;                          // [gridDim.x][kPartWidth]
+  int64_t part_half;                      // offset of the device loop's second partial buffer
   BfsCtrl* ctrl;
   int64_t beta_width;                     // Partition::max_width
   int bu_aligned;                         // local block starts/ends on words
@@ -90,6 +91,18 @@ __device__ __forceinline__ int role_scratch(int f) { return (f >> 4) & 3; }
 __device__ __forceinline__ int roles(int front, int spare, int scratch) {
   return front | (spare << 2) | (scratch << 4);
 }
+
+// What a step reads of the loop state and where it reports: taken from the
+// control block (host-driven steps) or from a workgroup's own copy of the
+// loop state (device-loop slots).
+struct StepIo {
+  long long qb, qe, qin, window_edges, scan_end;
+  int front, level;
+  unsigned long long* td_tail;  // top-down append cursor
+  int* error;                   // set on an invariant violation
+  int64_t* part;                // this step's partials [gridDim.x][kPartWidth]
+  int64_t base_c, base_d;       // WRITE: the workgroup's queue and degree prefix
+};
 
 // About 16 KiB of LDS, so eight 256-thread workgroups (32 waves) fit a CU.
 struct TdLds {
@@ -126,13 +139,14 @@ union StepLds {
 // queue directly; the host-driven step recounts the bitmap and rewrites the
 // queue in ascending order (COUNT, WRITE) as BitmapToQueue would.
 // ------------------------------------------------------------------------
-__device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
-  const int64_t qb = c->qb;
-  const int64_t F = c->qe - c->qb;
-  const int64_t T = c->window_edges;
+__device__ void td_phase(const StepArgs& a, const StepIo& io, TdLds& L) {
+  const int level = io.level;
+  const int64_t qb = io.qb;
+  const int64_t F = io.qe - io.qb;
+  const int64_t T = io.window_edges;
   const int32_t* q = a.queue + qb;
   const int64_t* qoff = a.qoff + qb;
-  const int fr = c->front;
+  const int fr = io.front;
   uint64_t* out_bm = bm_sel(a, role_spare(fr));
   uint64_t* clr = bm_sel(a, role_scratch(fr));
   // The scratch bitmap becomes the next spare: clear it.
@@ -141,7 +155,7 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
   for (int64_t w = gtid; w < a.words; w += gsize) clr[w] = 0;
 
   const int tid = threadIdx.x;
-  const long long qin = c->qin;
+  const long long qin = io.qin;
   int64_t blk_cnt = 0, blk_degs = 0, my_zero = 0;  // blk_* valid in thread 0
   for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
     const int64_t e1 = min(T, e0 + kEPB);
@@ -167,7 +181,7 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
     // kEPB + 1 entries; a violation is reported instead of overrunning LDS.
     int nv = static_cast<int>(L.nv);
     if (nv > kEPB + 2) {
-      if (tid == 0) c->error = 1;
+      if (tid == 0) *io.error = 1;
       nv = kEPB + 2;
     }
     for (int i = tid; i < nv; i += kThreads) {
@@ -223,7 +237,7 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
     const int64_t ed = block_excl_scan<kThreads>(dsum, L.sd, &td);
     if (tc > 0) {
       if (tid == 0)
-        L.tail = atomicAdd(&c->td_tail, (static_cast<unsigned long long>(tc) << 34) |
+        L.tail = atomicAdd(io.td_tail, (static_cast<unsigned long long>(tc) << 34) |
                                             static_cast<unsigned long long>(td));
       __syncthreads();
       const unsigned long long t = L.tail;
@@ -245,7 +259,7 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
   }
   const int64_t z = block_sum<kThreads>(my_zero, L.sc);
   if (threadIdx.x == 0) {
-    int64_t* p = a.part + blockIdx.x * kPartWidth;
+    int64_t* p = io.part + blockIdx.x * kPartWidth;
     p[0] = blk_cnt;
     p[1] = blk_degs;
     p[2] = z;
@@ -259,15 +273,16 @@ __device__ void td_phase(const StepArgs& a, BfsCtrl* c, TdLds& L, int level) {
 // in-neighbours until a frontier bit is found; the wave's ballot is the
 // next-bitmap word.  Partials: awake, edges examined, vertices scanned.
 // ------------------------------------------------------------------------
-__device__ void bu_phase(const StepArgs& a, BfsCtrl* c, SmallLds& L, int level) {
-  const int fr = c->front;
+__device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
+  const int level = io.level;
+  const int fr = io.front;
   const uint64_t* __restrict__ front = bm_sel(a, role_front(fr));
   uint64_t* __restrict__ next = bm_sel(a, role_scratch(fr));
   const int lane = lane_id();
   const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
   // Rows from scan_end on have no in-edges: never found, not scanned.
-  const int64_t nchunks = (c->scan_end + 63) >> 6;
+  const int64_t nchunks = (io.scan_end + 63) >> 6;
   const int64_t* __restrict__ in_off = a.in_off;
   const int32_t* __restrict__ in_nbr = a.in_nbr;
   int32_t* __restrict__ parent = a.parent;
@@ -328,7 +343,7 @@ __device__ void bu_phase(const StepArgs& a, BfsCtrl* c, SmallLds& L, int level) 
     int64_t s[3] = {0, 0, 0};
     for (int i = 0; i < kThreads / 64; i++)
       for (int k = 0; k < 3; k++) s[k] += L.red[k][i];
-    int64_t* p = a.part + blockIdx.x * kPartWidth;
+    int64_t* p = io.part + blockIdx.x * kPartWidth;
     p[0] = s[0];
     p[1] = s[1];
     p[2] = s[2];
@@ -346,8 +361,8 @@ __device__ __forceinline__ void block_words(const StepArgs& a, int64_t* w0, int6
 }
 
 template <bool kWrite>
-__device__ void count_or_write(const StepArgs& a, BfsCtrl* c, SmallLds& L) {
-  const uint64_t* front = bm_sel(a, role_front(c->front));
+__device__ void count_or_write(const StepArgs& a, const StepIo& io, SmallLds& L) {
+  const uint64_t* front = bm_sel(a, role_front(io.front));
   int64_t w0, w1, wpt;
   block_words(a, &w0, &w1, &wpt);
   const int64_t t0 = min(w1, w0 + threadIdx.x * wpt), t1 = min(w1, t0 + wpt);
@@ -374,7 +389,7 @@ __device__ void count_or_write(const StepArgs& a, BfsCtrl* c, SmallLds& L) {
     const int64_t sd = block_sum<kThreads>(degs, L.red[1]);
     const int64_t sz = block_sum<kThreads>(zero, L.red[2]);
     if (threadIdx.x == 0) {
-      int64_t* p = a.part + blockIdx.x * kPartWidth;
+      int64_t* p = io.part + blockIdx.x * kPartWidth;
       p[0] = sc;
       p[1] = sd;
       p[2] = sz;
@@ -382,12 +397,12 @@ __device__ void count_or_write(const StepArgs& a, BfsCtrl* c, SmallLds& L) {
     }
     return;
   }
-  // WRITE: the post kernel replaced the partials by exclusive prefixes.
+  // WRITE: the workgroup's prefix over the counted partials.
   int64_t tc, td;
-  int64_t pos = a.part[blockIdx.x * kPartWidth + 0] + block_excl_scan<kThreads>(cnt, L.scan_c, &tc);
-  int64_t run = a.part[blockIdx.x * kPartWidth + 1] + block_excl_scan<kThreads>(degs, L.scan_d, &td);
-  int32_t* oq = a.queue + c->qin;
-  int64_t* oqoff = a.qoff + c->qin;
+  int64_t pos = io.base_c + block_excl_scan<kThreads>(cnt, L.scan_c, &tc);
+  int64_t run = io.base_d + block_excl_scan<kThreads>(degs, L.scan_d, &td);
+  int32_t* oq = a.queue + io.qin;
+  int64_t* oqoff = a.qoff + io.qin;
   for (int64_t wi = t0; wi < t1; wi++) {
     const int64_t w = a.wbegin + wi;
     uint64_t x = front[w];
@@ -408,11 +423,12 @@ __device__ void count_or_write(const StepArgs& a, BfsCtrl* c, SmallLds& L) {
 }
 
 // Appends one step record: the counters go to the head `h` (the post
-// kernel's LDS copy, or the control block itself), the record to c's trace.
+// kernel's LDS copy, or the control block itself), the record to c's trace
+// (none if c is null).
 __device__ void trace_step(BfsHead& h, BfsCtrl* c, int kind, long long in, long long out, long long edges,
                            long long scout) {
   const int s = h.step;
-  if (s < DMM_MAX_STEPS) {
+  if (c && s < DMM_MAX_STEPS) {
     c->step_kind[s] = kind;
     c->step_in[s] = in;
     c->step_out[s] = out;
@@ -443,17 +459,32 @@ __device__ void decide(BfsHead& h, BfsCtrl* c, const BfsParams* prm) {
   }
 }
 
+// One host-driven step (kForced) over the control block's state; the
+// partials go to the first buffer, and a WRITE takes the prefixes the COUNT
+// post left there.
 template <int kForced>
 __global__ __launch_bounds__(kThreads) void k_step(StepArgs a, const BfsParams* prm) {
   __shared__ StepLds lds;
   BfsCtrl* c = a.ctrl;
-  const int phase = kForced >= 0 ? kForced : c->phase;
-  const int level = c->level;
+  const int phase = kForced;
+  StepIo io;
+  io.qb = c->qb;
+  io.qe = c->qe;
+  io.qin = c->qin;
+  io.window_edges = c->window_edges;
+  io.scan_end = c->scan_end;
+  io.front = c->front;
+  io.level = c->level;
+  io.td_tail = &c->td_tail;
+  io.error = &c->error;
+  io.part = a.part;
+  io.base_c = phase == kPhaseWrite ? a.part[blockIdx.x * kPartWidth + 0] : 0;
+  io.base_d = phase == kPhaseWrite ? a.part[blockIdx.x * kPartWidth + 1] : 0;
   switch (phase) {
-    case kPhaseTD: td_phase(a, c, lds.td, level); break;
-    case kPhaseBU: bu_phase(a, c, lds.sm, level); break;
-    case kPhaseCount: count_or_write<false>(a, c, lds.sm); break;
-    case kPhaseWrite: count_or_write<true>(a, c, lds.sm); break;
+    case kPhaseTD: td_phase(a, io, lds.td); break;
+    case kPhaseBU: bu_phase(a, io, lds.sm); break;
+    case kPhaseCount: count_or_write<false>(a, io, lds.sm); break;
+    case kPhaseWrite: count_or_write<true>(a, io, lds.sm); break;
     default: break;
   }
   (void)prm;
@@ -640,6 +671,101 @@ __global__ __launch_bounds__(kPostThreads) void k_post(StepArgs a, const BfsPara
   post_body<kPostThreads, kForced>(a, prm, nblocks, L);
 }
 
+// Device-loop slot k (parity = k & 1).  Every workgroup first applies the
+// post of slot k-1 to its own LDS copy of the loop state: the same inputs
+// (head[parity] and slot k-1's partials) give every workgroup the same
+// result, which workgroup 0 records (head[parity ^ 1] and the trace).  Then
+// each runs the step that post scheduled, its partials into buffer
+// [parity].  Double-buffered heads, partials and append cursors keep a slot
+// from overwriting what its own workgroups still read; the kernel boundary
+// between slots is the only synchronisation, and no post kernel runs.
+struct SlotLds {
+  union {
+    BfsHead h;
+    long long w[kHeadWords];
+  } sh;
+  int64_t red[5][kThreads / 64];
+  int64_t base_c, base_d;
+};
+
+__global__ __launch_bounds__(kThreads) void k_slot(StepArgs a, const BfsParams* prm, int parity) {
+  __shared__ StepLds lds;
+  __shared__ SlotLds L;
+  BfsCtrl* c = a.ctrl;
+  const BfsHead* hin = parity ? &c->alt : static_cast<const BfsHead*>(c);
+  BfsHead* hout = parity ? static_cast<BfsHead*>(c) : &c->alt;
+  const int64_t* pin = a.part + (parity ^ 1) * a.part_half;
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  const int me = blockIdx.x;
+  if (tid < kHeadWords) L.sh.w[tid] = reinterpret_cast<const long long*>(hin)[tid];
+  // Totals of slot k-1's partials, and this workgroup's prefix of the
+  // count and degree partials (the WRITE positions after a COUNT).
+  int64_t t0 = 0, t1 = 0, t2 = 0, b0 = 0, b1 = 0;
+  for (int b = tid; b < G; b += kThreads) {
+    const int64_t v0 = pin[b * kPartWidth + 0];
+    const int64_t v1 = pin[b * kPartWidth + 1];
+    const int64_t v2 = pin[b * kPartWidth + 2];
+    t0 += v0;
+    t1 += v1;
+    t2 += v2;
+    if (b < me) {
+      b0 += v0;
+      b1 += v1;
+    }
+  }
+  {
+    const int64_t w0 = wave_sum(t0), w1 = wave_sum(t1), w2 = wave_sum(t2), w3 = wave_sum(b0), w4 = wave_sum(b1);
+    if (lane_id() == 0) {
+      const int w = tid >> 6;
+      L.red[0][w] = w0;
+      L.red[1][w] = w1;
+      L.red[2][w] = w2;
+      L.red[3][w] = w3;
+      L.red[4][w] = w4;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int64_t sum[5] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < kThreads / 64; w++)
+      for (int k = 0; k < 5; k++) sum[k] += L.red[k][w];
+    BfsHead& h = L.sh.h;
+    h.error |= c->slot_error;
+    if (h.ran) post_update<-1>(h, me == 0 ? c : nullptr, a, prm, h.ran, sum[0], sum[1], sum[2]);
+    h.ran = (h.phase >= kPhaseTD && h.phase <= kPhaseWrite) ? h.phase : 0;
+    L.base_c = sum[3];
+    L.base_d = sum[4];
+  }
+  __syncthreads();
+  const BfsHead& h = L.sh.h;
+  if (me == 0) {
+    if (tid < kHeadWords) reinterpret_cast<long long*>(hout)[tid] = L.sh.w[tid];
+    if (tid == 0) c->slot_tail[parity ^ 1] = 0;  // slot k+1's cursor (slot k-1 is done with it)
+  }
+  const int phase = h.ran;
+  if (phase == 0) return;
+  StepIo io;
+  io.qb = h.qb;
+  io.qe = h.qe;
+  io.qin = h.qin;
+  io.window_edges = h.window_edges;
+  io.scan_end = h.scan_end;
+  io.front = h.front;
+  io.level = h.level;
+  io.td_tail = &c->slot_tail[parity];
+  io.error = &c->slot_error;
+  io.part = a.part + parity * a.part_half;
+  io.base_c = L.base_c;
+  io.base_d = L.base_d;
+  switch (phase) {
+    case kPhaseTD: td_phase(a, io, lds.td); break;
+    case kPhaseBU: bu_phase(a, io, lds.sm); break;
+    case kPhaseCount: count_or_write<false>(a, io, lds.sm); break;
+    default: count_or_write<true>(a, io, lds.sm); break;
+  }
+}
+
 // Loop state for a fresh DOBFS (src/bfs.cc:150-159).  With decide_now the
 // first phase is scheduled (device loop); otherwise the host drives.
 __device__ void loop_init(const StepArgs& a, const BfsParams* prm, int decide_now) {
@@ -668,6 +794,10 @@ __device__ void loop_init(const StepArgs& a, const BfsParams* prm, int decide_no
   c->pend_degs = deg;
   c->pend_zero = 0;
   c->td_tail = 0;
+  c->slot_tail[0] = 0;
+  c->slot_tail[1] = 0;
+  c->slot_error = 0;
+  c->ran = 0;
   c->pairs = 0;
   c->td_in = 0;
   c->td_edges = 0;
@@ -828,6 +958,7 @@ StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   a.bm1 = ctx->d_bm[1];
   a.bm2 = ctx->d_bm[2];
   a.part = ctx->d_part;
+  a.part_half = (ctx->part_cap / 2) * kPartWidth;
   a.ctrl = ctx->d_ctrl;
   // Partition::max_width of the PE (src/partition.h:98).
   a.beta_width = ctx->beta_width;
@@ -837,7 +968,7 @@ StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
 
 int grid_blocks(dmm_ctx* ctx) {
   const int g = std::max(1, ctx->cu_count) * std::max(1, std::min(kMaxGridPerCU, ctx->tuning.grid_per_cu));
-  return static_cast<int>(std::min<int64_t>(g, ctx->part_cap));
+  return static_cast<int>(std::min<int64_t>(g, ctx->part_cap / 2));  // the device loop double-buffers
 }
 
 int read_ctrl_head(dmm_ctx* ctx) {
@@ -1023,7 +1154,8 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
   if (with_init) {
     hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, 1);
   }
-  for (int i = 0; i < nsteps; i++) launch_pair<-1>(ctx, a, prm, blocks);
+  for (int i = 0; i < nsteps; i++)
+    hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, i & 1);
   // Results back to the caller's IDs, unreached as -1 (a no-op until the
   // loop is done; either kernel marks the cleanup done).
   if (ctx->relabeled)
@@ -1040,9 +1172,11 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
 
 int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool with_depth,
                     dmm_bfs_stats* stats) {
+  // Slot counts are even, so every graph ends with the latest loop state
+  // in the first head (and a continuation starts from it).
   constexpr int kCont = 6;
   const int d = with_depth ? 1 : 0;
-  const int want = std::max(4, ctx->bfs_chain_len);
+  const int want = (std::max(4, ctx->bfs_chain_len) + 1) & ~1;
   if (!ctx->bfs_graph[d] || ctx->bfs_graph_steps[d] != want) {
     if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
     ctx->bfs_graph[d] = nullptr;
@@ -1071,13 +1205,13 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
   }
   ctx->bfs_valid = true;
   ctx->depth_valid = with_depth;
-  // Size the first chain to the most pairs any of the last searches needed,
-  // so a typical search finishes in one graph launch without trailing
-  // no-op pairs.
+  // Size the first chain to the most steps any of the last searches ran,
+  // plus the slot that applies the last step's post, so a typical search
+  // finishes in one graph launch without trailing no-op slots.
   ctx->bfs_recent[ctx->bfs_recent_pos++ % kRecentSearches] = ctx->h_ctrl->pairs;
   int most = 0;
   for (int i = 0; i < kRecentSearches; i++) most = std::max(most, ctx->bfs_recent[i]);
-  ctx->bfs_chain_len = std::min(128, std::max(4, most));
+  ctx->bfs_chain_len = std::min(128, std::max(4, most + 1));
   if (stats) {
     float ms = 0;
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));

@@ -51,7 +51,7 @@ struct BfsHead {
   int front;       // bitmap roles: front | spare << 2 | scratch << 4
   int count_src;   // 0: count after a TD step, 1: after the BU loop
   int error;       // != 0: a device-side invariant was violated
-  int pad;
+  int ran;         // device loop: the phase the slot runs (its post pending)
 };
 constexpr int kHeadWords = sizeof(BfsHead) / 8;
 static_assert(sizeof(BfsHead) % 8 == 0 && kHeadWords <= 64, "BfsHead is copied as 8-byte words by one wave");
@@ -63,6 +63,13 @@ struct BfsCtrl : BfsHead {
   long long step_in[DMM_MAX_STEPS];
   long long step_edges[DMM_MAX_STEPS];
   long long step_scout[DMM_MAX_STEPS];
+  // Device loop: the second loop-state head (slot k reads head k & 1 and
+  // writes the other), the slots' top-down append cursors, and the
+  // invariant flag the slots' steps raise.
+  BfsHead alt;
+  unsigned long long slot_tail[2];
+  int slot_error;
+  int pad2;
 };
 
 enum : int {
