@@ -70,6 +70,7 @@ struct StepArgs {
   BfsCtrl* ctrl;
   int64_t beta_width;                     // Partition::max_width
   int bu_aligned;                         // local block starts/ends on words
+  int undirected;                         // in-CSR == out-CSR
 };
 
 struct BfsParams {
@@ -109,7 +110,7 @@ struct TdLds {
   int32_t rel[kEPB + 2];
   int64_t base[kEPB + 2];
   int32_t u[kEPB + 2];
-  int64_t i0, nv;
+  int64_t ie[2];
   int64_t sc[kThreads / 64 + 1];
   int64_t sd[kThreads / 64 + 1];
   unsigned long long tail;
@@ -125,6 +126,23 @@ union StepLds {
   TdLds td;
   SmallLds sm;
 };
+
+// Last index i in [lo, hi) with key[i] <= x, for a non-decreasing key with
+// key[lo] <= x.  One wavefront searches 64-ary: each round probes 64 evenly
+// spaced keys at once (one load per lane) and keeps the interval after the
+// last one <= x, so a window of F entries takes ceil(log64 F) dependent
+// loads instead of log2 F.
+__device__ int64_t wave_last_le(const int64_t* key, int64_t lo, int64_t hi, int64_t x) {
+  const int lane = lane_id();
+  while (hi - lo > 1) {
+    const int64_t step = (hi - lo + 63) >> 6;
+    const int64_t p = lo + lane * step;
+    const unsigned long long m = __ballot(p < hi && key[p] <= x);  // lanes 0..j, lane 0 always
+    lo += (63 - __clzll(m)) * step;
+    hi = min(hi, lo + step);
+  }
+  return lo;
+}
 
 // ------------------------------------------------------------------------
 // Top-down step: src/bfs.cc:81-100.  Edge-balanced over the window: chunk k
@@ -159,27 +177,23 @@ __device__ void td_phase(const StepArgs& a, const StepIo& io, TdLds& L) {
   int64_t blk_cnt = 0, blk_degs = 0, my_zero = 0;  // blk_* valid in thread 0
   for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
     const int64_t e1 = min(T, e0 + kEPB);
-    if (tid == 0) {
-      // i0 = last i with qoff[i] <= e0; i1 = last i with qoff[i] <= e1 - 1.
-      int64_t lo = 0, hi = F;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (qoff[mid] <= e0) lo = mid; else hi = mid;
+    // i0 = last i with qoff[i] <= e0 (wave 0); i1 = last i with qoff[i] <=
+    // e1 - 1 (wave 1).
+    // A window that fits the LDS staging is staged whole.
+    if (F <= kEPB + 2) {
+      if (tid == 0) {
+        L.ie[0] = 0;
+        L.ie[1] = F - 1;
       }
-      const int64_t i0 = lo;
-      hi = F;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (qoff[mid] <= e1 - 1) lo = mid; else hi = mid;
-      }
-      L.i0 = i0;
-      L.nv = lo - i0 + 1;
+    } else if (tid < 128) {
+      const int64_t r = wave_last_le(qoff, 0, F, tid < 64 ? e0 : e1 - 1);
+      if ((tid & 63) == 0) L.ie[tid >> 6] = r;
     }
     __syncthreads();
-    const int64_t i0 = L.i0;
+    const int64_t i0 = L.ie[0];
     // Queued vertices have out-degree >= 1, so a chunk spans at most
     // kEPB + 1 entries; a violation is reported instead of overrunning LDS.
-    int nv = static_cast<int>(L.nv);
+    int nv = static_cast<int>(L.ie[1] - i0 + 1);
     if (nv > kEPB + 2) {
       if (tid == 0) *io.error = 1;
       nv = kEPB + 2;
@@ -217,8 +231,10 @@ __device__ void td_phase(const StepArgs& a, const StepIo& io, TdLds& L) {
           if (atomicCAS(&a.parent[lv], cur, u) == cur) {
             if (a.depth) a.depth[lv] = level + 1;
             atomicOr(reinterpret_cast<unsigned long long*>(&out_bm[v >> 6]), 1ull << (v & 63));
-            // parent = -1 stands for out-degree 0 and 1: read the degree.
-            const int64_t d = a.out_off[lv + 1] - a.out_off[lv];
+            // parent = -out-degree, but -1 stands for out-degree 0 and 1:
+            // in an undirected graph the edge u - v makes it 1; otherwise
+            // read the degree.
+            const int64_t d = cur < -1 ? -int64_t(cur) : a.undirected ? 1 : a.out_off[lv + 1] - a.out_off[lv];
             if (d > 0) {
               cv[k] = v;
               cd[k] = d;
@@ -962,6 +978,7 @@ StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   a.ctrl = ctx->d_ctrl;
   // Partition::max_width of the PE (src/partition.h:98).
   a.beta_width = ctx->beta_width;
+  a.undirected = ctx->directed ? 0 : 1;
   a.bu_aligned = ((ctx->vbegin & 63) == 0 && ((ctx->vend & 63) == 0 || ctx->vend == ctx->n)) ? 1 : 0;
   return a;
 }
