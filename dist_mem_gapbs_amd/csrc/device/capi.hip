@@ -468,9 +468,10 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
   for (int64_t r = 0; r < rows; r++)
     if (ioff[r + 1] - ioff[r] >= ctx->heavy_threshold) heavy.push_back(static_cast<int32_t>(r));
   ctx->n_heavy = static_cast<int64_t>(heavy.size());
-  // When rows come in non-increasing in-degree order (the degree-ordered
-  // layout of an undirected graph) the light rows form one range after the
-  // heavy prefix and the rows without in-edges form the tail.
+  // When rows come in non-increasing in-degree class order (the degree
+  // layout of an undirected graph; the threshold is a power of two) the
+  // light rows form one range after the heavy prefix and the rows without
+  // in-edges form the tail.
   {
     int64_t lb = 0;
     while (lb < rows && ioff[lb + 1] - ioff[lb] >= ctx->heavy_threshold) lb++;

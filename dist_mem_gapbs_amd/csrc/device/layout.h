@@ -9,7 +9,9 @@
 namespace dmm {
 
 // perm: original -> internal ID; inv: internal -> original.  Internal IDs
-// sort vertices by non-increasing out-degree, ties by original ID.
+// sort vertices by non-increasing out-degree class (bit length of the
+// degree: 0 for isolated, then [1], [2, 3], [4, 7], ...), ties by original
+// ID.  Any threshold that is a power of two splits the order in two ranges.
 void degree_order(int64_t n, const int64_t* out_off, std::vector<int32_t>* perm, std::vector<int32_t>* inv);
 
 // Uploads one CSR in the internal numbering (rows reordered, IDs renamed,

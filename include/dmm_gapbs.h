@@ -144,8 +144,9 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin,
                      const int32_t* in_neighs, int64_t num_edges_directed);
 
 /* Internal vertex numbering of whole-graph uploads.  DMM_LAYOUT_DEGREE
- * (the default) renumbers vertices by non-increasing out-degree inside the
- * ctx so the most-gathered per-vertex data stays in L2; every result is
+ * (the default) renumbers vertices by non-increasing out-degree class
+ * (degrees [2^k, 2^(k+1)) form a class; original order within a class)
+ * inside the ctx so the most-gathered per-vertex data stays in L2; every result is
  * returned in the caller's IDs.  DMM_LAYOUT_NATURAL keeps the caller's IDs
  * and is required by the single-step API (dmm_step_*, dmm_state_*), whose
  * state is exposed as is.  Takes effect at the next dmm_graph_upload; PE
