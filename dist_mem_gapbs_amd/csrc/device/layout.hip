@@ -1,8 +1,11 @@
 // Degree-ordered vertex layout.
 //
 // A whole-graph upload renumbers the vertices by non-increasing out-degree
-// (ties by original ID) before the CSR goes to HBM.  The high-degree
-// vertices -- the endpoints of most edges in power-law graphs -- then occupy
+// class -- the bit length of the degree, so classes [2^k, 2^(k+1)) -- with
+// ties in original-ID order, before the CSR goes to HBM.  Within a class
+// the original order is kept, so mapping results back to original IDs
+// gathers from a few dozen sequential streams instead of at random.  The
+// high-degree vertices -- the endpoints of most edges in power-law graphs -- then occupy
 // the first positions of every per-vertex array, so the random gathers of
 // PageRank contributions, bottom-up frontier probes and 64-lane frontier
 // words mostly hit a few MB that stay resident in each XCD's 4 MB L2
@@ -105,21 +108,25 @@ int grid_for(dmm_ctx* ctx, int64_t items) {
 }  // namespace
 
 void degree_order(int64_t n, const int64_t* out_off, std::vector<int32_t>* perm, std::vector<int32_t>* inv) {
-  // Counting sort by out-degree, descending, stable in the original ID.
-  int64_t maxdeg = 0;
-  for (int64_t v = 0; v < n; v++) maxdeg = std::max(maxdeg, out_off[v + 1] - out_off[v]);
-  std::vector<int64_t> pos(maxdeg + 2, 0);
-  for (int64_t v = 0; v < n; v++) pos[maxdeg - (out_off[v + 1] - out_off[v])]++;
+  // Counting sort by degree class, descending, stable in the original ID.
+  // Class = bit length of the out-degree (0 for degree 0).
+  auto cls = [&](int64_t v) {
+    const uint64_t d = static_cast<uint64_t>(out_off[v + 1] - out_off[v]);
+    return d == 0 ? 0 : 64 - __builtin_clzll(d);
+  };
+  constexpr int kClasses = 65;
+  std::vector<int64_t> pos(kClasses + 1, 0);
+  for (int64_t v = 0; v < n; v++) pos[kClasses - 1 - cls(v)]++;
   int64_t run = 0;
-  for (int64_t d = 0; d <= maxdeg; d++) {
-    const int64_t c = pos[d];
-    pos[d] = run;
-    run += c;
+  for (int c = 0; c < kClasses; c++) {
+    const int64_t k = pos[c];
+    pos[c] = run;
+    run += k;
   }
   perm->resize(n);
   inv->resize(n);
   for (int64_t v = 0; v < n; v++) {
-    const int64_t r = pos[maxdeg - (out_off[v + 1] - out_off[v])]++;
+    const int64_t r = pos[kClasses - 1 - cls(v)]++;
     (*perm)[v] = static_cast<int32_t>(r);
     (*inv)[r] = static_cast<int32_t>(v);
   }
