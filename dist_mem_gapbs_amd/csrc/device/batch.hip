@@ -64,6 +64,7 @@ struct BArgs {
   const int64_t* in_off;
   const int32_t* in_nbr;
   int64_t nloc, vbegin;
+  int64_t bend;       // rows [bend, nloc) have no in-edges: no lanes after level 0
   uint64_t* levels;   // level_cap * nloc
   int64_t level_cap;
   uint64_t* seen;
@@ -83,7 +84,7 @@ struct TdLds {
   int64_t base[kEPB + 2];
   int32_t u[kEPB + 2];
   uint64_t f[kEPB + 2];
-  int64_t i0, nv;
+  int64_t ie[2];
 };
 
 struct SmallLds {
@@ -98,12 +99,13 @@ union BLds {
 };
 
 __device__ void zero_next_next(const BArgs& a, int level) {
-  // F[level + 2] is written by atomics if level + 1 runs top-down.
+  // F[level + 2] is written by atomics if level + 1 runs top-down.  Its
+  // words from bend on stay zero (cleared when the storage was allocated).
   if (level + 2 >= a.level_cap) return;
   uint64_t* z = a.levels + (level + 2) * a.nloc;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t i = gtid; i < a.nloc; i += gsize) z[i] = 0;
+  for (int64_t i = gtid; i < a.bend; i += gsize) z[i] = 0;
 }
 
 __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
@@ -117,25 +119,21 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
   const int tid = threadIdx.x;
   for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
     const int64_t e1 = min(T, e0 + kEPB);
-    if (tid == 0) {
-      int64_t lo = 0, hi = F;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (qoff[mid] <= e0) lo = mid; else hi = mid;
+    // The list entries the chunk spans: a list that fits the staging is
+    // staged whole, otherwise waves 0 and 1 search both ends.
+    if (F <= kEPB + 2) {
+      if (tid == 0) {
+        L.ie[0] = 0;
+        L.ie[1] = F - 1;
       }
-      const int64_t i0 = lo;
-      hi = F;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (qoff[mid] <= e1 - 1) lo = mid; else hi = mid;
-      }
-      L.i0 = i0;
-      L.nv = lo - i0 + 1;
+    } else if (tid < 128) {
+      const int64_t r = wave_last_le(qoff, 0, F, tid < 64 ? e0 : e1 - 1);
+      if ((tid & 63) == 0) L.ie[tid >> 6] = r;
     }
     __syncthreads();
-    const int64_t i0 = L.i0;
+    const int64_t i0 = L.ie[0];
     // Listed vertices have out-degree >= 1: a chunk spans <= kEPB + 1 of them.
-    int nv = static_cast<int>(L.nv);
+    int nv = static_cast<int>(L.ie[1] - i0 + 1);
     if (nv > kEPB + 2) {
       if (tid == 0) c->error = 1;
       nv = kEPB + 2;
@@ -260,7 +258,8 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
   const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
   int64_t scanned = 0, cnt = 0, degs = 0;
-  for (int64_t u = grp; u < a.nloc; u += ngrp) {
+  // Rows from bend on have no in-edges; their F[L + 1] words stay zero.
+  for (int64_t u = grp; u < a.bend; u += ngrp) {
     const uint64_t s = a.seen[u] | Fl[u];
     const uint64_t need = a.lanes & ~s;
     const int64_t b = in_off[u], e = in_off[u + 1];
@@ -302,9 +301,10 @@ __device__ void count_or_write(const BArgs& a, BCtrl* c, SmallLds& L, int level)
   // Workgroup b owns the contiguous range [v0, v1) and walks it in tiles
   // of kThreads consecutive vertices (coalesced); WRITE orders entries by
   // a block scan per tile on top of the workgroup's prefix from k_bpost.
+  // Levels >= 1 have no lanes from bend on.
   const uint64_t* Fl = a.levels + level * a.nloc;
-  const int64_t vpb = (a.nloc + gridDim.x - 1) / gridDim.x;
-  const int64_t v0 = min(a.nloc, int64_t(blockIdx.x) * vpb), v1 = min(a.nloc, v0 + vpb);
+  const int64_t vpb = (a.bend + gridDim.x - 1) / gridDim.x;
+  const int64_t v0 = min(a.bend, int64_t(blockIdx.x) * vpb), v1 = min(a.bend, v0 + vpb);
   if (!kWrite) {
     int64_t cnt = 0, degs = 0;
     for (int64_t u = v0 + threadIdx.x; u < v1; u += kThreads) {
@@ -476,24 +476,41 @@ __global__ __launch_bounds__(kThreads) void k_binit(BArgs a, const int32_t* srcs
 }
 
 __global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
-  // One thread: set the source lanes and build the level-0 list.
+  // One wavefront, lane s = source s: lanes with the same local vertex
+  // merge into its level-0 word; the first of them lists the vertex (if it
+  // has out-edges), in source order, with the out-degree prefix.
   BCtrl* c = a.ctrl;
-  if (threadIdx.x != 0) return;
-  int cnt = 0;
-  long long degs = 0;
-  for (int s = 0; s < nsrc; s++) {
-    const int64_t lu = int64_t(srcs[s]) - a.vbegin;
-    if (lu < 0 || lu >= a.nloc) continue;
-    const uint64_t old = a.levels[lu];
-    a.levels[lu] = old | (1ull << s);
-    // A source without out-edges reaches only itself: no list entry.
-    if (old == 0ull && a.out_off[lu + 1] > a.out_off[lu]) {
-      a.act0[cnt] = srcs[s];
-      a.aoff0[cnt] = degs;
-      degs += a.out_off[lu + 1] - a.out_off[lu];
-      cnt++;
+  const int s = threadIdx.x;
+  const int32_t src = s < nsrc ? srcs[s] : -1;
+  const int64_t lu = int64_t(src) - a.vbegin;
+  const int local = s < nsrc && lu >= 0 && lu < a.nloc ? 1 : 0;
+  uint64_t same = 0;
+  bool first = local != 0;
+  for (int t = 0; t < 64; t++) {
+    const int32_t o = __shfl(src, t);
+    const int ol = __shfl(local, t);
+    if (local && ol && o == src) {
+      same |= 1ull << t;
+      if (t < s) first = false;
     }
   }
+  int64_t d = 0;
+  if (first) {
+    a.levels[lu] = same;  // k_binit cleared level 0
+    d = a.out_off[lu + 1] - a.out_off[lu];
+  }
+  const bool listed = first && d > 0;
+  if (!listed) d = 0;
+  const unsigned long long m = __ballot(listed);
+  const int64_t ex = wave_incl_scan(d) - d;
+  if (listed) {
+    const int rank = __popcll(m & ((1ull << s) - 1));
+    a.act0[rank] = src;
+    a.aoff0[rank] = ex;
+  }
+  const int cnt = __popcll(m);
+  const long long degs = wave_sum(d);
+  if (s != 0) return;
   c->acount = cnt;
   c->aedges = degs;
   c->pend_cnt = cnt;
@@ -576,6 +593,7 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   a.in_nbr = ctx->d_in_nbr;
   a.nloc = ctx->nloc;
   a.vbegin = ctx->vbegin;
+  a.bend = ctx->relabeled ? ctx->scan_end : ctx->nloc;
   a.levels = ctx->d_levels;
   a.level_cap = ctx->level_cap;
   a.seen = ctx->d_seen;
@@ -697,6 +715,8 @@ int ensure_levels(dmm_ctx* ctx, int64_t cap) {
   int rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_levels), sizeof(uint64_t) * cap * ctx->nloc,
                      "batch level frontiers");
   if (rc) return rc;
+  // Steps clear only the rows below bend of the levels they fill.
+  launch_fill_u64(ctx->stream, ctx->d_levels, cap * ctx->nloc, 0);
   ctx->level_cap = cap;
   // Captured chains hold the old level pointer.
   if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);

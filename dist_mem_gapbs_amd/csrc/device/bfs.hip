@@ -127,23 +127,6 @@ union StepLds {
   SmallLds sm;
 };
 
-// Last index i in [lo, hi) with key[i] <= x, for a non-decreasing key with
-// key[lo] <= x.  One wavefront searches 64-ary: each round probes 64 evenly
-// spaced keys at once (one load per lane) and keeps the interval after the
-// last one <= x, so a window of F entries takes ceil(log64 F) dependent
-// loads instead of log2 F.
-__device__ int64_t wave_last_le(const int64_t* key, int64_t lo, int64_t hi, int64_t x) {
-  const int lane = lane_id();
-  while (hi - lo > 1) {
-    const int64_t step = (hi - lo + 63) >> 6;
-    const int64_t p = lo + lane * step;
-    const unsigned long long m = __ballot(p < hi && key[p] <= x);  // lanes 0..j, lane 0 always
-    lo += (63 - __clzll(m)) * step;
-    hi = min(hi, lo + step);
-  }
-  return lo;
-}
-
 // ------------------------------------------------------------------------
 // Top-down step: src/bfs.cc:81-100.  Edge-balanced over the window: chunk k
 // covers window edges [k*kEPB, (k+1)*kEPB); the frontier entries it spans

@@ -325,6 +325,23 @@ __device__ __forceinline__ T block_sum(T x, T* scratch) {
   return r;  // valid in thread 0
 }
 
+// Last index i in [lo, hi) with key[i] <= x, for a non-decreasing key with
+// key[lo] <= x.  One wavefront searches 64-ary: each round probes 64 evenly
+// spaced keys at once (one load per lane) and keeps the interval after the
+// last one <= x, so a window of F entries takes ceil(log64 F) dependent
+// loads instead of log2 F.
+__device__ inline int64_t wave_last_le(const int64_t* key, int64_t lo, int64_t hi, int64_t x) {
+  const int lane = lane_id();
+  while (hi - lo > 1) {
+    const int64_t step = (hi - lo + 63) >> 6;
+    const int64_t p = lo + lane * step;
+    const unsigned long long m = __ballot(p < hi && key[p] <= x);  // lanes 0..j, lane 0 always
+    lo += (63 - __clzll(m)) * step;
+    hi = min(hi, lo + step);
+  }
+  return lo;
+}
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Fill kernels (used instead of hipMemsetAsync so the sequences can be
