@@ -31,6 +31,9 @@ int bfs_finish(dmm_ctx* ctx);
 int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool with_depth,
                     dmm_bfs_stats* stats);
 
+// The head array: each row's first kHeadNbrs in-neighbours, -1 padded.
+void bfs_build_heads(dmm_ctx* ctx);
+
 // Reached vertices / edges of the current parent array (stats only).
 int bfs_reach_stats(dmm_ctx* ctx, int64_t* reached, int64_t* reached_edges);
 

@@ -47,7 +47,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
-constexpr int kBuProbe = 4;         // in-neighbours probed per round (bottom-up)
+constexpr int kBuProbe = kHeadNbrs;  // in-neighbours probed per round (bottom-up)
+static_assert(kHeadNbrs == 4, "the head array is read as int4");
 constexpr int kItems = 4;                 // TD edges per thread per chunk
 constexpr int kEPB = kThreads * kItems;   // TD edges per chunk
 
@@ -65,6 +66,7 @@ struct StepArgs {
   uint64_t* bm0;
   uint64_t* bm1;
   uint64_t* bm2;
+  const int32_t* head;                    // [nloc][kHeadNbrs] first in-neighbours
   int64_t* part;                          // [gridDim.x][kPartWidth]
   int64_t part_half;                      // offset of the device loop's second partial buffer
   BfsCtrl* ctrl;
@@ -292,32 +294,58 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
     if (lu < a.nloc) {
       if (parent[lu] < 0) {
         scanned++;
-        const int64_t s = in_off[lu], e = in_off[lu + 1];
         // kBuProbe in-neighbours per round: their IDs, then their frontier
         // bits, all loads in flight at once; the first hit in row order is
-        // the parent (the sequential scan's choice and edge count).
+        // the parent (the sequential scan's choice and edge count).  The
+        // first round reads the head array, where the rows of a wave's 64
+        // vertices lie side by side (one coalesced 1 KB load); the CSR row
+        // is read only past it.
+        const int4 hd = reinterpret_cast<const int4*>(a.head)[lu];
+        const int32_t h0[kBuProbe] = {hd.x, hd.y, hd.z, hd.w};
+        bool hh[kBuProbe];
+#pragma unroll
+        for (int j = 0; j < kBuProbe; j++) hh[j] = h0[j] >= 0 && ((front[h0[j] >> 6] >> (h0[j] & 63)) & 1ull);
+        int hj = kBuProbe;
         int32_t pv = 0;
-        int64_t hit = e;
-        for (int64_t k = s; k < e && hit == e; k += kBuProbe) {
-          int32_t v[kBuProbe];
 #pragma unroll
-          for (int j = 0; j < kBuProbe; j++) v[j] = k + j < e ? in_nbr[k + j] : -1;
-          bool h[kBuProbe];
+        for (int j = kBuProbe - 1; j >= 0; j--)
+          if (hh[j]) {
+            hj = j;
+            pv = h0[j];
+          }
+        int64_t ex;  // edges examined
+        if (hj < kBuProbe) {
+          found = true;
+          ex = hj + 1;
+        } else if (h0[kBuProbe - 1] < 0) {
+          ex = 0;  // the whole row (shorter than the head) missed
 #pragma unroll
-          for (int j = 0; j < kBuProbe; j++) h[j] = v[j] >= 0 && ((front[v[j] >> 6] >> (v[j] & 63)) & 1ull);
+          for (int j = 0; j < kBuProbe; j++) ex += h0[j] >= 0 ? 1 : 0;
+        } else {
+          const int64_t s = in_off[lu], e = in_off[lu + 1];
+          int64_t hit = e;
+          for (int64_t k = s + kBuProbe; k < e && hit == e; k += kBuProbe) {
+            int32_t v[kBuProbe];
 #pragma unroll
-          for (int j = kBuProbe - 1; j >= 0; j--)
-            if (h[j]) {
-              hit = k + j;
-              pv = v[j];
-            }
+            for (int j = 0; j < kBuProbe; j++) v[j] = k + j < e ? in_nbr[k + j] : -1;
+            bool h[kBuProbe];
+#pragma unroll
+            for (int j = 0; j < kBuProbe; j++) h[j] = v[j] >= 0 && ((front[v[j] >> 6] >> (v[j] & 63)) & 1ull);
+#pragma unroll
+            for (int j = kBuProbe - 1; j >= 0; j--)
+              if (h[j]) {
+                hit = k + j;
+                pv = v[j];
+              }
+          }
+          found = hit < e;
+          ex = found ? hit - s + 1 : e - s;
         }
-        found = hit < e;
         if (found) {
           parent[lu] = pv;
           if (a.depth) a.depth[lu] = level + 1;
         }
-        edges += found ? hit - s + 1 : e - s;
+        edges += ex;
       }
     }
     const unsigned long long m = __ballot(found);
@@ -937,6 +965,21 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   for (int64_t i = gtid; i < n; i += gsize) p[i] = v;
 }
 
+// head[r] = the first kHeadNbrs in-neighbours of row r, -1 past its end.
+__global__ __launch_bounds__(kThreads) void k_build_heads(const int64_t* __restrict__ in_off,
+                                                         const int32_t* __restrict__ in_nbr, int64_t n,
+                                                         int4* __restrict__ head) {
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t r = gtid; r < n; r += gsize) {
+    const int64_t s = in_off[r], e = in_off[r + 1];
+    int32_t h[kHeadNbrs];
+#pragma unroll
+    for (int j = 0; j < kHeadNbrs; j++) h[j] = s + j < e ? in_nbr[s + j] : -1;
+    head[r] = make_int4(h[0], h[1], h[2], h[3]);
+  }
+}
+
 StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   StepArgs a;
   a.out_off = ctx->d_out_off;
@@ -956,6 +999,7 @@ StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   a.bm0 = ctx->d_bm[0];
   a.bm1 = ctx->d_bm[1];
   a.bm2 = ctx->d_bm[2];
+  a.head = ctx->d_head;
   a.part = ctx->d_part;
   a.part_half = (ctx->part_cap / 2) * kPartWidth;
   a.ctrl = ctx->d_ctrl;
@@ -1127,6 +1171,12 @@ int bfs_finish(dmm_ctx* ctx) {
   DMM_HIP_CHECK(ctx, hipGetLastError());
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DMM_OK;
+}
+
+void bfs_build_heads(dmm_ctx* ctx) {
+  const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(65536, ceil_div(ctx->nloc, kThreads))));
+  hipLaunchKernelGGL(k_build_heads, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_in_off, ctx->d_in_nbr,
+                     ctx->nloc, reinterpret_cast<int4*>(ctx->d_head));
 }
 
 int bfs_reach_stats(dmm_ctx* ctx, int64_t* reached, int64_t* reached_edges) {

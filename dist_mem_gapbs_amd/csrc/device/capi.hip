@@ -86,7 +86,7 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
                   ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
-                  ctx->d_piece_part, ctx->d_bsegs};
+                  ctx->d_piece_part, ctx->d_bsegs, ctx->d_head};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -97,6 +97,7 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_parent = ctx->d_depth = ctx->d_queue = nullptr;
   ctx->d_qoff = nullptr;
   ctx->d_bm[0] = ctx->d_bm[1] = ctx->d_bm[2] = nullptr;
+  ctx->d_head = nullptr;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
   ctx->d_heavy = nullptr;
@@ -459,8 +460,11 @@ int dmm_graph_upload(dmm_ctx* ctx, int64_t num_nodes, int64_t vbegin, int64_t ve
       (rc = A((void**)&ctx->d_qoff, sizeof(int64_t) * (rows + 64), "queue offsets")) ||
       (rc = A((void**)&ctx->d_bm[0], sizeof(uint64_t) * ctx->words, "bitmap")) ||
       (rc = A((void**)&ctx->d_bm[1], sizeof(uint64_t) * ctx->words, "bitmap")) ||
-      (rc = A((void**)&ctx->d_bm[2], sizeof(uint64_t) * ctx->words, "bitmap")))
+      (rc = A((void**)&ctx->d_bm[2], sizeof(uint64_t) * ctx->words, "bitmap")) ||
+      (rc = A((void**)&ctx->d_head, sizeof(int32_t) * dmm::kHeadNbrs * rows, "in-neighbour heads")))
     return rc;
+  dmm::bfs_build_heads(ctx);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
   // PageRank heavy-vertex list: rows with in-degree >= threshold get a whole
   // wavefront each.
   const int64_t* ioff = directed ? up_in_off : up_out_off;

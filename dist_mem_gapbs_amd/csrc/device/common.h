@@ -22,6 +22,7 @@ constexpr int kCountBits = 30;
 constexpr int kMaxGridPerCU = 32;   // upper bound of workgroups per CU
 constexpr int kPartWidth = 4;       // int64 partials per workgroup
 constexpr int kRecentSearches = 32; // searches sizing the device-loop chain
+constexpr int kHeadNbrs = 4;        // in-neighbours per row in the head array
 
 // Loop state of a BFS on the device.  The head (BfsHead) is what the post
 // kernel reads and rewrites after every step; the trace arrays follow it.
@@ -145,6 +146,7 @@ struct dmm_ctx {
   int32_t* d_queue = nullptr;    // nloc + 1 (sliding queue)
   int64_t* d_qoff = nullptr;     // nloc + 1 (window-relative degree prefix)
   uint64_t* d_bm[3] = {nullptr, nullptr, nullptr};  // words each
+  int32_t* d_head = nullptr;  // first kHeadNbrs in-neighbours of each row, -1 padded
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed
   void* h_params = nullptr;      // pinned staging for d_params
