@@ -30,7 +30,7 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 // edges that a tile kernel sums from an LDS copy of that window; shorter
 // runs stay in the row's list and are gathered from HBM.
 constexpr int kTileWindow = 32768;
-constexpr int kTileMinRun = 4;
+constexpr int kTileMinRun = 2;
 constexpr int kTilePiece = 64;
 // A window is tiled only if its long runs hold at least this many edges
 // (the 128 KB LDS copy is then amortized); otherwise they stay in the lists.
