@@ -53,7 +53,7 @@ struct BCtrl {
   int nlevels;
   int sweeps;
   int error;
-  int nz_ok;  // nz describes F[level] (the last step was a COUNT)
+  int pad;
   int level_kind[DMM_MAX_STEPS];
   long long level_active[DMM_MAX_STEPS];
 };
@@ -66,7 +66,6 @@ struct BArgs {
   int64_t nloc, vbegin;
   int64_t bend;       // rows [bend, nloc) have no in-edges: no lanes after level 0
   uint64_t* levels;   // level_cap * nloc
-  uint64_t* nz;       // bit u = F[level][u] != 0, written by COUNT; zero from bend on
   int64_t level_cap;
   uint64_t* seen;
   int32_t* act0;
@@ -194,10 +193,10 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
 // early), later rounds keep kSweepUnroll frontier-word loads in flight per
 // lane; the group ORs them and the scan stops once every needed lane is
 // found.  nz (may be NULL) is a
-// non-zero bitmap of the frontier words (bit v), with an optional summary
-// bitmap nz2 (bit w set iff nz word w != 0, so 256 KB for 2^27 vertices,
-// resident in L2); both are probed before a word load, so sparse frontiers
-// cost one L2 probe per edge.
+// non-zero bitmap of the frontier words (bit v), with a summary bitmap nz2
+// (bit w set iff nz word w != 0, so 256 KB for 2^27 vertices, resident in
+// L2); both are probed before a word load, so sparse frontiers cost one L2
+// probe per edge.
 // Returns the end of the examined range; *found = the needed lanes found.
 constexpr int kSweepUnroll = 4;
 __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr, int64_t b, int64_t e,
@@ -210,7 +209,7 @@ __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr,
   {
     const int64_t k = k0 + sub;
     int32_t v = k < e ? in_nbr[k] : -1;
-    if (nz2 && v >= 0 && !((nz2[v >> 12] >> ((v >> 6) & 63)) & 1ull)) v = -1;
+    if (nz && v >= 0 && !((nz2[v >> 12] >> ((v >> 6) & 63)) & 1ull)) v = -1;
     if (nz && v >= 0 && !((nz[v >> 6] >> (v & 63)) & 1ull)) v = -1;
     if (v >= 0) acc |= front[v];
 #pragma unroll
@@ -228,12 +227,10 @@ __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr,
       const int64_t k = k0 + j * kGroup + sub;
       v[j] = k < e ? in_nbr[k] : -1;
     }
-    if (nz2) {
+    if (nz) {
 #pragma unroll
       for (int j = 0; j < kSweepUnroll; j++)
         if (v[j] >= 0 && !((nz2[v[j] >> 12] >> ((v[j] >> 6) & 63)) & 1ull)) v[j] = -1;
-    }
-    if (nz) {
 #pragma unroll
       for (int j = 0; j < kSweepUnroll; j++)
         if (v[j] >= 0 && !((nz[v[j] >> 6] >> (v[j] & 63)) & 1ull)) v[j] = -1;
@@ -260,9 +257,6 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
   const int sub = threadIdx.x & (kGroup - 1);
   const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
-  // A frontier on under half the rows: probe its non-zero bitmap (resident
-  // in L2) before each frontier-word load.
-  const uint64_t* nzl = c->nz_ok && c->acount * 2 < a.bend ? a.nz : nullptr;
   int64_t scanned = 0, cnt = 0, degs = 0;
   // Rows from bend on have no in-edges; their F[L + 1] words stay zero.
   for (int64_t u = grp; u < a.bend; u += ngrp) {
@@ -271,7 +265,7 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
     const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
     if (need && e > b) {
-      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, Fl, nzl, nullptr, &nxt);
+      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, Fl, nullptr, nullptr, &nxt);
       scanned += stop - b;
     }
     if (sub == 0) {
@@ -307,20 +301,14 @@ __device__ void count_or_write(const BArgs& a, BCtrl* c, SmallLds& L, int level)
   // Workgroup b owns the contiguous range [v0, v1) and walks it in tiles
   // of kThreads consecutive vertices (coalesced); WRITE orders entries by
   // a block scan per tile on top of the workgroup's prefix from k_bpost.
-  // Levels >= 1 have no lanes from bend on.  Ranges start on multiples
-  // of 64, so COUNT also writes the non-zero bitmap of F[level] word by
-  // word (one ballot per wave and tile).
+  // Levels >= 1 have no lanes from bend on.
   const uint64_t* Fl = a.levels + level * a.nloc;
-  const int64_t vpb = ((a.bend + gridDim.x - 1) / gridDim.x + 63) & ~int64_t(63);
+  const int64_t vpb = (a.bend + gridDim.x - 1) / gridDim.x;
   const int64_t v0 = min(a.bend, int64_t(blockIdx.x) * vpb), v1 = min(a.bend, v0 + vpb);
   if (!kWrite) {
     int64_t cnt = 0, degs = 0;
-    for (int64_t t = v0; t < v1; t += kThreads) {
-      const int64_t u = t + threadIdx.x;
-      const bool on = u < v1 && Fl[u] != 0;
-      const unsigned long long m = __ballot(on);
-      if ((threadIdx.x & 63) == 0 && u < v1) a.nz[u >> 6] = m;
-      if (on) {
+    for (int64_t u = v0 + threadIdx.x; u < v1; u += kThreads) {
+      if (Fl[u]) {
         const int64_t d = a.out_off[u + 1] - a.out_off[u];
         if (d > 0) {
           cnt++;
@@ -462,7 +450,6 @@ __global__ __launch_bounds__(kPostThreads) void k_bpost(BArgs a, int nblocks) {
     choose_next(a, c, s1, s2, false);
     return;
   }
-  c->nz_ok = phase == kBCount ? 1 : 0;
   if (phase == kBCount) {
     c->acount = s0;
     c->aedges = s1;
@@ -534,7 +521,6 @@ __global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
   c->nlevels = 1;
   c->sweeps = 0;
   c->error = 0;
-  c->nz_ok = 0;
   c->sweep_scanned_total = 0;
   if (cnt == 0) {
     c->phase = kBDone;
@@ -609,7 +595,6 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   a.vbegin = ctx->vbegin;
   a.bend = ctx->relabeled ? ctx->scan_end : ctx->nloc;
   a.levels = ctx->d_levels;
-  a.nz = ctx->d_bnz;
   a.level_cap = ctx->level_cap;
   a.seen = ctx->d_seen;
   a.act0 = ctx->d_bact;
@@ -732,12 +717,6 @@ int ensure_levels(dmm_ctx* ctx, int64_t cap) {
   if (rc) return rc;
   // Steps clear only the rows below bend of the levels they fill.
   launch_fill_u64(ctx->stream, ctx->d_levels, cap * ctx->nloc, 0);
-  if (!ctx->d_bnz) {
-    const int64_t nzw = (ctx->nloc + 63) / 64;
-    if ((rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bnz), sizeof(uint64_t) * nzw, "batch non-zero bitmap")))
-      return rc;
-    launch_fill_u64(ctx->stream, ctx->d_bnz, nzw, 0);
-  }
   ctx->level_cap = cap;
   // Captured chains hold the old level pointer.
   if (ctx->batch_graph) (void)hipGraphExecDestroy(ctx->batch_graph);

@@ -217,7 +217,6 @@ struct dmm_ctx {
   // Batched BFS state.
   uint64_t* d_seen = nullptr;
   uint64_t* d_levels = nullptr;  // level_cap x nloc frontier words
-  uint64_t* d_bnz = nullptr;     // non-zero bitmap of the counted level (batched BFS)
   int64_t level_cap = 0;
   int32_t* d_bact = nullptr;     // 2 x (nloc + 64) active lists
   int64_t* d_baoff = nullptr;    // 2 x (nloc + 64) degree prefixes
