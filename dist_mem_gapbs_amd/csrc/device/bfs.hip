@@ -377,40 +377,48 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
   }
 }
 
-// Word range of workgroup b in the COUNT / WRITE phases: contiguous, in
-// ascending order, so a scan over workgroups orders the whole bitmap.
-__device__ __forceinline__ void block_words(const StepArgs& a, int64_t* w0, int64_t* w1, int64_t* wpt) {
+// COUNT / WRITE: workgroup b takes the contiguous bitmap words [w0, w1)
+// (ascending, so a scan over workgroups orders the whole bitmap), and each
+// thread an equal contiguous span of their bits, so a dense word is split
+// between threads instead of walked bit by bit by one.
+template <typename F>
+__device__ __forceinline__ void for_block_bits(const StepArgs& a, const uint64_t* front, F&& f) {
   const int64_t nw = a.wend - a.wbegin;
   const int64_t wpb = (nw + gridDim.x - 1) / gridDim.x;
-  *w0 = min(nw, int64_t(blockIdx.x) * wpb);
-  *w1 = min(nw, *w0 + wpb);
-  *wpt = (wpb + kThreads - 1) / kThreads;
-}
-
-template <bool kWrite>
-__device__ void count_or_write(const StepArgs& a, const StepIo& io, SmallLds& L) {
-  const uint64_t* front = bm_sel(a, role_front(io.front));
-  int64_t w0, w1, wpt;
-  block_words(a, &w0, &w1, &wpt);
-  const int64_t t0 = min(w1, w0 + threadIdx.x * wpt), t1 = min(w1, t0 + wpt);
-  int64_t cnt = 0, degs = 0, zero = 0;
-  for (int64_t wi = t0; wi < t1; wi++) {
-    const int64_t w = a.wbegin + wi;
-    uint64_t x = front[w];
+  const int64_t w0 = min(nw, int64_t(blockIdx.x) * wpb), w1 = min(nw, w0 + wpb);
+  const int64_t nbits = (w1 - w0) * 64;
+  const int64_t bpt = (nbits + kThreads - 1) / kThreads;
+  const int64_t b0 = min(nbits, int64_t(threadIdx.x) * bpt), b1 = min(nbits, b0 + bpt);
+  for (int64_t bb = b0; bb < b1;) {
+    const int64_t wl = bb >> 6;
+    const int64_t end = min(b1, (wl + 1) << 6);
+    const int lo = static_cast<int>(bb & 63), hi = static_cast<int>(end - (wl << 6));
+    const int64_t w = a.wbegin + w0 + wl;
+    uint64_t x = front[w] & (~0ull << lo);
+    if (hi < 64) x &= (1ull << hi) - 1;
     while (x) {
       const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
       x &= x - 1;
       const int64_t lv = w * 64 + b - a.vbegin;
       if (lv < 0 || lv >= a.nloc) continue;
-      const int64_t d = a.out_off[lv + 1] - a.out_off[lv];
-      if (d > 0) {
-        cnt++;
-        degs += d;
-      } else {
-        zero++;
-      }
+      f(w * 64 + b, a.out_off[lv + 1] - a.out_off[lv]);
     }
+    bb = end;
   }
+}
+
+template <bool kWrite>
+__device__ void count_or_write(const StepArgs& a, const StepIo& io, SmallLds& L) {
+  const uint64_t* front = bm_sel(a, role_front(io.front));
+  int64_t cnt = 0, degs = 0, zero = 0;
+  for_block_bits(a, front, [&](int64_t, int64_t d) {
+    if (d > 0) {
+      cnt++;
+      degs += d;
+    } else {
+      zero++;
+    }
+  });
   if (!kWrite) {
     const int64_t sc = block_sum<kThreads>(cnt, L.red[0]);
     const int64_t sd = block_sum<kThreads>(degs, L.red[1]);
@@ -430,23 +438,14 @@ __device__ void count_or_write(const StepArgs& a, const StepIo& io, SmallLds& L)
   int64_t run = io.base_d + block_excl_scan<kThreads>(degs, L.scan_d, &td);
   int32_t* oq = a.queue + io.qin;
   int64_t* oqoff = a.qoff + io.qin;
-  for (int64_t wi = t0; wi < t1; wi++) {
-    const int64_t w = a.wbegin + wi;
-    uint64_t x = front[w];
-    while (x) {
-      const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
-      x &= x - 1;
-      const int64_t lv = w * 64 + b - a.vbegin;
-      if (lv < 0 || lv >= a.nloc) continue;
-      const int64_t d = a.out_off[lv + 1] - a.out_off[lv];
-      if (d > 0) {
-        oq[pos] = static_cast<int32_t>(w * 64 + b);
-        oqoff[pos] = run;
-        pos++;
-        run += d;
-      }
+  for_block_bits(a, front, [&](int64_t v, int64_t d) {
+    if (d > 0) {
+      oq[pos] = static_cast<int32_t>(v);
+      oqoff[pos] = run;
+      pos++;
+      run += d;
     }
-  }
+  });
 }
 
 // Appends one step record: the counters go to the head `h` (the post
@@ -908,14 +907,16 @@ __global__ void k_slide_to_empty(BfsCtrl* c) {
 
 // Cleanup of gapbs-verify/bfs.cc:168-171 (DMM: at write-out, src/bfs.cc:
 // 224-228).
-__global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, int only_if_done) {
-  if (only_if_done && a.ctrl->done == 0) return;
+// done (may be NULL): the device loop's flag; nothing until it is set,
+// then 2 marks the cleanup done.
+__global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, int* done) {
+  if (done && *done == 0) return;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
   for (int64_t v = gtid; v < a.nloc; v += gsize) {
     if (a.parent[v] < -1) a.parent[v] = -1;
   }
-  if (only_if_done && gtid == 0) a.ctrl->done = 2;
+  if (done && gtid == 0) *done = 2;
 }
 
 // Reached vertices and the out-degree sum over them (Graph500 edge count x2).
@@ -1015,9 +1016,12 @@ int grid_blocks(dmm_ctx* ctx) {
   return static_cast<int>(std::min<int64_t>(g, ctx->part_cap / 2));  // the device loop double-buffers
 }
 
-int read_ctrl_head(dmm_ctx* ctx) {
+// The loop-state head (the second one after a device loop that ended on an
+// odd slot count) into the host mirror's first head.
+int read_ctrl_head(dmm_ctx* ctx, int parity = 0) {
   const size_t head = sizeof(BfsHead);
-  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_ctrl, ctx->d_ctrl, head, hipMemcpyDeviceToHost, ctx->stream));
+  const BfsHead* src = parity ? &ctx->d_ctrl->alt : static_cast<const BfsHead*>(ctx->d_ctrl);
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_ctrl, src, head, hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->h_ctrl->error)
     return set_err(ctx, DMM_ERR_STATE, "queue window holds a vertex of out-degree 0 (invariant violated)");
@@ -1087,6 +1091,7 @@ int bfs_init(dmm_ctx* ctx, int32_t source) {
   if (rc) return rc;
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
   hipLaunchKernelGGL(k_init, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, prm, 0);
+  ctx->bfs_parity = 0;
   DMM_HIP_CHECK(ctx, hipGetLastError());
   ctx->bfs_valid = true;
   return read_ctrl_head(ctx);
@@ -1165,8 +1170,8 @@ int bfs_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size) {
 
 int bfs_finish(dmm_ctx* ctx) {
   StepArgs a = make_args(ctx);
-  hipLaunchKernelGGL(k_finish, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, 0);
-  if (ctx->relabeled) launch_map_parent(ctx, true, nullptr);
+  hipLaunchKernelGGL(k_finish, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, nullptr);
+  if (ctx->relabeled) launch_map_parent(ctx, true, nullptr, &ctx->d_ctrl->scan_end);
   ctx->depth_valid = true;
   DMM_HIP_CHECK(ctx, hipGetLastError());
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1195,7 +1200,10 @@ int bfs_reach_stats(dmm_ctx* ctx, int64_t* reached, int64_t* reached_edges) {
 
 // ---- device-resident loop ------------------------------------------------
 
-static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth, hipGraphExec_t* out) {
+// nsteps slots from head `parity` on; the cleanup reads the head the last
+// slot writes.
+static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth, int parity,
+                       hipGraphExec_t* out) {
   StepArgs a = make_args(ctx, with_depth);
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
   hipGraph_t g = nullptr;
@@ -1205,13 +1213,14 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
     hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, 1);
   }
   for (int i = 0; i < nsteps; i++)
-    hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, i & 1);
+    hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, (parity + i) & 1);
   // Results back to the caller's IDs, unreached as -1 (a no-op until the
   // loop is done; either kernel marks the cleanup done).
+  BfsHead* last = ((parity + nsteps) & 1) ? &ctx->d_ctrl->alt : static_cast<BfsHead*>(ctx->d_ctrl);
   if (ctx->relabeled)
-    launch_map_parent(ctx, with_depth, &ctx->d_ctrl->done);
+    launch_map_parent(ctx, with_depth, &last->done, &last->scan_end);
   else
-    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, 1);
+    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, &last->done);
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -1222,20 +1231,21 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
 
 int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool with_depth,
                     dmm_bfs_stats* stats) {
-  // Slot counts are even, so every graph ends with the latest loop state
-  // in the first head (and a continuation starts from it).
+  // The first graph starts on head 0 and ends on head (want & 1); the
+  // continuation (an even slot count) starts and ends on that head.
   constexpr int kCont = 6;
   const int d = with_depth ? 1 : 0;
-  const int want = (std::max(4, ctx->bfs_chain_len) + 1) & ~1;
+  const int want = std::max(4, ctx->bfs_chain_len);
+  const int par = want & 1;
   if (!ctx->bfs_graph[d] || ctx->bfs_graph_steps[d] != want) {
     if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
     ctx->bfs_graph[d] = nullptr;
-    int rc = build_chain(ctx, want, true, with_depth, &ctx->bfs_graph[d]);
+    int rc = build_chain(ctx, want, true, with_depth, 0, &ctx->bfs_graph[d]);
     if (rc) return rc;
     ctx->bfs_graph_steps[d] = want;
   }
-  if (!ctx->bfs_cont[d]) {
-    int rc = build_chain(ctx, kCont, false, with_depth, &ctx->bfs_cont[d]);
+  if (!ctx->bfs_cont[d][par]) {
+    int rc = build_chain(ctx, kCont, false, with_depth, par, &ctx->bfs_cont[d][par]);
     if (rc) return rc;
   }
   int rc = write_params(ctx, source, alpha, beta);
@@ -1243,14 +1253,15 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
   DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_graph[d], ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
-  rc = read_ctrl_head(ctx);
+  ctx->bfs_parity = par;
+  rc = read_ctrl_head(ctx, par);
   if (rc) return rc;
   int rounds = 0;
   while (ctx->h_ctrl->done != 2) {
     if (++rounds > (1 << 20)) return set_err(ctx, DMM_ERR_STATE, "device BFS loop did not terminate");
-    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_cont[d], ctx->stream));
+    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_cont[d][par], ctx->stream));
     DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
-    rc = read_ctrl_head(ctx);
+    rc = read_ctrl_head(ctx, par);
     if (rc) return rc;
   }
   ctx->bfs_valid = true;

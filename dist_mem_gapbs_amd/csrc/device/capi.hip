@@ -62,8 +62,11 @@ double now_ms() {
 void drop_graphs(dmm_ctx* ctx) {
   for (int d = 0; d < 2; d++) {
     if (ctx->bfs_graph[d]) (void)hipGraphExecDestroy(ctx->bfs_graph[d]);
-    if (ctx->bfs_cont[d]) (void)hipGraphExecDestroy(ctx->bfs_cont[d]);
-    ctx->bfs_graph[d] = ctx->bfs_cont[d] = nullptr;
+    for (int p = 0; p < 2; p++) {
+      if (ctx->bfs_cont[d][p]) (void)hipGraphExecDestroy(ctx->bfs_cont[d][p]);
+      ctx->bfs_cont[d][p] = nullptr;
+    }
+    ctx->bfs_graph[d] = nullptr;
     ctx->bfs_graph_steps[d] = 0;
   }
   if (ctx->pr_graph) (void)hipGraphExecDestroy(ctx->pr_graph);
@@ -143,15 +146,28 @@ int need_graph(dmm_ctx* ctx) {
 }
 
 // The single-step API exposes the device state as is: natural layout only.
+// It works on the first loop-state head, where a device loop that ended on
+// the second one has its state copied first.
 int need_natural(dmm_ctx* ctx) {
   int rc = need_graph(ctx);
   if (rc) return rc;
   if (ctx->relabeled)
     return set_err(ctx, DMM_ERR_STATE, "the step API needs DMM_LAYOUT_NATURAL (dmm_ctx_set_layout before upload)");
+  if (ctx->bfs_parity) {
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_ctrl, &ctx->d_ctrl->alt, sizeof(dmm::BfsHead), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+    ctx->bfs_parity = 0;
+  }
   return DMM_OK;
 }
 
 int32_t to_internal(const dmm_ctx* ctx, int32_t v) { return ctx->relabeled ? ctx->h_perm[v] : v; }
+
+// The loop-state head holding the latest state (a device loop that ended on
+// an odd slot count leaves it in the second head).
+const dmm::BfsHead* live_head(const dmm_ctx* ctx) {
+  return ctx->bfs_parity ? &ctx->d_ctrl->alt : static_cast<const dmm::BfsHead*>(ctx->d_ctrl);
+}
 
 // Validates one CSR block: monotone offsets starting at 0 and IDs in range.
 int check_csr(dmm_ctx* ctx, int64_t rows, const int64_t* off, const int32_t* nbr, int64_t n,
@@ -222,6 +238,9 @@ int copy_trace(dmm_ctx* ctx, dmm_bfs_stats* st) {
   dmm::BfsCtrl* h = ctx->h_ctrl;
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_ctrl, sizeof(dmm::BfsCtrl), hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  // A device loop that ended on an odd slot count left its state in the
+  // second head.
+  if (ctx->bfs_parity) static_cast<dmm::BfsHead&>(*h) = h->alt;
   const int ns = std::min(h->step, DMM_MAX_STEPS);
   st->num_steps = ns;
   st->num_levels = 0;
@@ -709,7 +728,7 @@ int dmm_state_get_queue(dmm_ctx* ctx, int32_t* queue, int64_t capacity, int64_t*
   int rc = need_natural(ctx);
   if (rc) return rc;
   if (!ctx->bfs_valid) return set_err(ctx, DMM_ERR_STATE, "dmm_step_init first");
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, live_head(ctx), sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int64_t qb = ctx->h_ctrl->qb, qe = ctx->h_ctrl->qe;
   if (size) *size = qe - qb;
   if (qe - qb > capacity) return set_err(ctx, DMM_ERR_CAPACITY, "queue window of %lld > capacity", (long long)(qe - qb));
@@ -774,7 +793,7 @@ int dmm_state_get_bitmap(dmm_ctx* ctx, int32_t which, uint64_t* words) {
   int rc = need_natural(ctx);
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, live_head(ctx), sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
   const int idx = which == 0 ? f : t;
   DMM_HIP_CHECK(ctx, hipMemcpy(words, ctx->d_bm[idx], sizeof(uint64_t) * ctx->words, hipMemcpyDeviceToHost));
@@ -786,7 +805,7 @@ int dmm_state_set_bitmap(dmm_ctx* ctx, int32_t which, const uint64_t* words) {
   if (rc) return rc;
   if (!words || which < 0 || which > 1) return set_err(ctx, DMM_ERR_INVALID, "bad bitmap request");
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
+  DMM_HIP_CHECK(ctx, hipMemcpy(ctx->h_ctrl, live_head(ctx), sizeof(dmm::BfsHead), hipMemcpyDeviceToHost));
   const int f = ctx->h_ctrl->front & 3, t = (ctx->h_ctrl->front >> 4) & 3;
   const int idx = which == 0 ? f : t;
   DMM_HIP_CHECK(ctx, hipMemcpy(ctx->d_bm[idx], words, sizeof(uint64_t) * ctx->words, hipMemcpyHostToDevice));

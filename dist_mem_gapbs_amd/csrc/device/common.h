@@ -161,7 +161,8 @@ struct dmm_ctx {
   bool bfs_valid = false;
   // Captured device loops, [0] without and [1] with depth tracking.
   hipGraphExec_t bfs_graph[2] = {nullptr, nullptr};  // init + steps + finish
-  hipGraphExec_t bfs_cont[2] = {nullptr, nullptr};   // more steps + finish
+  hipGraphExec_t bfs_cont[2][2] = {};  // more steps + finish, [depth][parity of the first slot]
+  int bfs_parity = 0;  // head holding the latest loop state after the last device loop
   int bfs_graph_steps[2] = {0, 0};
   int bfs_chain_len = 16;
   int bfs_recent[dmm::kRecentSearches] = {};  // pairs used by recent searches

@@ -190,10 +190,10 @@ int compute_scan_end(dmm_ctx* ctx, const int64_t* in_off) {
   return DMM_OK;
 }
 
-void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag) {
+void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag, const long long* scan_end) {
   hipLaunchKernelGGL(k_map_parent, dim3(grid_for(ctx, ctx->nloc)), dim3(kThreads), 0, ctx->stream, ctx->d_parent,
                      with_depth ? ctx->d_depth : nullptr, ctx->d_perm, ctx->d_inv, ctx->nloc,
-                     &ctx->d_ctrl->scan_end, ctx->d_parent_out, ctx->d_depth_out, done_flag);
+                     scan_end, ctx->d_parent_out, ctx->d_depth_out, done_flag);
 }
 
 void launch_map_scores_f32(dmm_ctx* ctx) {
