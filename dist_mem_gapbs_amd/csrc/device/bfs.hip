@@ -847,7 +847,23 @@ __device__ void loop_init(const StepArgs& a, const BfsParams* prm, int decide_no
 // loop state (one thread; it touches none of the arrays above).  Parents of
 // rows from init_end on already hold their initial values (rows no search
 // can reach); depths are always written in full.
-__global__ __launch_bounds__(kThreads) void k_init(StepArgs a, const BfsParams* prm, int decide_now) {
+// prm_in: where the search's parameters are (the device copy, or mapped
+// host memory); prm_copy (may be NULL): the device copy the later kernels
+// read, written here by workgroup 0.
+__global__ __launch_bounds__(kThreads) void k_init(StepArgs a, const BfsParams* prm_in, BfsParams* prm_copy,
+                                                   int decide_now) {
+  constexpr int kWords = sizeof(BfsParams) / 8;
+  static_assert(sizeof(BfsParams) % 8 == 0 && kWords <= kThreads, "BfsParams moves as 8-byte words");
+  __shared__ union {
+    BfsParams p;
+    long long w[kWords];
+  } sp;
+  if (threadIdx.x < kWords) {
+    sp.w[threadIdx.x] = reinterpret_cast<const long long*>(prm_in)[threadIdx.x];
+    if (prm_copy && blockIdx.x == 0) reinterpret_cast<long long*>(prm_copy)[threadIdx.x] = sp.w[threadIdx.x];
+  }
+  __syncthreads();
+  const BfsParams* prm = &sp.p;
   if (blockIdx.x == 0 && threadIdx.x == 0) loop_init(a, prm, decide_now);
   const int32_t source = prm->source;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
@@ -907,16 +923,17 @@ __global__ void k_slide_to_empty(BfsCtrl* c) {
 
 // Cleanup of gapbs-verify/bfs.cc:168-171 (DMM: at write-out, src/bfs.cc:
 // 224-228).
-// done (may be NULL): the device loop's flag; nothing until it is set,
-// then 2 marks the cleanup done.
-__global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, int* done) {
-  if (done && *done == 0) return;
+// head (may be NULL): the device loop's final loop state; nothing until
+// it is done, then done = 2 marks the cleanup done, also in the mapped
+// host flags.
+__global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, BfsHead* head, LoopFlags* flags) {
+  if (head && head->done == 0) return;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
   for (int64_t v = gtid; v < a.nloc; v += gsize) {
     if (a.parent[v] < -1) a.parent[v] = -1;
   }
-  if (done && gtid == 0) *done = 2;
+  if (head && gtid == 0) finish_flags(head, flags);
 }
 
 // Reached vertices and the out-degree sum over them (Graph500 edge count x2).
@@ -1028,7 +1045,10 @@ int read_ctrl_head(dmm_ctx* ctx, int parity = 0) {
   return DMM_OK;
 }
 
-int write_params(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta) {
+// The search's parameters into the pinned (mapped) staging; with copy, also
+// to the device copy (host-driven path; the device loop's InitParent reads
+// the staging itself).
+int write_params(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, bool copy = true) {
   BfsParams* p = reinterpret_cast<BfsParams*>(ctx->h_params);
   p->source = source;
   p->alpha = alpha;
@@ -1044,8 +1064,9 @@ int write_params(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta) {
   p->scan_end = scan_end;
   p->init_end = (ctx->parent_tail_dirty || src_tail) ? ctx->nloc : scan_end;
   ctx->parent_tail_dirty = src_tail;
-  DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params, sizeof(BfsParams), hipMemcpyHostToDevice,
-                                    ctx->stream));
+  if (copy)
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_params, ctx->h_params, sizeof(BfsParams), hipMemcpyHostToDevice,
+                                      ctx->stream));
   return DMM_OK;
 }
 
@@ -1090,7 +1111,8 @@ int bfs_init(dmm_ctx* ctx, int32_t source) {
   int rc = write_params(ctx, source, ctx->tuning.alpha, ctx->tuning.beta);
   if (rc) return rc;
   const BfsParams* prm = reinterpret_cast<const BfsParams*>(ctx->d_params);
-  hipLaunchKernelGGL(k_init, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, prm, 0);
+  hipLaunchKernelGGL(k_init, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, prm,
+                     static_cast<BfsParams*>(nullptr), 0);
   ctx->bfs_parity = 0;
   DMM_HIP_CHECK(ctx, hipGetLastError());
   ctx->bfs_valid = true;
@@ -1170,8 +1192,9 @@ int bfs_bitmap_to_queue(dmm_ctx* ctx, int64_t* queue_size) {
 
 int bfs_finish(dmm_ctx* ctx) {
   StepArgs a = make_args(ctx);
-  hipLaunchKernelGGL(k_finish, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a, nullptr);
-  if (ctx->relabeled) launch_map_parent(ctx, true, nullptr, &ctx->d_ctrl->scan_end);
+  hipLaunchKernelGGL(k_finish, dim3(grid_blocks(ctx)), dim3(kThreads), 0, ctx->stream, a,
+                     static_cast<BfsHead*>(nullptr), static_cast<LoopFlags*>(nullptr));
+  if (ctx->relabeled) launch_map_parent(ctx, true, ctx->d_ctrl, nullptr);
   ctx->depth_valid = true;
   DMM_HIP_CHECK(ctx, hipGetLastError());
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1210,7 +1233,8 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
   DMM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   const int blocks = grid_blocks(ctx);
   if (with_init) {
-    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, 1);
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a,
+                       static_cast<const BfsParams*>(ctx->hd_params), reinterpret_cast<BfsParams*>(ctx->d_params), 1);
   }
   for (int i = 0; i < nsteps; i++)
     hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, (parity + i) & 1);
@@ -1218,9 +1242,9 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
   // loop is done; either kernel marks the cleanup done).
   BfsHead* last = ((parity + nsteps) & 1) ? &ctx->d_ctrl->alt : static_cast<BfsHead*>(ctx->d_ctrl);
   if (ctx->relabeled)
-    launch_map_parent(ctx, with_depth, &last->done, &last->scan_end);
+    launch_map_parent(ctx, with_depth, last, ctx->hd_flags);
   else
-    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, &last->done);
+    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, last, ctx->hd_flags);
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -1248,28 +1272,33 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
     int rc = build_chain(ctx, kCont, false, with_depth, par, &ctx->bfs_cont[d][par]);
     if (rc) return rc;
   }
-  int rc = write_params(ctx, source, alpha, beta);
+  // The parameters go through mapped host memory and the end state comes
+  // back the same way: no copies on the stream.
+  int rc = write_params(ctx, source, alpha, beta, false);
   if (rc) return rc;
-  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
+  volatile LoopFlags* fl = ctx->h_flags;
+  fl->done = 0;
+  if (stats) DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
   DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_graph[d], ctx->stream));
-  DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+  if (stats) DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
   ctx->bfs_parity = par;
-  rc = read_ctrl_head(ctx, par);
-  if (rc) return rc;
+  ctx->h_ctrl_stale = true;
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   int rounds = 0;
-  while (ctx->h_ctrl->done != 2) {
+  while (fl->done != 2) {
     if (++rounds > (1 << 20)) return set_err(ctx, DMM_ERR_STATE, "device BFS loop did not terminate");
     DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bfs_cont[d][par], ctx->stream));
-    DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
-    rc = read_ctrl_head(ctx, par);
-    if (rc) return rc;
+    if (stats) DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   }
+  if (fl->error)
+    return set_err(ctx, DMM_ERR_STATE, "queue window holds a vertex of out-degree 0 (invariant violated)");
   ctx->bfs_valid = true;
   ctx->depth_valid = with_depth;
   // Size the first chain to the most steps any of the last searches ran,
   // plus the slot that applies the last step's post, so a typical search
   // finishes in one graph launch without trailing no-op slots.
-  ctx->bfs_recent[ctx->bfs_recent_pos++ % kRecentSearches] = ctx->h_ctrl->pairs;
+  ctx->bfs_recent[ctx->bfs_recent_pos++ % kRecentSearches] = fl->pairs;
   int most = 0;
   for (int i = 0; i < kRecentSearches; i++) most = std::max(most, ctx->bfs_recent[i]);
   ctx->bfs_chain_len = std::min(128, std::max(4, most + 1));

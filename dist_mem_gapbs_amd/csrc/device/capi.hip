@@ -158,6 +158,11 @@ int need_natural(dmm_ctx* ctx) {
                                       ctx->stream));
     ctx->bfs_parity = 0;
   }
+  if (ctx->h_ctrl_stale) {
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_ctrl, ctx->d_ctrl, sizeof(dmm::BfsHead), hipMemcpyDeviceToHost, ctx->stream));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->h_ctrl_stale = false;
+  }
   return DMM_OK;
 }
 
@@ -348,7 +353,10 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
                            "workgroup partials")))
     return bail(rc);
   if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(dmm::BfsCtrl), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&ctx->h_params, 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&ctx->h_params, 64, hipHostMallocMapped) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_flags, sizeof(dmm::LoopFlags), hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer(&ctx->hd_params, ctx->h_params, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&ctx->hd_flags, ctx->h_flags, 0) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_prctrl, sizeof(dmm::PrCtrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&ctx->h_prparams, dmm::kPrParamsBytes, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&ctx->h_bctrl, dmm::batch_ctrl_bytes(), hipHostMallocDefault) != hipSuccess ||
@@ -368,7 +376,8 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
   void* bufs[] = {ctx->d_ctrl, ctx->d_params, ctx->d_scratch64, ctx->d_prctrl, ctx->d_prparams,
                   ctx->d_err_trace, ctx->d_bctrl, ctx->d_bsrc, ctx->d_part};
   for (void* p : bufs) dmm::dev_free(p);
-  void* hbufs[] = {ctx->h_ctrl, ctx->h_params, ctx->h_prctrl, ctx->h_prparams, ctx->h_bctrl, ctx->h_bsrc};
+  void* hbufs[] = {ctx->h_ctrl, ctx->h_params, ctx->h_prctrl, ctx->h_prparams, ctx->h_bctrl, ctx->h_bsrc,
+                   ctx->h_flags};
   for (void* p : hbufs)
     if (p) (void)hipHostFree(p);
   for (auto e : ctx->step_events)

@@ -24,6 +24,15 @@ constexpr int kPartWidth = 4;       // int64 partials per workgroup
 constexpr int kRecentSearches = 32; // searches sizing the device-loop chain
 constexpr int kHeadNbrs = 4;        // in-neighbours per row in the head array
 
+// What the host needs of a finished device-loop search, written by its
+// last kernel into mapped host memory (no copy back).
+struct LoopFlags {
+  int done;   // 2 once the search and its cleanup are done
+  int error;
+  int pairs;
+  int pad;
+};
+
 // Loop state of a BFS on the device.  The head (BfsHead) is what the post
 // kernel reads and rewrites after every step; the trace arrays follow it.
 struct BfsHead {
@@ -149,7 +158,10 @@ struct dmm_ctx {
   int32_t* d_head = nullptr;  // first kHeadNbrs in-neighbours of each row, -1 padded
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed
-  void* h_params = nullptr;      // pinned staging for d_params
+  void* h_params = nullptr;      // pinned staging for d_params, mapped: the device loop's InitParent reads it
+  void* hd_params = nullptr;     // device address of h_params
+  dmm::LoopFlags* h_flags = nullptr;  // pinned, mapped: the device loop's end state
+  dmm::LoopFlags* hd_flags = nullptr; // device address of h_flags
   int64_t* d_scratch64 = nullptr;  // small reduction outputs
   int64_t* d_part = nullptr;     // per-workgroup partials (kPartWidth each)
   int64_t part_cap = 0;          // workgroups the partials can hold
@@ -163,6 +175,7 @@ struct dmm_ctx {
   hipGraphExec_t bfs_graph[2] = {nullptr, nullptr};  // init + steps + finish
   hipGraphExec_t bfs_cont[2][2] = {};  // more steps + finish, [depth][parity of the first slot]
   int bfs_parity = 0;  // head holding the latest loop state after the last device loop
+  bool h_ctrl_stale = false;  // h_ctrl predates the last device loop
   int bfs_graph_steps[2] = {0, 0};
   int bfs_chain_len = 16;
   int bfs_recent[dmm::kRecentSearches] = {};  // pairs used by recent searches
@@ -346,6 +359,19 @@ __device__ inline int64_t wave_last_le(const int64_t* key, int64_t lo, int64_t h
 }
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// End of a device-loop search (one thread): its cleanup is done, and the
+// host learns so, with the error flag and step count, through the mapped
+// flags (may be NULL).
+__device__ inline void finish_flags(BfsHead* head, LoopFlags* flags) {
+  head->done = 2;
+  if (!flags) return;
+  volatile LoopFlags* f = flags;
+  f->error = head->error;
+  f->pairs = head->pairs;
+  __threadfence_system();
+  f->done = 2;
+}
 
 // Fill kernels (used instead of hipMemsetAsync so the sequences can be
 // captured into hipGraphs).

@@ -25,9 +25,10 @@ int relabel_upload(dmm_ctx* ctx, const int64_t* old_off, const int32_t* old_nbr,
 int compute_scan_end(dmm_ctx* ctx, const int64_t* in_off);
 
 // Parents (and depths) back to original IDs, -1 where unreached (device to
-// device).  done_flag (may be NULL): skip while *done_flag == 0 (device loop
-// not finished), else set it to 2.  scan_end: the loop state's rows bound.
-void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag, const long long* scan_end);
+// device), with head's rows bound.  With flags (a device-loop search, head
+// its final loop state): nothing until the search is done, then the done
+// state goes to head and to the mapped host flags.
+void launch_map_parent(dmm_ctx* ctx, bool with_depth, BfsHead* head, LoopFlags* flags);
 void launch_map_scores_f32(dmm_ctx* ctx);
 void launch_map_scores_f64(dmm_ctx* ctx);
 

@@ -47,16 +47,17 @@ __global__ __launch_bounds__(kThreads) void k_relabel_rows(const int64_t* __rest
 
 // parent_out[v] = inv[parent[perm[v]]], -1 where unreached (the cleanup of
 // gapbs-verify/bfs.cc:168-171); rows from *scan_end on were not reached
-// and are not read.  With done_flag: nothing until the device loop has
-// finished (*done_flag != 0), then marks the cleanup done (2).
+// and are not read.  With gate (the device loop's final loop state):
+// nothing until the search is done, then the cleanup is marked done.
 __global__ __launch_bounds__(kThreads) void k_map_parent(const int32_t* __restrict__ parent,
                                                         const int32_t* __restrict__ depth,
                                                         const int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ inv, int64_t n,
                                                         const long long* scan_end_p,
                                                         int32_t* __restrict__ parent_out,
-                                                        int32_t* __restrict__ depth_out, int* done_flag) {
-  if (done_flag && *done_flag == 0) return;
+                                                        int32_t* __restrict__ depth_out, BfsHead* gate,
+                                                        LoopFlags* flags) {
+  if (gate && gate->done == 0) return;
   const int64_t scan_end = *scan_end_p;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kThreads) void k_map_parent(const int32_t* __restri
     parent_out[v] = po;
     if (depth) depth_out[v] = dep;
   }
-  if (done_flag && gtid == 0) *done_flag = 2;
+  if (gate && gtid == 0) finish_flags(gate, flags);
 }
 
 // Largest internal out-neighbour ID + 1 (block maximum, one atomic each).
@@ -190,10 +191,11 @@ int compute_scan_end(dmm_ctx* ctx, const int64_t* in_off) {
   return DMM_OK;
 }
 
-void launch_map_parent(dmm_ctx* ctx, bool with_depth, int* done_flag, const long long* scan_end) {
+void launch_map_parent(dmm_ctx* ctx, bool with_depth, BfsHead* head, LoopFlags* flags) {
+  BfsHead* gate = flags ? head : nullptr;
   hipLaunchKernelGGL(k_map_parent, dim3(grid_for(ctx, ctx->nloc)), dim3(kThreads), 0, ctx->stream, ctx->d_parent,
                      with_depth ? ctx->d_depth : nullptr, ctx->d_perm, ctx->d_inv, ctx->nloc,
-                     scan_end, ctx->d_parent_out, ctx->d_depth_out, done_flag);
+                     &head->scan_end, ctx->d_parent_out, ctx->d_depth_out, gate, flags);
 }
 
 void launch_map_scores_f32(dmm_ctx* ctx) {
