@@ -77,6 +77,7 @@ struct BArgs {
   uint64_t lanes;     // mask of used lanes
   int64_t m;          // directed edges of the block (td threshold base)
   int td_div;
+  int undirected;     // in-CSR == out-CSR
 };
 
 struct TdLds {
@@ -168,12 +169,17 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
         // seen[v] may or may not hold F[L][v] yet (lazy commit): mask both.
         uint64_t nw = L.f[lo] & ~a.seen[lv];
         if (nw) nw &= ~Fl[lv];
+        // Lanes another frontier vertex already pushed to v this level need
+        // no atomic (a plain read may miss some: then the atomic is only
+        // redundant).
+        if (nw) nw &= ~Fn[lv];
         if (nw) {
           atomicOr(reinterpret_cast<unsigned long long*>(&Fn[lv]), static_cast<unsigned long long>(nw));
           // A vertex without out-edges never enters an active list, so its
           // lanes are committed to seen here (the only writers of seen[v]
-          // for such v are these atomics).
-          if (a.out_off[lv + 1] == a.out_off[lv])
+          // for such v are these atomics).  Undirected graphs have none
+          // that an edge reaches.
+          if (!a.undirected && a.out_off[lv + 1] == a.out_off[lv])
             atomicOr(reinterpret_cast<unsigned long long*>(&a.seen[lv]), static_cast<unsigned long long>(nw));
         }
       }
@@ -606,6 +612,7 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   a.lanes = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
   a.m = std::max<int64_t>(1, ctx->m_out);
   a.td_div = std::max(1, td_div);
+  a.undirected = ctx->directed ? 0 : 1;
   return a;
 }
 
