@@ -39,7 +39,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
-constexpr int kItems = 4;
+constexpr int kItems = 2;
 constexpr int kEPB = kThreads * kItems;
 constexpr int kGroup = 8;  // lanes per vertex in the bottom-up sweep
 
@@ -204,7 +204,7 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
 // L2); both are probed before a word load, so sparse frontiers cost one L2
 // probe per edge.
 // Returns the end of the examined range; *found = the needed lanes found.
-constexpr int kSweepUnroll = 4;
+constexpr int kSweepUnroll = 32;
 __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr, int64_t b, int64_t e,
                                              int sub, uint64_t need, const uint64_t* __restrict__ front,
                                              const uint64_t* __restrict__ nz, const uint64_t* __restrict__ nz2,
