@@ -48,6 +48,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
 constexpr int kBuProbe = kHeadNbrs;  // in-neighbours probed per round (bottom-up)
+constexpr int kBuMore = 4;           // ... per round past the head array
 static_assert(kHeadNbrs == 4, "the head array is read as int4");
 constexpr int kItems = 4;                 // TD edges per thread per chunk
 constexpr int kEPB = kThreads * kItems;   // TD edges per chunk
@@ -324,15 +325,15 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
         } else {
           const int64_t s = in_off[lu], e = in_off[lu + 1];
           int64_t hit = e;
-          for (int64_t k = s + kBuProbe; k < e && hit == e; k += kBuProbe) {
-            int32_t v[kBuProbe];
+          for (int64_t k = s + kBuProbe; k < e && hit == e; k += kBuMore) {
+            int32_t v[kBuMore];
 #pragma unroll
-            for (int j = 0; j < kBuProbe; j++) v[j] = k + j < e ? in_nbr[k + j] : -1;
-            bool h[kBuProbe];
+            for (int j = 0; j < kBuMore; j++) v[j] = k + j < e ? in_nbr[k + j] : -1;
+            bool h[kBuMore];
 #pragma unroll
-            for (int j = 0; j < kBuProbe; j++) h[j] = v[j] >= 0 && ((front[v[j] >> 6] >> (v[j] & 63)) & 1ull);
+            for (int j = 0; j < kBuMore; j++) h[j] = v[j] >= 0 && ((front[v[j] >> 6] >> (v[j] & 63)) & 1ull);
 #pragma unroll
-            for (int j = kBuProbe - 1; j >= 0; j--)
+            for (int j = kBuMore - 1; j >= 0; j--)
               if (h[j]) {
                 hit = k + j;
                 pv = v[j];
@@ -1257,7 +1258,7 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
                     dmm_bfs_stats* stats) {
   // The first graph starts on head 0 and ends on head (want & 1); the
   // continuation (an even slot count) starts and ends on that head.
-  constexpr int kCont = 6;
+  constexpr int kCont = 2;
   const int d = with_depth ? 1 : 0;
   const int want = std::max(4, ctx->bfs_chain_len);
   const int par = want & 1;
@@ -1295,13 +1296,22 @@ int bfs_device_loop(dmm_ctx* ctx, int32_t source, int32_t alpha, int32_t beta, b
     return set_err(ctx, DMM_ERR_STATE, "queue window holds a vertex of out-degree 0 (invariant violated)");
   ctx->bfs_valid = true;
   ctx->depth_valid = with_depth;
-  // Size the first chain to the most steps any of the last searches ran,
-  // plus the slot that applies the last step's post, so a typical search
-  // finishes in one graph launch without trailing no-op slots.
+  // Size the first chain to the second-most steps of the last searches,
+  // plus the slot that applies the last step's post: a typical search
+  // finishes in one graph launch with few trailing no-op slots, and the
+  // longest one takes a short continuation.
   ctx->bfs_recent[ctx->bfs_recent_pos++ % kRecentSearches] = fl->pairs;
-  int most = 0;
-  for (int i = 0; i < kRecentSearches; i++) most = std::max(most, ctx->bfs_recent[i]);
-  ctx->bfs_chain_len = std::min(128, std::max(4, most + 1));
+  int most = 0, next = 0;
+  for (int i = 0; i < kRecentSearches; i++) {
+    const int p = ctx->bfs_recent[i];
+    if (p > most) {
+      next = most;
+      most = p;
+    } else if (p > next) {
+      next = p;
+    }
+  }
+  ctx->bfs_chain_len = std::min(128, std::max(4, next + 1));
   if (stats) {
     float ms = 0;
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
