@@ -196,15 +196,19 @@ __device__ void td_phase(const BArgs& a, BCtrl* c, TdLds& L, int level) {
 // out-degree sum.
 // Pull over one row's in-edges [b, e) by an 8-lane group: the first round
 // reads kGroup edges (rows whose needed lanes are all found there stop
-// early), later rounds keep kSweepUnroll frontier-word loads in flight per
-// lane; the group ORs them and the scan stops once every needed lane is
-// found.  nz (may be NULL) is a
+// early), later rounds keep kUnroll frontier-word loads in flight per lane;
+// the group ORs them and the scan stops once every needed lane is found.
+// The whole-graph sweep uses kSweepUnroll; the PE-block kernels, which probe
+// the non-zero bitmaps first and stop earlier, use kBlockUnroll.  nz (may be
+// NULL) is a
 // non-zero bitmap of the frontier words (bit v), with a summary bitmap nz2
 // (bit w set iff nz word w != 0, so 256 KB for 2^27 vertices, resident in
 // L2); both are probed before a word load, so sparse frontiers cost one L2
 // probe per edge.
 // Returns the end of the examined range; *found = the needed lanes found.
 constexpr int kSweepUnroll = 32;
+constexpr int kBlockUnroll = 4;
+template <int kUnroll>
 __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr, int64_t b, int64_t e,
                                              int sub, uint64_t need, const uint64_t* __restrict__ front,
                                              const uint64_t* __restrict__ nz, const uint64_t* __restrict__ nz2,
@@ -226,28 +230,28 @@ __device__ __forceinline__ int64_t sweep_row(const int32_t* __restrict__ in_nbr,
       return min<int64_t>(k0, e);
     }
   }
-  for (; k0 < e; k0 += kGroup * kSweepUnroll) {
-    int32_t v[kSweepUnroll];
+  for (; k0 < e; k0 += kGroup * kUnroll) {
+    int32_t v[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kSweepUnroll; j++) {
+    for (int j = 0; j < kUnroll; j++) {
       const int64_t k = k0 + j * kGroup + sub;
       v[j] = k < e ? in_nbr[k] : -1;
     }
     if (nz) {
 #pragma unroll
-      for (int j = 0; j < kSweepUnroll; j++)
+      for (int j = 0; j < kUnroll; j++)
         if (v[j] >= 0 && !((nz2[v[j] >> 12] >> ((v[j] >> 6) & 63)) & 1ull)) v[j] = -1;
 #pragma unroll
-      for (int j = 0; j < kSweepUnroll; j++)
+      for (int j = 0; j < kUnroll; j++)
         if (v[j] >= 0 && !((nz[v[j] >> 6] >> (v[j] & 63)) & 1ull)) v[j] = -1;
     }
 #pragma unroll
-    for (int j = 0; j < kSweepUnroll; j++)
+    for (int j = 0; j < kUnroll; j++)
       if (v[j] >= 0) acc |= front[v[j]];
 #pragma unroll
     for (int d = kGroup / 2; d >= 1; d >>= 1) acc |= __shfl_xor(static_cast<unsigned long long>(acc), d, kGroup);
     if ((acc & need) == need) {
-      stop = min<int64_t>(k0 + kGroup * kSweepUnroll, e);
+      stop = min<int64_t>(k0 + kGroup * kUnroll, e);
       break;
     }
   }
@@ -271,7 +275,7 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
     const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
     if (need && e > b) {
-      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, Fl, nullptr, nullptr, &nxt);
+      const int64_t stop = sweep_row<kSweepUnroll>(in_nbr, b, e, sub, need, Fl, nullptr, nullptr, &nxt);
       scanned += stop - b;
     }
     if (sub == 0) {
@@ -643,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restr
     const int64_t b = in_off[u], e = in_off[u + 1];
     uint64_t nxt = 0;
     if (need && e > b && e - b <= kBlockSplit) {
-      const int64_t stop = sweep_row(in_nbr, b, e, sub, need, front, fnz, fnz2, &nxt);
+      const int64_t stop = sweep_row<kBlockUnroll>(in_nbr, b, e, sub, need, front, fnz, fnz2, &nxt);
       scanned += stop - b;
     }
     if (sub == 0) next[u] = nxt;
@@ -687,7 +691,7 @@ __global__ __launch_bounds__(kThreads) void k_block_segs(const BlockSeg* __restr
     const uint64_t need = ~seen[sg.row];
     if (!need) continue;
     uint64_t nxt = 0;
-    const int64_t stop = sweep_row(in_nbr, sg.b, sg.e, sub, need, front, fnz, fnz2, &nxt);
+    const int64_t stop = sweep_row<kBlockUnroll>(in_nbr, sg.b, sg.e, sub, need, front, fnz, fnz2, &nxt);
     scanned += stop - sg.b;
     if (sub == 0 && nxt)
       atomicOr(reinterpret_cast<unsigned long long*>(&next[sg.row]), static_cast<unsigned long long>(nxt));
