@@ -725,22 +725,38 @@ __global__ __launch_bounds__(kThreads) void k_slot(StepArgs a, const BfsParams* 
   const int tid = threadIdx.x;
   const int G = gridDim.x;
   const int me = blockIdx.x;
-  if (tid < kHeadWords) L.sh.w[tid] = reinterpret_cast<const long long*>(hin)[tid];
+  // Every load of the prologue is issued before the first wait: the head
+  // word (lanes past the head re-read its last word), the parameters and
+  // the step error, and the partials.
+  const long long hw = reinterpret_cast<const long long*>(hin)[min(tid, kHeadWords - 1)];
+  const BfsParams pv = *prm;
+  const int err_in = c->slot_error;
   // Totals of slot k-1's partials, and this workgroup's prefix of the
-  // count and degree partials (the WRITE positions after a COUNT).
+  // count and degree partials (the WRITE positions after a COUNT).  A round
+  // loads kRound partials per thread (indices past G re-read partial G-1
+  // and are masked).
   int64_t t0 = 0, t1 = 0, t2 = 0, b0 = 0, b1 = 0;
-  for (int b = tid; b < G; b += kThreads) {
-    const int64_t v0 = pin[b * kPartWidth + 0];
-    const int64_t v1 = pin[b * kPartWidth + 1];
-    const int64_t v2 = pin[b * kPartWidth + 2];
-    t0 += v0;
-    t1 += v1;
-    t2 += v2;
-    if (b < me) {
-      b0 += v0;
-      b1 += v1;
+  constexpr int kRound = 4;
+  for (int base = 0; base < G; base += kRound * kThreads) {
+    int64_t v[kRound][3];
+#pragma unroll
+    for (int j = 0; j < kRound; j++) {
+      const int b = min(base + j * kThreads + tid, G - 1);
+#pragma unroll
+      for (int k = 0; k < 3; k++) v[j][k] = pin[b * kPartWidth + k];
+    }
+#pragma unroll
+    for (int j = 0; j < kRound; j++) {
+      const int b = base + j * kThreads + tid;
+      const int64_t live = b < G ? -1 : 0, pre = b < me ? -1 : 0;
+      t0 += v[j][0] & live;
+      t1 += v[j][1] & live;
+      t2 += v[j][2] & live;
+      b0 += v[j][0] & pre;
+      b1 += v[j][1] & pre;
     }
   }
+  if (tid < kHeadWords) L.sh.w[tid] = hw;
   {
     const int64_t w0 = wave_sum(t0), w1 = wave_sum(t1), w2 = wave_sum(t2), w3 = wave_sum(b0), w4 = wave_sum(b1);
     if (lane_id() == 0) {
@@ -758,8 +774,8 @@ __global__ __launch_bounds__(kThreads) void k_slot(StepArgs a, const BfsParams* 
     for (int w = 0; w < kThreads / 64; w++)
       for (int k = 0; k < 5; k++) sum[k] += L.red[k][w];
     BfsHead& h = L.sh.h;
-    h.error |= c->slot_error;
-    if (h.ran) post_update<-1>(h, me == 0 ? c : nullptr, a, prm, h.ran, sum[0], sum[1], sum[2]);
+    h.error |= err_in;
+    if (h.ran) post_update<-1>(h, me == 0 ? c : nullptr, a, &pv, h.ran, sum[0], sum[1], sum[2]);
     h.ran = (h.phase >= kPhaseTD && h.phase <= kPhaseWrite) ? h.phase : 0;
     L.base_c = sum[3];
     L.base_d = sum[4];
