@@ -715,22 +715,24 @@ struct SlotLds {
   int64_t base_c, base_d;
 };
 
-__global__ __launch_bounds__(kThreads) void k_slot(StepArgs a, const BfsParams* prm, int parity) {
+// The prologue's addresses lead the argument list (hin = the head slot k-1
+// wrote, pin = its partials, errp = the step error flag, G = gridDim.x), so
+// that they can be preloaded into registers at wave start.
+__global__ __launch_bounds__(kThreads) void k_slot(const long long* __restrict__ hin, const int64_t* __restrict__ pin,
+                                                   const int* errp, const BfsParams* prm, int G, StepArgs a,
+                                                   int parity) {
   __shared__ StepLds lds;
   __shared__ SlotLds L;
-  BfsCtrl* c = a.ctrl;
-  const BfsHead* hin = parity ? &c->alt : static_cast<const BfsHead*>(c);
-  BfsHead* hout = parity ? static_cast<BfsHead*>(c) : &c->alt;
-  const int64_t* pin = a.part + (parity ^ 1) * a.part_half;
   const int tid = threadIdx.x;
-  const int G = gridDim.x;
   const int me = blockIdx.x;
   // Every load of the prologue is issued before the first wait: the head
   // word (lanes past the head re-read its last word), the parameters and
   // the step error, and the partials.
-  const long long hw = reinterpret_cast<const long long*>(hin)[min(tid, kHeadWords - 1)];
+  const long long hw = hin[min(tid, kHeadWords - 1)];
   const BfsParams pv = *prm;
-  const int err_in = c->slot_error;
+  const int err_in = *errp;
+  BfsCtrl* c = a.ctrl;
+  BfsHead* hout = parity ? static_cast<BfsHead*>(c) : &c->alt;
   // Totals of slot k-1's partials, and this workgroup's prefix of the
   // count and degree partials (the WRITE positions after a COUNT).  A round
   // loads kRound partials per thread (indices past G re-read partial G-1
@@ -1253,8 +1255,13 @@ static int build_chain(dmm_ctx* ctx, int nsteps, bool with_init, bool with_depth
     hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kThreads), 0, ctx->stream, a,
                        static_cast<const BfsParams*>(ctx->hd_params), reinterpret_cast<BfsParams*>(ctx->d_params), 1);
   }
-  for (int i = 0; i < nsteps; i++)
-    hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, prm, (parity + i) & 1);
+  for (int i = 0; i < nsteps; i++) {
+    const int par = (parity + i) & 1;
+    const BfsHead* hin = par ? &ctx->d_ctrl->alt : static_cast<const BfsHead*>(ctx->d_ctrl);
+    hipLaunchKernelGGL(k_slot, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+                       reinterpret_cast<const long long*>(hin), a.part + (par ^ 1) * a.part_half,
+                       &ctx->d_ctrl->slot_error, prm, blocks, a, par);
+  }
   // Results back to the caller's IDs, unreached as -1 (a no-op until the
   // loop is done; either kernel marks the cleanup done).
   BfsHead* last = ((parity + nsteps) & 1) ? &ctx->d_ctrl->alt : static_cast<BfsHead*>(ctx->d_ctrl);
