@@ -296,16 +296,25 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
       if (parent[lu] < 0) {
         scanned++;
         // kBuProbe in-neighbours per round: their IDs, then their frontier
-        // bits, all loads in flight at once; the first hit in row order is
-        // the parent (the sequential scan's choice and edge count).  The
-        // first round reads the head array, where the rows of a wave's 64
-        // vertices lie side by side (one coalesced 1 KB load); the CSR row
-        // is read only past it.
+        // bits; the first hit in row order is the parent (the sequential
+        // scan's choice and edge count).  The first round reads the head
+        // array, where the rows of a wave's 64 vertices lie side by side
+        // (one coalesced 1 KB load); the CSR row is read only past it, with
+        // all of a round's loads in flight at once.
         const int4 hd = reinterpret_cast<const int4*>(a.head)[lu];
         const int32_t h0[kBuProbe] = {hd.x, hd.y, hd.z, hd.w};
+        // The first head neighbour is probed alone and the other three only
+        // after it misses: the first in-neighbour (lowest ID, the highest
+        // degree class) is usually a hot hub in the frontier, so the colder
+        // probes, each an L1 miss, are mostly skipped.
         bool hh[kBuProbe];
+        hh[0] = h0[0] >= 0 && ((front[h0[0] >> 6] >> (h0[0] & 63)) & 1ull);
 #pragma unroll
-        for (int j = 0; j < kBuProbe; j++) hh[j] = h0[j] >= 0 && ((front[h0[j] >> 6] >> (h0[j] & 63)) & 1ull);
+        for (int j = 1; j < kBuProbe; j++) hh[j] = false;
+        if (!hh[0]) {
+#pragma unroll
+          for (int j = 1; j < kBuProbe; j++) hh[j] = h0[j] >= 0 && ((front[h0[j] >> 6] >> (h0[j] & 63)) & 1ull);
+        }
         int hj = kBuProbe;
         int32_t pv = 0;
 #pragma unroll
