@@ -35,6 +35,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kGroup = 4;         // lanes per light vertex (KAHAN / FP64)
 constexpr int kUnroll = 8;        // gathers in flight per lane on heavy items
+constexpr int kLightUnroll = 4;   // gathers in flight per lane on light rows
 constexpr int kFusedThreads = 1024;
 constexpr int kHubBytes = 128 * 1024;  // LDS copy of the first contributions
 
@@ -141,7 +142,20 @@ __global__ __launch_bounds__(kThreads) void k_serial_light(PrArgs a, const float
     const int64_t s = a.in_off[u], e = a.in_off[u + 1];
     if (e - s >= a.heavy_threshold) continue;
     float incoming_total = 0;
-    for (int64_t k = s; k < e; k++) incoming_total += cin[a.in_nbr[k]];
+    // In-neighbour order, as src/pr.cc:57-59; the loads run kLightUnroll
+    // ahead of the adds, which keep their order.
+    int64_t k = s;
+    for (; k + kLightUnroll <= e; k += kLightUnroll) {
+      int32_t v[kLightUnroll];
+      float x[kLightUnroll];
+#pragma unroll
+      for (int j = 0; j < kLightUnroll; j++) v[j] = a.in_nbr[k + j];
+#pragma unroll
+      for (int j = 0; j < kLightUnroll; j++) x[j] = cin[v[j]];
+#pragma unroll
+      for (int j = 0; j < kLightUnroll; j++) incoming_total += x[j];
+    }
+    for (; k < e; k++) incoming_total += cin[a.in_nbr[k]];
     err += finish_f32(a, u, incoming_total, score, cout);
   }
   double t = block_err_sum(err, red);
@@ -318,7 +332,18 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
       const int64_t s = a.in_off[u], e = a.in_off[u + 1];
       if (e - s < a.heavy_threshold) {
         mine = true;
-        for (int64_t k = s + sub; k < e; k += kGroup) acc.add(hub_gather(hub, hub_n, cin, a.in_nbr[k]));
+        int64_t k = s + sub;
+        for (; k + (kLightUnroll - 1) * kGroup < e; k += kLightUnroll * kGroup) {
+          int32_t v[kLightUnroll];
+          T x[kLightUnroll];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) v[j] = a.in_nbr[k + j * kGroup];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, v[j]);
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) acc.add(x[j]);
+        }
+        for (; k < e; k += kGroup) acc.add(hub_gather(hub, hub_n, cin, a.in_nbr[k]));
       }
     }
     acc.template reduce<kGroup>(sub);
@@ -491,14 +516,32 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_tiled(PrArgs a, TiledAr
       const int64_t s = t.off[u], e = t.off[u + 1];
       if (e - s < a.heavy_threshold) {
         mine = true;
-        // The row's list edges, then its pieces, as one strided range.
+        // The row's list edges, then its pieces, as one strided range; each
+        // lane takes its own entries in order, kLightUnroll loads in flight.
         const int64_t pb = t.poff[u], np = t.poff[u + 1] - pb, nl = e - s;
-        for (int64_t k = sub; k < nl + np; k += kGroup) {
-          if (k < nl)
-            acc.add(hub_gather(hub, hub_n, cin, t.nbr[s + k]));
-          else
-            acc.merge(t.ppart[t.pslot[pb + k - nl]]);
+        int64_t k = sub;
+        for (; k + (kLightUnroll - 1) * kGroup < nl; k += kLightUnroll * kGroup) {
+          int32_t v[kLightUnroll];
+          float x[kLightUnroll];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) v[j] = t.nbr[s + k + j * kGroup];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, v[j]);
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) acc.add(x[j]);
         }
+        for (; k < nl; k += kGroup) acc.add(hub_gather(hub, hub_n, cin, t.nbr[s + k]));
+        for (; k + (kLightUnroll - 1) * kGroup < nl + np; k += kLightUnroll * kGroup) {
+          int32_t q[kLightUnroll];
+          float2 pp[kLightUnroll];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) q[j] = t.pslot[pb + k + j * kGroup - nl];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) pp[j] = t.ppart[q[j]];
+#pragma unroll
+          for (int j = 0; j < kLightUnroll; j++) acc.merge(pp[j]);
+        }
+        for (; k < nl + np; k += kGroup) acc.merge(t.ppart[t.pslot[pb + k - nl]]);
       }
     }
     acc.reduce<kGroup>(sub);
