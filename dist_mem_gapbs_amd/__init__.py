@@ -9,10 +9,10 @@ from ._lib import (DMM_PR_FP32_KAHAN, DMM_PR_FP32_SERIAL, DMM_PR_FP64, DmmError,
                    load)
 from .engine import Context, device_count, stats_dict
 from .graph import (HostGraph, build_graph, from_edgelist, generate_edgelist, partition,
-                    pick_sources)
+                    pick_sources, read_serialized_graph)
 
 __all__ = [
     "Context", "DmmError", "HostGraph", "LIB_PATH", "DMM_PR_FP32_SERIAL", "DMM_PR_FP32_KAHAN",
     "DMM_PR_FP64", "build_graph", "device_count", "from_edgelist", "generate_edgelist", "load",
-    "partition", "pick_sources", "stats_dict",
+    "partition", "pick_sources", "read_serialized_graph", "stats_dict",
 ]

@@ -137,6 +137,8 @@ SIGNATURES = {
     "dmm_host_graph_in_offsets": (_PI64, [_P]),
     "dmm_host_graph_in_neighs": (_PI32, [_P]),
     "dmm_host_graph_free": (None, [_P]),
+    "dmm_write_serialized_graph": (_I32, [_P, C.c_char_p]),
+    "dmm_read_serialized_graph": (_I32, [C.c_char_p, C.POINTER(_P)]),
     "dmm_pick_sources": (_I32, [_I64, _PI64, _I32, _I32, _PI32]),
     "dmm_rng_sequence": (_I32, [_I32, _I64, _I64, _I64, _PI64]),
     "dmm_partition": (_I32, [_I64, _I32, _I32, _PI64, _PI64, _PI64, _PI64]),

@@ -89,8 +89,10 @@ class CLBase {
       std::cout << "No graph input specified. (Use -h for help)" << std::endl;
       return false;
     }
-    if (filename_ != "") {
-      std::cout << "File input is not supported by this build; use -g or -u" << std::endl;
+    // Files: the serialized graph cache (.sg, src/reader.h:291-345); the
+    // text edge-list readers are not part of this build.
+    if (filename_ != "" && (filename_.size() < 3 || filename_.compare(filename_.size() - 3, 3, ".sg") != 0)) {
+      std::cout << "Only .sg graph files are supported by this build; use -g or -u otherwise" << std::endl;
       return false;
     }
     if (scale_ != -1) symmetrize_ = true;
@@ -229,6 +231,15 @@ class Builder {
     Timer t;
     t.Start();
     dmm_host_graph* hg = nullptr;
+    if (cli_.filename() != "") {  // src/builder.h:320-322
+      if (dmm_read_serialized_graph(cli_.filename().c_str(), &hg) != DMM_OK) {
+        std::cerr << "graph read failed: " << dmm_last_error(nullptr) << std::endl;
+        return false;
+      }
+      t.Stop();
+      PrintTime("Read Time", t.Seconds());
+      return g->Init(hg, device);
+    }
     if (dmm_build_graph(cli_.scale(), cli_.degree(), cli_.uniform() ? 1 : 0, 0, &hg) != DMM_OK) {
       std::cerr << "graph build failed: " << dmm_last_error(nullptr) << std::endl;
       return false;

@@ -10,6 +10,7 @@
 #include "generator.h"
 #include "partition.h"
 #include "rng.h"
+#include "sgio.h"
 
 namespace dmm {
 std::string& thread_error() {
@@ -109,6 +110,25 @@ extern "C" const int32_t* dmm_host_graph_in_neighs(const dmm_host_graph* g) {
   return g ? g->g.in_nbr() : nullptr;
 }
 extern "C" void dmm_host_graph_free(dmm_host_graph* g) { delete g; }
+
+extern "C" int dmm_write_serialized_graph(const dmm_host_graph* g, const char* path) {
+  if (!g) return fail(nullptr, DMM_ERR_INVALID, "graph is NULL");
+  return dmm::write_sg(g->g, path);
+}
+
+extern "C" int dmm_read_serialized_graph(const char* path, dmm_host_graph** out) {
+  if (!out) return fail(nullptr, DMM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  dmm_host_graph* hg = new (std::nothrow) dmm_host_graph;
+  if (!hg) return fail(nullptr, DMM_ERR_NOMEM, "host allocation failed");
+  const int rc = dmm::read_sg(path, &hg->g);
+  if (rc != DMM_OK) {
+    delete hg;
+    return rc;
+  }
+  *out = hg;
+  return DMM_OK;
+}
 
 extern "C" int dmm_pick_sources(int64_t num_nodes, const int64_t* out_offsets,
                                 int32_t given_source, int32_t count, int32_t* out) {

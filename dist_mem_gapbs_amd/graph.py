@@ -43,6 +43,10 @@ class HostGraph:
     def in_neigh(self, v):
         return self.in_neighs[self.in_offsets[v]:self.in_offsets[v + 1]]
 
+    def write_serialized(self, path):
+        """WriterBase::WriteSerializedGraph (src/writer.h:39-69): the .sg cache."""
+        L.check(self._lib.dmm_write_serialized_graph(self._h, str(path).encode()))
+
     def free(self):
         if self._h:
             self._lib.dmm_host_graph_free(self._h)
@@ -80,6 +84,15 @@ def from_edgelist(pairs, num_nodes=-1, symmetrize=True, nthreads=0):
     h = C.c_void_p()
     L.check(lib.dmm_build_graph_from_edgelist(len(pairs), L.ptr(pairs, C.c_int32), num_nodes,
                                               1 if symmetrize else 0, nthreads, C.byref(h)))
+    return HostGraph(h)
+
+
+def read_serialized_graph(path):
+    """Reader::ReadSerializedGraph (src/reader.h:291-345): a .sg file, checked
+    before use (DmmError with DMM_ERR_GRAPH for a malformed file)."""
+    lib = L.load()
+    h = C.c_void_p()
+    L.check(lib.dmm_read_serialized_graph(str(path).encode(), C.byref(h)))
     return HostGraph(h)
 
 

@@ -93,6 +93,21 @@ const int64_t* dmm_host_graph_in_offsets(const dmm_host_graph* g);
 const int32_t* dmm_host_graph_in_neighs(const dmm_host_graph* g);
 void dmm_host_graph_free(dmm_host_graph* g);
 
+/* Replaces WriterBase::WriteSerializedGraph (src/writer.h:39-69; the writer
+ * behind converter's -b file.sg): the .sg cache of a graph.  Layout: bool
+ * directed (1 byte), int64 num_edges (= num_edges_directed), int64 num_nodes,
+ * int64 out_offsets[num_nodes+1], int32 out_neighs[num_edges], and for a
+ * directed graph int64 in_offsets[num_nodes+1], int32 in_neighs[num_edges].
+ * Errors (cannot open or write) are status codes, where the reference exits. */
+int dmm_write_serialized_graph(const dmm_host_graph* g, const char* path);
+
+/* Replaces Reader::ReadSerializedGraph (src/reader.h:291-345; the -f file.sg
+ * path of BuilderBase::MakeGraph, src/builder.h:314-322).  The file is
+ * checked before use: sizes against its length, offsets a CSR index, IDs in
+ * [0, num_nodes).  A bad file is DMM_ERR_GRAPH, a missing one
+ * DMM_ERR_INVALID. */
+int dmm_read_serialized_graph(const char* path, dmm_host_graph** out);
+
 /* Replaces SourcePicker::PickNext x count (src/benchmark.h:49-87): mt19937
  * seeded with kRandSeed, uniform over [0, num_nodes), rejecting out-degree 0.
  * given_source >= 0 returns it every time (the -r flag). */

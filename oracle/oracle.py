@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liboracle.so")
 REF_BFS = os.path.join(_HERE, "_ref", "libref_bfs.so")
 REF_PR = os.path.join(_HERE, "_ref", "libref_pr.so")
+REF_SG = os.path.join(_HERE, "_ref", "libref_sg.so")
 
 _P64 = C.POINTER(C.c_int64)
 _P32 = C.POINTER(C.c_int32)
@@ -196,6 +197,55 @@ def _ref_libs():
         p.ref_pagerank.argtypes = [C.c_void_p, C.c_int32, C.c_double, _PF]
         _ref["bfs"], _ref["pr"] = b, p
     return _ref["bfs"], _ref["pr"]
+
+
+def ref_sg_available():
+    return os.path.exists(REF_SG)
+
+
+def _ref_sg():
+    if "sg" not in _ref:
+        g = C.CDLL(REF_SG)
+        g.ref_write_sg.argtypes = [C.c_int64, C.c_int, _P64, _P32, _P64, _P32, C.c_char_p]
+        g.ref_read_sg.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(_P64),
+                                  C.POINTER(_P32), C.POINTER(_P64), C.POINTER(_P32)]
+        g.ref_free.argtypes = [C.c_void_p]
+        _ref["sg"] = g
+    return _ref["sg"]
+
+
+def ref_write_sg(path, num_nodes, directed, offs, neigh, in_offs=None, in_neigh=None):
+    """The reference's WriterBase::WriteSerializedGraph (gapbs-verify/writer.h:39-69)."""
+    g = _ref_sg()
+    offs = np.ascontiguousarray(offs, np.int64)
+    neigh = np.ascontiguousarray(neigh, np.int32)
+    io = np.ascontiguousarray(in_offs if directed else offs, np.int64)
+    inb = np.ascontiguousarray(in_neigh if directed else neigh, np.int32)
+    g.ref_write_sg(num_nodes, 1 if directed else 0, offs.ctypes.data_as(_P64), neigh.ctypes.data_as(_P32),
+                   io.ctypes.data_as(_P64), inb.ctypes.data_as(_P32), str(path).encode())
+
+
+def ref_read_sg(path):
+    """The reference's Reader::ReadSerializedGraph (gapbs-verify/reader.h:254-300):
+    (num_nodes, directed, out_offsets, out_neighs, in_offsets, in_neighs)."""
+    g = _ref_sg()
+    n, d = C.c_int64(), C.c_int()
+    o, nb, io, inb = _P64(), _P32(), _P64(), _P32()
+    g.ref_read_sg(str(path).encode(), C.byref(n), C.byref(d), C.byref(o), C.byref(nb), C.byref(io), C.byref(inb))
+    n = n.value
+
+    def take(p, k, dt):
+        a = np.ctypeslib.as_array(p, (max(k, 1),))[:k].astype(dt, copy=True)
+        g.ref_free(C.cast(p, C.c_void_p))
+        return a
+
+    offs = take(o, n + 1, np.int64)
+    neigh = take(nb, int(offs[-1]), np.int32)
+    in_offs = in_neigh = None
+    if d.value:
+        in_offs = take(io, n + 1, np.int64)
+        in_neigh = take(inb, int(in_offs[-1]), np.int32)
+    return n, bool(d.value), offs, neigh, in_offs, in_neigh
 
 
 def ref_make_graph(scale, degree, uniform):

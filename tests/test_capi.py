@@ -99,3 +99,9 @@ def test_apps_parse_like_reference():
     assert "No graph input specified" in r.stdout and r.returncode != 0
     r = subprocess.run([pr, "-h"], capture_output=True, text=True)
     assert "-i <i>" in r.stdout and "-t <t>" in r.stdout
+    # -f takes the .sg cache only (src/builder.h:320-322); a missing file is
+    # reported before any device call.
+    r = subprocess.run([bfs, "-f", "graph.el"], capture_output=True, text=True)
+    assert "Only .sg graph files" in r.stdout and r.returncode != 0
+    r = subprocess.run([pr, "-f", os.path.join(ROOT, "no_such_graph.sg")], capture_output=True, text=True)
+    assert "couldn't open file" in r.stderr and r.returncode != 0

@@ -142,6 +142,24 @@ def test_isolated_sources_between_searches(ctx, dmm, orc, graph):
                 assert st.reached == int((ref >= 0).sum())
 
 
+def test_graph_from_sg_file(ctx, dmm, orc, graph, tmp_path):
+    # A graph read back from its .sg cache (the -f file.sg path of
+    # src/builder.h:314-322) uploads and searches like the generated one.
+    g = graph(12, 16, False)
+    p = tmp_path / "kron12.sg"
+    g.write_serialized(p)
+    h = dmm.read_serialized_graph(p)
+    ctx.upload(h)
+    og = orc.CSR.of(g)
+    for s in dmm.pick_sources(g, 4):
+        parent, depth, _ = ctx.DOBFS(int(s))
+        assert np.array_equal(depth, orc.bfs_depth(og, int(s))), f"depth mismatch from {s}"
+        ok, bad = orc.bfs_verify(og, int(s), parent)
+        assert ok, f"parent tree fails the verifier at {bad} (source {s})"
+    scores, _ = ctx.PageRankPull(20, 1e-4, dmm.DMM_PR_FP32_SERIAL)
+    assert np.array_equal(scores, orc.pr_f32(og, 20, 1e-4)[0])
+
+
 def test_directed_graph(ctx, dmm, orc):
     rng = np.random.default_rng(5)
     pairs = rng.integers(0, 2000, size=(12000, 2))
