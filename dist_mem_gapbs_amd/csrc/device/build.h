@@ -1,0 +1,18 @@
+// GPU CSR builder (device/build.hip).
+#ifndef DMM_DEVICE_BUILD_H_
+#define DMM_DEVICE_BUILD_H_
+
+#include <cstdint>
+
+#include "../host/graph.h"
+#include "common.h"
+
+namespace dmm {
+
+// The squished CSR of ne (u, v) pairs (IDs already checked to lie in
+// [0, n), n >= 1 when ne > 0), built on ctx's device and copied into g.
+int build_graph_device(dmm_ctx* ctx, const int32_t* pairs, int64_t ne, int64_t n, bool symmetrize, HostGraph* g);
+
+}  // namespace dmm
+
+#endif  // DMM_DEVICE_BUILD_H_

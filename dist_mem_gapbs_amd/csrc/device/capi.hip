@@ -13,6 +13,7 @@
 #include "../host/error.h"
 #include "batch.h"
 #include "bfs.h"
+#include "build.h"
 #include "common.h"
 #include "layout.h"
 #include "pr.h"
@@ -386,6 +387,35 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs, int64_t num_nodes,
+                           int32_t symmetrize, dmm_host_graph** out) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (!out) return set_err(ctx, DMM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (num_pairs < 0 || (num_pairs > 0 && !pairs)) return set_err(ctx, DMM_ERR_INVALID, "bad edge list");
+  // The host builder's checks and FindMaxNodeID (src/builder.h:72-89).
+  int64_t max_id = -1;
+  for (int64_t i = 0; i < 2 * num_pairs; i++) {
+    const int32_t x = pairs[i];
+    if (x < 0 || (num_nodes >= 0 && x >= num_nodes))
+      return set_err(ctx, DMM_ERR_GRAPH, "edge %lld (%d, %d) out of range", (long long)(i / 2), pairs[i & ~1ll],
+                     pairs[i | 1]);
+    max_id = std::max<int64_t>(max_id, x);
+  }
+  if (num_nodes < 0) num_nodes = max_id + 1;
+  if (num_nodes > 0x7FFFFFFFll)
+    return set_err(ctx, DMM_ERR_INVALID, "num_nodes %lld exceeds int32 IDs", (long long)num_nodes);
+  dmm_host_graph* hg = new (std::nothrow) dmm_host_graph;
+  if (!hg) return set_err(ctx, DMM_ERR_NOMEM, "host allocation failed");
+  const int rc = dmm::build_graph_device(ctx, pairs, num_pairs, num_nodes, symmetrize != 0, &hg->g);
+  if (rc != DMM_OK) {
+    delete hg;
+    return rc;
+  }
+  *out = hg;
+  return DMM_OK;
 }
 
 int dmm_ctx_set_layout(dmm_ctx* ctx, int32_t layout) {

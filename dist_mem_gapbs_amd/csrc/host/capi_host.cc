@@ -19,10 +19,6 @@ std::string& thread_error() {
 }
 }  // namespace dmm
 
-struct dmm_host_graph {
-  dmm::HostGraph g;
-};
-
 using dmm::fail;
 
 static int check_scale(int32_t scale, int32_t degree) {

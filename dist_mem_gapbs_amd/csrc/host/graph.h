@@ -47,4 +47,9 @@ struct HostGraph {
 
 }  // namespace dmm
 
+// The C ABI's opaque host graph (include/dmm_gapbs.h).
+struct dmm_host_graph {
+  dmm::HostGraph g;
+};
+
 #endif  // DMM_HOST_GRAPH_H_

@@ -58,6 +58,16 @@ class Context:
         return self.vend - self.vbegin
 
     # ---- graph --------------------------------------------------------------
+    def build_graph(self, pairs, num_nodes=-1, symmetrize=True):
+        """Builder::MakeGraphFromEL + SquishGraph (src/builder.h:290-312) on this
+        context's GPU; returns the HostGraph (not uploaded)."""
+        from .graph import HostGraph
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        h = C.c_void_p()
+        self._check(self._lib.dmm_build_graph_device(self._h, len(pairs), L.ptr(pairs, C.c_int32), num_nodes,
+                                                     1 if symmetrize else 0, C.byref(h)))
+        return HostGraph(h)
+
     def upload(self, g=None, *, num_nodes=None, out_offsets=None, out_neighs=None,
                in_offsets=None, in_neighs=None, vbegin=0, vend=None, num_edges_directed=-1):
         """Uploads a HostGraph (whole graph) or explicit block arrays."""

@@ -142,6 +142,16 @@ int dmm_device_count(int32_t* count);
 int dmm_ctx_create(int32_t device, dmm_ctx** out);
 void dmm_ctx_destroy(dmm_ctx* ctx);
 
+/* Replaces BuilderBase::MakeGraphFromEL + SquishGraph (src/builder.h:290-312,
+ * 168-226) on ctx's GPU: the same squished CSR as dmm_build_graph_from_edgelist
+ * (rows sorted, duplicates and self-loops removed; num_nodes < 0 means max ID
+ * + 1; symmetrize != 0 builds the undirected graph, otherwise out- and
+ * in-CSR), from a radix sort of (row, column) keys.  The result is a host
+ * graph, ready for dmm_graph_upload or dmm_write_serialized_graph; the
+ * device memory the build used is freed before the call returns. */
+int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs,
+                           int64_t num_nodes, int32_t symmetrize, dmm_host_graph** out);
+
 /* Uploads one PE's share of a CSR graph (replaces the symmetric-heap CSR of
  * src/builder.h:236-288 and the remote Neighborhood fetch of
  * src/graph.h:138-162).
