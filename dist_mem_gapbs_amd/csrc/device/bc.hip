@@ -1,0 +1,148 @@
+// Brandes' forward phase (shmem_BFS, src/bc.cc:55-96) on one GPU: a
+// level-synchronous top-down BFS that also counts shortest paths and marks
+// successor edges.
+//
+// Level L (frontier = queue[qb, qe), depth L-1): one wavefront per frontier
+// vertex u, its lanes striding u's out-edges in 64-edge rounds.  For edge
+// k = (u, v):
+//   - an unvisited v (plain read -1) is claimed with a CAS to L, and the
+//     winner appends v to the queue (src/bc.cc:77-79); a failed CAS returns
+//     v's depth, so no stale read decides the next test;
+//   - if depth(v) == L the edge is a successor and v's path count grows by
+//     u's (src/bc.cc:80-83).  u's count is final: every addition to it came
+//     from the previous level's launch.
+// Successor bits of a 64-edge round come from one ballot and go to at most
+// two bitmap words (two atomicOr from lane 0), not one atomic per edge.
+// Depths, counts and bits do not depend on the order of the atomics, so they
+// equal the serial restatement bit for bit; only the queue order inside a
+// level does.
+#include <algorithm>
+
+#include "bc.h"
+#include "common.h"
+
+namespace dmm {
+
+namespace {
+
+constexpr int kCThreads = 256;
+
+int grid_of(int64_t items) {
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(65536, (items + kCThreads - 1) / kCThreads)));
+}
+
+__global__ __launch_bounds__(kCThreads) void k_bc_init(int64_t n, int64_t words, int32_t source,
+                                                       int32_t* __restrict__ depth,
+                                                       unsigned long long* __restrict__ pc,
+                                                       unsigned long long* __restrict__ succ,
+                                                       int32_t* __restrict__ queue,
+                                                       unsigned long long* __restrict__ tail) {
+  const int64_t gtid = int64_t(blockIdx.x) * kCThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kCThreads;
+  for (int64_t v = gtid; v < n; v += gsize) {
+    depth[v] = v == source ? 0 : -1;
+    pc[v] = v == source ? 1ull : 0ull;
+  }
+  for (int64_t w = gtid; w < words; w += gsize) succ[w] = 0;
+  if (gtid == 0) {
+    queue[0] = source;
+    *tail = 1;
+  }
+}
+
+__global__ __launch_bounds__(kCThreads) void k_bc_level(const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ nbr, int64_t qb, int64_t qe,
+                                                        int32_t level, int32_t* __restrict__ depth,
+                                                        unsigned long long* __restrict__ pc,
+                                                        unsigned long long* __restrict__ succ,
+                                                        int32_t* __restrict__ queue,
+                                                        unsigned long long* __restrict__ tail) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kCThreads) >> 6;
+  for (int64_t i = qb + wave; i < qe; i += nwaves) {
+    const int32_t u = queue[i];
+    const int64_t s = off[u], e = off[u + 1];
+    const unsigned long long cu = pc[u];
+    for (int64_t k0 = s; k0 < e; k0 += 64) {
+      const int64_t k = k0 + lane;
+      bool is_succ = false;
+      if (k < e) {
+        const int32_t v = nbr[k];
+        int32_t dv = depth[v];
+        if (dv == -1) {
+          const int32_t old = atomicCAS(&depth[v], -1, level);
+          if (old == -1) {
+            queue[atomicAdd(tail, 1ull)] = v;
+            dv = level;
+          } else {
+            dv = old;
+          }
+        }
+        if (dv == level) {
+          is_succ = true;
+          atomicAdd(&pc[v], cu);
+        }
+      }
+      const unsigned long long m = __ballot(is_succ);
+      if (lane == 0 && m) {
+        const int sh = static_cast<int>(k0 & 63);
+        atomicOr(&succ[k0 >> 6], m << sh);
+        if (sh && (m >> (64 - sh))) atomicOr(&succ[(k0 >> 6) + 1], m >> (64 - sh));
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts, uint64_t* succ,
+               int32_t* num_levels) {
+  const int64_t n = ctx->nloc;
+  const int64_t m = ctx->m_out;
+  const int64_t words = (m + 63) / 64;
+  int rc;
+  if (!ctx->d_bc_depth) {
+    if ((rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_depth), sizeof(int32_t) * n, "BC depths")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_pc), sizeof(uint64_t) * n, "BC path counts")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_succ), sizeof(uint64_t) * std::max<int64_t>(1, words),
+                        "BC successor bitmap")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_queue), sizeof(int32_t) * n, "BC queue")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_tail), sizeof(uint64_t), "BC queue tail")))
+      return rc;
+  }
+  hipStream_t s = ctx->stream;
+  int32_t* d_depth = ctx->d_bc_depth;
+  unsigned long long* d_pc = reinterpret_cast<unsigned long long*>(ctx->d_bc_pc);
+  unsigned long long* d_succ = reinterpret_cast<unsigned long long*>(ctx->d_bc_succ);
+  unsigned long long* d_tail = reinterpret_cast<unsigned long long*>(ctx->d_bc_tail);
+  hipLaunchKernelGGL(k_bc_init, dim3(grid_of(std::max(n, words))), dim3(kCThreads), 0, s, n, words, source, d_depth,
+                     d_pc, d_succ, ctx->d_bc_queue, d_tail);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  int64_t qb = 0, qe = 1;
+  int32_t level = 1;
+  while (qb < qe) {
+    // Each queued vertex appears once, so the queue never passes n entries.
+    const int64_t waves = qe - qb;
+    hipLaunchKernelGGL(k_bc_level, dim3(grid_of(waves * 64)), dim3(kCThreads), 0, s, ctx->d_out_off,
+                       ctx->d_out_nbr, qb, qe, level, d_depth, d_pc, d_succ, ctx->d_bc_queue, d_tail);
+    DMM_HIP_CHECK(ctx, hipGetLastError());
+    unsigned long long t = 0;
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(&t, d_tail, sizeof(t), hipMemcpyDeviceToHost, s));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (static_cast<int64_t>(t) > n) return set_err(ctx, DMM_ERR_STATE, "BC queue passed %lld entries", (long long)n);
+    qb = qe;
+    qe = static_cast<int64_t>(t);
+    level++;
+  }
+  if (num_levels) *num_levels = level - 1;
+  if (depth) DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, d_depth, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  if (path_counts)
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(path_counts, d_pc, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+  if (succ && words > 0)
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(succ, d_succ, sizeof(uint64_t) * words, hipMemcpyDeviceToHost, s));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  return DMM_OK;
+}
+
+}  // namespace dmm

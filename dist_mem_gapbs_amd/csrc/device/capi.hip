@@ -12,6 +12,7 @@
 
 #include "../host/error.h"
 #include "batch.h"
+#include "bc.h"
 #include "bfs.h"
 #include "build.h"
 #include "common.h"
@@ -90,7 +91,8 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
                   ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
-                  ctx->d_piece_part, ctx->d_bsegs, ctx->d_head};
+                  ctx->d_piece_part, ctx->d_bsegs, ctx->d_head,
+                  ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -102,6 +104,8 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_qoff = nullptr;
   ctx->d_bm[0] = ctx->d_bm[1] = ctx->d_bm[2] = nullptr;
   ctx->d_head = nullptr;
+  ctx->d_bc_depth = ctx->d_bc_queue = nullptr;
+  ctx->d_bc_pc = ctx->d_bc_succ = ctx->d_bc_tail = nullptr;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
   ctx->d_heavy = nullptr;
@@ -387,6 +391,20 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+int dmm_bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts, uint64_t* succ,
+                   int32_t* num_levels) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (ctx->nloc <= 0) return set_err(ctx, DMM_ERR_STATE, "no graph uploaded");
+  if (ctx->relabeled)
+    return set_err(ctx, DMM_ERR_STATE, "dmm_bc_forward needs DMM_LAYOUT_NATURAL (edge positions are the caller's)");
+  if (ctx->vbegin != 0 || ctx->vend != ctx->n)
+    return set_err(ctx, DMM_ERR_STATE, "dmm_bc_forward runs on a whole-graph upload");
+  if (source < 0 || source >= ctx->n)
+    return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return dmm::bc_forward(ctx, source, depth, path_counts, succ, num_levels);
 }
 
 int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs, int64_t num_nodes,

@@ -156,6 +156,12 @@ struct dmm_ctx {
   int64_t* d_qoff = nullptr;     // nloc + 1 (window-relative degree prefix)
   uint64_t* d_bm[3] = {nullptr, nullptr, nullptr};  // words each
   int32_t* d_head = nullptr;  // first kHeadNbrs in-neighbours of each row, -1 padded
+  // Brandes' forward phase (allocated at its first call).
+  int32_t* d_bc_depth = nullptr;
+  uint64_t* d_bc_pc = nullptr;
+  uint64_t* d_bc_succ = nullptr;
+  int32_t* d_bc_queue = nullptr;
+  uint64_t* d_bc_tail = nullptr;
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed
   void* h_params = nullptr;      // pinned staging for d_params, mapped: the device loop's InitParent reads it

@@ -68,6 +68,17 @@ class Context:
                                                      1 if symmetrize else 0, C.byref(h)))
         return HostGraph(h)
 
+    def bc_forward(self, source):
+        """Brandes' forward phase (src/bc.cc:55-96) from source on the uploaded
+        whole graph (natural layout): (depth, path_counts, succ words, levels)."""
+        depth = np.empty(self.nloc, np.int32)
+        pc = np.empty(self.nloc, np.uint64)
+        succ = np.empty(max(1, (self.m_out + 63) // 64), np.uint64)
+        levels = C.c_int32()
+        self._check(self._lib.dmm_bc_forward(self._h, int(source), L.ptr(depth, C.c_int32), L.ptr(pc, C.c_uint64),
+                                             L.ptr(succ, C.c_uint64), C.byref(levels)))
+        return depth, pc, succ[:(self.m_out + 63) // 64], levels.value
+
     def upload(self, g=None, *, num_nodes=None, out_offsets=None, out_neighs=None,
                in_offsets=None, in_neighs=None, vbegin=0, vend=None, num_edges_directed=-1):
         """Uploads a HostGraph (whole graph) or explicit block arrays."""
@@ -88,6 +99,7 @@ class Context:
             L.ptr(io, C.c_int64), L.ptr(inn, C.c_int32), num_edges_directed))
         self.n, self.vbegin, self.vend = num_nodes, vbegin, vend
         self.words = (num_nodes + 63) // 64
+        self.m_out = int(oo[-1])
 
     def set_layout(self, layout):
         """L.DMM_LAYOUT_DEGREE (default) or L.DMM_LAYOUT_NATURAL; applies to

@@ -152,6 +152,17 @@ void dmm_ctx_destroy(dmm_ctx* ctx);
 int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs,
                            int64_t num_nodes, int32_t symmetrize, dmm_host_graph** out);
 
+/* Replaces shmem_BFS, Brandes' forward phase (src/bc.cc:55-96), on the
+ * uploaded whole graph from `source`: depth (int32[N], -1 = unreachable),
+ * path_counts (uint64[N], the number of shortest paths, CountT of
+ * src/bc.cc:51), succ (uint64[(num_edges_directed + 63) / 64], bit
+ * out_offsets[u] + k set iff the k-th out-neighbour v of u has depth[v] ==
+ * depth[u] + 1, src/bc.cc:80-83) and num_levels (deepest depth + 1).  Any
+ * output may be NULL.  Needs DMM_LAYOUT_NATURAL (the successor bits index the
+ * caller's edge positions); else DMM_ERR_STATE. */
+int dmm_bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts,
+                   uint64_t* succ, int32_t* num_levels);
+
 /* Uploads one PE's share of a CSR graph (replaces the symmetric-heap CSR of
  * src/builder.h:236-288 and the remote Neighborhood fetch of
  * src/graph.h:138-162).

@@ -567,3 +567,46 @@ int32_t orc_dmm_spmd_bfs(int64_t n, const int64_t* out_offs, const int32_t* out_
   free(th);
   return 0;
 }
+
+/* src/bc.cc:55-96, one PE, serially: a level-ordered queue; every edge of a
+ * frontier vertex u claims an unvisited v for the next level and, when v
+ * lies on that level, marks the edge a successor and adds u's path count to
+ * v's.  Depths, counts and successor bits do not depend on the order in
+ * which a level's vertices or edges are visited. */
+int32_t orc_bc_forward(int64_t n, const int64_t* out_offs, const int32_t* out_neigh, int32_t source,
+                       int32_t* depth, uint64_t* path_counts, uint64_t* succ) {
+  const int64_t words = (out_offs[n] + 63) / 64;
+  for (int64_t v = 0; v < n; v++) {
+    depth[v] = -1;
+    path_counts[v] = 0;
+  }
+  for (int64_t w = 0; w < words; w++) succ[w] = 0;
+  if (source < 0 || source >= n) return 0;
+  int32_t* queue = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  int64_t qb = 0, qe = 0;
+  depth[source] = 0;
+  path_counts[source] = 1;
+  queue[qe++] = source;
+  int32_t levels = 0;
+  while (qb < qe) {
+    const int64_t level_end = qe;
+    levels++;
+    for (; qb < level_end; qb++) {
+      const int32_t u = queue[qb];
+      const int32_t d = depth[u] + 1;
+      for (int64_t e = out_offs[u]; e < out_offs[u + 1]; e++) {
+        const int32_t v = out_neigh[e];
+        if (depth[v] == -1) {
+          depth[v] = d;
+          queue[qe++] = v;
+        }
+        if (depth[v] == d) {
+          succ[e >> 6] |= 1ull << (e & 63);
+          path_counts[v] += path_counts[u];
+        }
+      }
+    }
+  }
+  free(queue);
+  return levels;
+}

@@ -52,6 +52,15 @@ int32_t orc_dobfs(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,
 void orc_bfs_depth(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,
                    int32_t source, int32_t* depth);
 
+/* Brandes' forward phase, the BFS of shmem_BFS (src/bc.cc:55-96): depths
+ * (-1 = unreachable), shortest-path counts (CountT = unsigned long long,
+ * src/bc.cc:51) and the successor bitmap over edge positions (bit
+ * out_offs[u] + k is set iff the k-th out-neighbour v of u has depth[v] ==
+ * depth[u] + 1; src/bc.cc:80-83).  succ holds (out_offs[n] + 63) / 64 words.
+ * Returns the number of levels (the deepest depth + 1, 0 for no source). */
+int32_t orc_bc_forward(int64_t n, const int64_t* out_offs, const int32_t* out_neigh, int32_t source,
+                       int32_t* depth, uint64_t* path_counts, uint64_t* succ);
+
 /* BFSVerifier: gapbs-verify/bfs.cc:195-241.  Returns 1 on PASS; on FAIL
  * returns 0 and stores the failing vertex in *bad (if non-NULL). */
 int32_t orc_bfs_verify(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,

@@ -41,6 +41,9 @@ def lib():
         l.orc_dobfs.argtypes = [C.c_int64, _P64, _P32, _P64, _P32, C.c_int64, C.c_int32, C.c_int32,
                                 C.c_int32, C.c_int64, _P32, _P32, _P64, C.c_int32]
         l.orc_bfs_depth.argtypes = [C.c_int64, _P64, _P32, C.c_int32, _P32]
+        l.orc_bc_forward.restype = C.c_int32
+        l.orc_bc_forward.argtypes = [C.c_int64, _P64, _P32, C.c_int32, _P32, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint64)]
         l.orc_bfs_verify.restype = C.c_int32
         l.orc_bfs_verify.argtypes = [C.c_int64, _P64, _P32, _P64, _P32, C.c_int32, _P32, _P64]
         l.orc_pr_pull_f32.restype = C.c_int32
@@ -107,6 +110,18 @@ def bfs_depth(g, source):
     d = np.empty(g.n, np.int32)
     lib().orc_bfs_depth(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), source, _p(d, C.c_int32))
     return d
+
+
+def bc_forward(g, source):
+    """Brandes' forward phase (src/bc.cc:55-96): (depth int32, path_counts
+    uint64, succ uint64 words over edge positions, levels)."""
+    d = np.empty(g.n, np.int32)
+    pc = np.empty(g.n, np.uint64)
+    succ = np.zeros(max(1, (int(g.oo[-1]) + 63) // 64), np.uint64)
+    levels = lib().orc_bc_forward(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32), source, _p(d, C.c_int32),
+                                  pc.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                  succ.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return d, pc, succ[:(int(g.oo[-1]) + 63) // 64], levels
 
 
 def bfs_verify(g, source, parent):
