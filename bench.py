@@ -24,7 +24,10 @@ Also reported on the same line:
 Synthetic graphs from the reference generator (no datasets).
 
 Other modes: --only {bfs,batch,pr} runs one Kronecker --big-scale leg (for
-profiling); --pe-share runs one PE's share of Kronecker --pe-scale (PE 3 of
+profiling); --only bc times Brandes' forward phase from --steps sources
+beside the oracle's serial forward phase (one core), and --only build the GPU
+CSR build beside the host builder, each checking the results agree;
+--pe-share runs one PE's share of Kronecker --pe-scale (PE 3 of
 np = 8), timing the block kernels against whole-graph inputs and checking
 every block result against the whole-graph result; --calibrate-copy adds one
 256 MiB elementwise kernel as the reference for the FETCH_SIZE / WRITE_SIZE
@@ -308,9 +311,70 @@ def pe_share_leg(dmm, args):
     print(json.dumps(out), flush=True)
 
 
+def build_leg(dmm, args):
+    """CSR build of the Kronecker --big-scale edge list: the host Builder
+    against the GPU build (dmm_build_graph_device, both PCIe transfers
+    included), with the two results compared array by array."""
+    pairs = dmm.generate_edgelist(args.big_scale, 16, False)
+    out = {"leg": "build", "scale": args.big_scale, "pairs": int(len(pairs))}
+    t0 = time.perf_counter()
+    host = dmm.from_edgelist(pairs)
+    out["host_build_s"] = time.perf_counter() - t0
+    out["host_threads"] = os.cpu_count()
+    with dmm.Context(0) as ctx:
+        ctx.build_graph(pairs[:1 << 16])  # first-call set-up outside the timing
+        t0 = time.perf_counter()
+        dev = ctx.build_graph(pairs)
+        out["device_build_s"] = time.perf_counter() - t0
+    out["identical"] = bool(np.array_equal(host.out_offsets, dev.out_offsets)
+                            and np.array_equal(host.out_neighs, dev.out_neighs))
+    out["num_edges_directed"] = int(host.num_edges_directed)
+    print(json.dumps(out), flush=True)
+
+
+def bc_leg(dmm, args):
+    """Brandes' forward phase (dmm_bc_forward) from --steps sources on the
+    Kronecker --big-scale graph, natural layout: edges of reached vertices per
+    second, and the oracle's serial forward phase timed on the same sources as
+    the one-core CPU reference (results compared bit for bit)."""
+    import dist_mem_gapbs_amd._lib as L
+    from oracle import oracle as orc
+    g = dmm.build_graph(args.big_scale, 16, uniform=False)
+    og = orc.CSR.of(g)
+    srcs = [int(s) for s in dmm.pick_sources(g, args.steps)]
+    deg = np.diff(g.out_offsets)
+    out = {"leg": "bc", "scale": args.big_scale, "sources": len(srcs)}
+    with dmm.Context(0) as ctx:
+        ctx.set_layout(L.DMM_LAYOUT_NATURAL)
+        ctx.upload(g)
+        ctx.bc_forward(srcs[0])
+        edges, t_gpu, same = 0, 0.0, True
+        t_cpu = 0.0
+        for s in srcs:
+            t0 = time.perf_counter()
+            d, pc, succ, levels = ctx.bc_forward(s)
+            t_gpu += time.perf_counter() - t0
+            edges += int(deg[d >= 0].sum())
+            t0 = time.perf_counter()
+            rd, rpc, rsucc, rlevels = orc.bc_forward(og, s)
+            t_cpu += time.perf_counter() - t0
+            same = same and np.array_equal(d, rd) and np.array_equal(pc, rpc) and np.array_equal(succ, rsucc)
+    out["gpu_ms_per_source"] = 1e3 * t_gpu / len(srcs)
+    out["gpu_gteps"] = edges / t_gpu / 1e9
+    out["cpu_port_ms_per_source"] = 1e3 * t_cpu / len(srcs)
+    out["cpu_port_gteps"] = edges / t_cpu / 1e9
+    out["cpu_port_cores"] = 1
+    out["bit_exact_vs_oracle"] = bool(same)
+    print(json.dumps(out), flush=True)
+
+
 def only_leg(dmm, args):
     """One leg on Kronecker --big-scale, printed as JSON (profiling runs)."""
     import dist_mem_gapbs_amd._lib as L
+    if args.only == "build":
+        return build_leg(dmm, args)
+    if args.only == "bc":
+        return bc_leg(dmm, args)
     g = dmm.build_graph(args.big_scale, 16, uniform=False)
     srcs = dmm.pick_sources(g, 64)
     out = {"leg": args.only, "scale": args.big_scale}
@@ -354,7 +418,7 @@ def main():
     ap.add_argument("--pe-share", action="store_true",
                     help="run only the one-PE-share leg (Kronecker --pe-scale, PE 3 of 8)")
     ap.add_argument("--pe-scale", type=int, default=27)
-    ap.add_argument("--only", choices=["bfs", "batch", "pr"],
+    ap.add_argument("--only", choices=["bfs", "batch", "pr", "bc", "build"],
                     help="run one leg alone on Kronecker --big-scale (for profiling)")
     args = ap.parse_args()
 
