@@ -2,20 +2,21 @@
 // level-synchronous top-down BFS that also counts shortest paths and marks
 // successor edges.
 //
-// Level L (frontier = queue[qb, qe), depth L-1): one wavefront per frontier
-// vertex u, its lanes striding u's out-edges in 64-edge rounds.  For edge
-// k = (u, v):
+// Level L (frontier = queue[qb, qe), depth L-1) is edge-balanced: the
+// frontier's out-degrees are summed (exclusive prefix qoff, hipCUB), and
+// lane g takes the g-th frontier edge, finding its vertex by a binary search
+// of qoff, so a hub's edges spread over the whole grid.  For edge k = (u, v):
 //   - an unvisited v (plain read -1) is claimed with a CAS to L, and the
 //     winner appends v to the queue (src/bc.cc:77-79); a failed CAS returns
 //     v's depth, so no stale read decides the next test;
-//   - if depth(v) == L the edge is a successor and v's path count grows by
-//     u's (src/bc.cc:80-83).  u's count is final: every addition to it came
-//     from the previous level's launch.
-// Successor bits of a 64-edge round come from one ballot and go to at most
-// two bitmap words (two atomicOr from lane 0), not one atomic per edge.
+//   - if depth(v) == L the edge is a successor: v's path count grows by u's
+//     and the edge's bit is set (src/bc.cc:80-83).  u's count is final:
+//     every addition to it came from the previous level's launch.
 // Depths, counts and bits do not depend on the order of the atomics, so they
 // equal the serial restatement bit for bit; only the queue order inside a
 // level does.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 
 #include "bc.h"
@@ -50,46 +51,57 @@ __global__ __launch_bounds__(kCThreads) void k_bc_init(int64_t n, int64_t words,
   }
 }
 
-__global__ __launch_bounds__(kCThreads) void k_bc_level(const int64_t* __restrict__ off,
-                                                        const int32_t* __restrict__ nbr, int64_t qb, int64_t qe,
-                                                        int32_t level, int32_t* __restrict__ depth,
-                                                        unsigned long long* __restrict__ pc,
-                                                        unsigned long long* __restrict__ succ,
-                                                        int32_t* __restrict__ queue,
-                                                        unsigned long long* __restrict__ tail) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kCThreads) >> 6;
-  for (int64_t i = qb + wave; i < qe; i += nwaves) {
-    const int32_t u = queue[i];
-    const int64_t s = off[u], e = off[u + 1];
-    const unsigned long long cu = pc[u];
-    for (int64_t k0 = s; k0 < e; k0 += 64) {
-      const int64_t k = k0 + lane;
-      bool is_succ = false;
-      if (k < e) {
-        const int32_t v = nbr[k];
-        int32_t dv = depth[v];
-        if (dv == -1) {
-          const int32_t old = atomicCAS(&depth[v], -1, level);
-          if (old == -1) {
-            queue[atomicAdd(tail, 1ull)] = v;
-            dv = level;
-          } else {
-            dv = old;
-          }
-        }
-        if (dv == level) {
-          is_succ = true;
-          atomicAdd(&pc[v], cu);
-        }
+// Degrees of the frontier queue[qb, qe), and a 0 after them (the input of
+// the exclusive prefix).
+__global__ __launch_bounds__(kCThreads) void k_bc_degrees(const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ queue, int64_t qb, int64_t qe,
+                                                          int64_t* __restrict__ qdeg) {
+  const int64_t gsize = int64_t(gridDim.x) * kCThreads;
+  for (int64_t i = int64_t(blockIdx.x) * kCThreads + threadIdx.x; i <= qe - qb; i += gsize) {
+    int64_t d = 0;  // entry F is 0, so the exclusive sum's last entry is the total
+    if (i < qe - qb) {
+      const int32_t u = queue[qb + i];
+      d = off[u + 1] - off[u];
+    }
+    qdeg[i] = d;
+  }
+}
+
+// Edge-balanced level: lane g takes the g-th out-edge of the frontier, its
+// vertex found by a binary search of qoff (exclusive degree prefix, qoff[F]
+// = T).  Successor bits take one atomicOr per successor edge (consecutive g
+// may cross vertices, so a wave's edge positions are not contiguous).
+__global__ __launch_bounds__(kCThreads) void k_bc_level_edges(const int64_t* __restrict__ off,
+                                                              const int32_t* __restrict__ nbr,
+                                                              int32_t* __restrict__ queue, int64_t qb,
+                                                              int64_t F, const int64_t* __restrict__ qoff,
+                                                              int64_t T, int32_t level, int32_t* __restrict__ depth,
+                                                              unsigned long long* __restrict__ pc,
+                                                              unsigned long long* __restrict__ succ,
+                                                              unsigned long long* __restrict__ tail) {
+  const int64_t gsize = int64_t(gridDim.x) * kCThreads;
+  for (int64_t g = int64_t(blockIdx.x) * kCThreads + threadIdx.x; g < T; g += gsize) {
+    int64_t lo = 0, hi = F;  // qoff[lo] <= g < qoff[hi]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (qoff[mid] <= g) lo = mid; else hi = mid;
+    }
+    const int32_t u = queue[qb + lo];
+    const int64_t k = off[u] + (g - qoff[lo]);
+    const int32_t v = nbr[k];
+    int32_t dv = depth[v];
+    if (dv == -1) {
+      const int32_t old = atomicCAS(&depth[v], -1, level);
+      if (old == -1) {
+        queue[atomicAdd(tail, 1ull)] = v;
+        dv = level;
+      } else {
+        dv = old;
       }
-      const unsigned long long m = __ballot(is_succ);
-      if (lane == 0 && m) {
-        const int sh = static_cast<int>(k0 & 63);
-        atomicOr(&succ[k0 >> 6], m << sh);
-        if (sh && (m >> (64 - sh))) atomicOr(&succ[(k0 >> 6) + 1], m >> (64 - sh));
-      }
+    }
+    if (dv == level) {
+      atomicAdd(&pc[v], pc[u]);
+      atomicOr(&succ[k >> 6], 1ull << (k & 63));
     }
   }
 }
@@ -108,8 +120,16 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_succ), sizeof(uint64_t) * std::max<int64_t>(1, words),
                         "BC successor bitmap")) ||
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_queue), sizeof(int32_t) * n, "BC queue")) ||
-        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_tail), sizeof(uint64_t), "BC queue tail")))
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_tail), sizeof(uint64_t), "BC queue tail")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_qdeg), sizeof(int64_t) * (n + 1), "BC degrees")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_qoff), sizeof(int64_t) * (n + 1), "BC prefix")))
       return rc;
+    // Scan scratch sized for the largest frontier (n vertices plus the 0).
+    size_t bytes = 0;
+    DMM_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->d_bc_qdeg, ctx->d_bc_qoff, n + 1,
+                                                        ctx->stream));
+    if ((rc = dev_alloc(ctx, &ctx->d_bc_temp, std::max<size_t>(1, bytes), "BC scan scratch"))) return rc;
+    ctx->bc_temp_bytes = bytes;
   }
   hipStream_t s = ctx->stream;
   int32_t* d_depth = ctx->d_bc_depth;
@@ -123,10 +143,20 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
   int32_t level = 1;
   while (qb < qe) {
     // Each queued vertex appears once, so the queue never passes n entries.
-    const int64_t waves = qe - qb;
-    hipLaunchKernelGGL(k_bc_level, dim3(grid_of(waves * 64)), dim3(kCThreads), 0, s, ctx->d_out_off,
-                       ctx->d_out_nbr, qb, qe, level, d_depth, d_pc, d_succ, ctx->d_bc_queue, d_tail);
+    const int64_t F = qe - qb;
+    hipLaunchKernelGGL(k_bc_degrees, dim3(grid_of(F + 1)), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_bc_queue,
+                       qb, qe, ctx->d_bc_qdeg);
     DMM_HIP_CHECK(ctx, hipGetLastError());
+    size_t bytes = ctx->bc_temp_bytes;
+    DMM_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->d_bc_temp, bytes, ctx->d_bc_qdeg, ctx->d_bc_qoff, F + 1, s));
+    int64_t T = 0;
+    DMM_HIP_CHECK(ctx, hipMemcpyAsync(&T, ctx->d_bc_qoff + F, sizeof(T), hipMemcpyDeviceToHost, s));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (T > 0) {
+      hipLaunchKernelGGL(k_bc_level_edges, dim3(grid_of(T)), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
+                         ctx->d_bc_queue, qb, F, ctx->d_bc_qoff, T, level, d_depth, d_pc, d_succ, d_tail);
+      DMM_HIP_CHECK(ctx, hipGetLastError());
+    }
     unsigned long long t = 0;
     DMM_HIP_CHECK(ctx, hipMemcpyAsync(&t, d_tail, sizeof(t), hipMemcpyDeviceToHost, s));
     DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));

@@ -92,7 +92,8 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
                   ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
                   ctx->d_piece_part, ctx->d_bsegs, ctx->d_head,
-                  ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail};
+                  ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
+                  ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -106,6 +107,9 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_head = nullptr;
   ctx->d_bc_depth = ctx->d_bc_queue = nullptr;
   ctx->d_bc_pc = ctx->d_bc_succ = ctx->d_bc_tail = nullptr;
+  ctx->d_bc_qdeg = ctx->d_bc_qoff = nullptr;
+  ctx->d_bc_temp = nullptr;
+  ctx->bc_temp_bytes = 0;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
   ctx->d_heavy = nullptr;

@@ -162,6 +162,10 @@ struct dmm_ctx {
   uint64_t* d_bc_succ = nullptr;
   int32_t* d_bc_queue = nullptr;
   uint64_t* d_bc_tail = nullptr;
+  int64_t* d_bc_qdeg = nullptr;  // frontier degrees, then their exclusive prefix
+  int64_t* d_bc_qoff = nullptr;
+  void* d_bc_temp = nullptr;     // scan scratch for n + 1 entries
+  size_t bc_temp_bytes = 0;
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed
   void* h_params = nullptr;      // pinned staging for d_params, mapped: the device loop's InitParent reads it
