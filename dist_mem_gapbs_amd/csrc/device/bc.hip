@@ -79,13 +79,32 @@ __global__ __launch_bounds__(kCThreads) void k_bc_level_edges(const int64_t* __r
                                                               unsigned long long* __restrict__ pc,
                                                               unsigned long long* __restrict__ succ,
                                                               unsigned long long* __restrict__ tail) {
-  const int64_t gsize = int64_t(gridDim.x) * kCThreads;
-  for (int64_t g = int64_t(blockIdx.x) * kCThreads + threadIdx.x; g < T; g += gsize) {
-    int64_t lo = 0, hi = F;  // qoff[lo] <= g < qoff[hi]
+  // A wavefront takes 64 consecutive edges [base, base + 64): lanes 0 and
+  // 63 search the whole prefix for the first and last of them (broadcast by
+  // shuffle), and every lane searches only between the two.  The search
+  // finds the last entry with qoff <= g, which passes over zero-degree
+  // frontier vertices.
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kCThreads) >> 6;
+  for (int64_t base = wave * 64; base < T; base += nwaves * 64) {
+    const int64_t g = base + lane;
+    const int64_t ge = min(g, T - 1);
+    int64_t lo = 0, hi = F;  // qoff[lo] <= ge < qoff[hi]
+    if (lane == 0 || lane == 63) {
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= ge) lo = mid; else hi = mid;
+      }
+    }
+    const int64_t first = __shfl(lo, 0), last = __shfl(lo, 63);
+    lo = first;
+    hi = last + 1;
     while (hi - lo > 1) {
       const int64_t mid = (lo + hi) >> 1;
-      if (qoff[mid] <= g) lo = mid; else hi = mid;
+      if (qoff[mid] <= ge) lo = mid; else hi = mid;
     }
+    if (g >= T) continue;
     const int32_t u = queue[qb + lo];
     const int64_t k = off[u] + (g - qoff[lo]);
     const int32_t v = nbr[k];
