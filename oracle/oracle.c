@@ -292,6 +292,137 @@ int32_t orc_pr_pull_f64(int64_t n, const int64_t* out_offs, const int64_t* in_of
   return iter;
 }
 
+/* orc_pr_pull_f32 / orc_pr_pull_f64 with the vertex loops split over
+ * threads.  Each vertex's sum visits its in-neighbours in the same order as
+ * the serial loop, and the L1 error is summed serially in vertex order from
+ * per-vertex terms, so the scores, the error trace and the stop iteration
+ * are bit-identical to the serial functions for any thread count. */
+typedef struct {
+  int64_t lo, hi;
+  const int64_t* out_offs;
+  const int64_t* in_offs;
+  const int32_t* in_neigh;
+  void* scores;  /* float* or double* */
+  void* contrib; /* same type as scores */
+  double* diff;
+  double base, damp;
+  int phase; /* 0: contributions, 1: sums */
+  int fp64;
+} pr_part;
+
+static void* pr_worker(void* arg) {
+  pr_part* p = (pr_part*)arg;
+  if (p->fp64) {
+    double* sc = (double*)p->scores;
+    double* co = (double*)p->contrib;
+    if (p->phase == 0) {
+      for (int64_t v = p->lo; v < p->hi; v++) co[v] = sc[v] / (double)out_degree(p->out_offs, v);
+    } else {
+      for (int64_t u = p->lo; u < p->hi; u++) {
+        double incoming_total = 0;
+        for (int64_t e = p->in_offs[u]; e < p->in_offs[u + 1]; e++) incoming_total += co[p->in_neigh[e]];
+        double old_score = sc[u];
+        sc[u] = p->base + p->damp * incoming_total;
+        p->diff[u] = fabs(sc[u] - old_score);
+      }
+    }
+  } else {
+    float* sc = (float*)p->scores;
+    float* co = (float*)p->contrib;
+    const float base = (float)p->base, damp = (float)p->damp;
+    if (p->phase == 0) {
+      for (int64_t v = p->lo; v < p->hi; v++) co[v] = sc[v] / (float)out_degree(p->out_offs, v);
+    } else {
+      for (int64_t u = p->lo; u < p->hi; u++) {
+        float incoming_total = 0;
+        for (int64_t e = p->in_offs[u]; e < p->in_offs[u + 1]; e++) incoming_total += co[p->in_neigh[e]];
+        float old_score = sc[u];
+        sc[u] = base + damp * incoming_total;
+        p->diff[u] = fabsf(sc[u] - old_score);
+      }
+    }
+  }
+  return NULL;
+}
+
+static int32_t pr_pull_mt(int64_t n, const int64_t* out_offs, const int64_t* in_offs,
+                          const int32_t* in_neigh, int32_t max_iters, double epsilon, int fp64,
+                          double base_score, double kDamp, int32_t nthreads, void* scores,
+                          double* err_trace) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  size_t cap = (size_t)(n > 0 ? n : 1);
+  void* contrib = malloc(cap * (fp64 ? sizeof(double) : sizeof(float)));
+  double* diff = (double*)malloc(cap * sizeof(double));
+  pthread_t tid[256];
+  pr_part part[256];
+  int64_t m = in_offs[n];
+  /* Thread ranges balanced by rows + in-edges. */
+  int64_t bounds[257];
+  bounds[0] = 0;
+  for (int t = 1; t < nthreads; t++) {
+    int64_t target = (n + m) / nthreads * t, lo = bounds[t - 1], hi = n;
+    while (lo < hi) {
+      int64_t mid = lo + (hi - lo) / 2;
+      if (mid + in_offs[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    bounds[t] = lo;
+  }
+  bounds[nthreads] = n;
+  int32_t iter;
+  for (iter = 0; iter < max_iters; iter++) {
+    for (int phase = 0; phase < 2; phase++) {
+      for (int t = 0; t < nthreads; t++) {
+        pr_part q = {bounds[t], bounds[t + 1], out_offs, in_offs, in_neigh, scores, contrib, diff,
+                     base_score, kDamp, phase, fp64};
+        part[t] = q;
+        pthread_create(&tid[t], NULL, pr_worker, &part[t]);
+      }
+      for (int t = 0; t < nthreads; t++) pthread_join(tid[t], NULL);
+    }
+    double error = 0;
+    for (int64_t u = 0; u < n; u++) error += diff[u];
+    if (err_trace) err_trace[iter] = error;
+    if (error < epsilon) {
+      iter++;
+      break;
+    }
+  }
+  free(contrib);
+  free(diff);
+  return iter;
+}
+
+int32_t orc_pr_pull_f32_mt(int64_t n, const int64_t* out_offs, const int64_t* in_offs,
+                           const int32_t* in_neigh, int32_t max_iters, double epsilon,
+                           int32_t nthreads, float* scores, double* err_trace) {
+  const float kDamp = 0.85f;
+  const float init_score = 1.0f / (float)n;
+  const float base_score = (1.0f - kDamp) / (float)n;
+  for (int64_t v = 0; v < n; v++) scores[v] = init_score;
+  /* base and damping are FP32 values; the double carriers hold them exactly. */
+  return pr_pull_mt(n, out_offs, in_offs, in_neigh, max_iters, epsilon, 0, (double)base_score,
+                    (double)kDamp, nthreads, scores, err_trace);
+}
+
+int32_t orc_pr_pull_f64_mt(int64_t n, const int64_t* out_offs, const int64_t* in_offs,
+                           const int32_t* in_neigh, int32_t max_iters, double epsilon,
+                           int32_t exact_constants, int32_t nthreads, double* scores, double* err_trace) {
+  double kDamp, init_score, base_score;
+  if (exact_constants) {
+    kDamp = 0.85;
+    init_score = 1.0 / (double)n;
+    base_score = (1.0 - 0.85) / (double)n;
+  } else {
+    kDamp = (double)0.85f;
+    init_score = (double)(1.0f / (float)n);
+    base_score = (double)((1.0f - 0.85f) / (float)n);
+  }
+  for (int64_t v = 0; v < n; v++) scores[v] = init_score;
+  return pr_pull_mt(n, out_offs, in_offs, in_neigh, max_iters, epsilon, 1, base_score, kDamp, nthreads,
+                    scores, err_trace);
+}
+
 /* gapbs-verify/pr.cc:77-93. */
 double orc_pr_residual_f32(int64_t n, const int64_t* out_offs, const int32_t* out_neigh,
                            const float* scores) {

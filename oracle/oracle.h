@@ -82,6 +82,16 @@ int32_t orc_pr_pull_f64(int64_t n, const int64_t* out_offs, const int64_t* in_of
                         const int32_t* in_neigh, int32_t max_iters, double epsilon,
                         int32_t exact_constants, double* scores, double* err_trace);
 
+/* orc_pr_pull_f32 / orc_pr_pull_f64 with their vertex loops on nthreads
+ * threads; bit-identical results (same per-vertex order, error summed
+ * serially in vertex order). */
+int32_t orc_pr_pull_f32_mt(int64_t n, const int64_t* out_offs, const int64_t* in_offs,
+                           const int32_t* in_neigh, int32_t max_iters, double epsilon,
+                           int32_t nthreads, float* scores, double* err_trace);
+int32_t orc_pr_pull_f64_mt(int64_t n, const int64_t* out_offs, const int64_t* in_offs,
+                           const int32_t* in_neigh, int32_t max_iters, double epsilon,
+                           int32_t exact_constants, int32_t nthreads, double* scores, double* err_trace);
+
 /* Residual verifier (the push-direction check, gapbs-verify/pr.cc:77-93):
  * one serial push iteration from `scores`; returns sum_n |base + d *
  * incoming[n] - scores[n]| (FP32 terms, FP64 sum). */

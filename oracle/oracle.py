@@ -51,6 +51,12 @@ def lib():
         l.orc_pr_pull_f64.restype = C.c_int32
         l.orc_pr_pull_f64.argtypes = [C.c_int64, _P64, _P64, _P32, C.c_int32, C.c_double, C.c_int32,
                                       _PD, _PD]
+        l.orc_pr_pull_f32_mt.restype = C.c_int32
+        l.orc_pr_pull_f32_mt.argtypes = [C.c_int64, _P64, _P64, _P32, C.c_int32, C.c_double, C.c_int32,
+                                         _PF, _PD]
+        l.orc_pr_pull_f64_mt.restype = C.c_int32
+        l.orc_pr_pull_f64_mt.argtypes = [C.c_int64, _P64, _P64, _P32, C.c_int32, C.c_double, C.c_int32,
+                                         C.c_int32, _PD, _PD]
         l.orc_pr_residual_f32.restype = C.c_double
         l.orc_pr_residual_f32.argtypes = [C.c_int64, _P64, _P32, _PF]
         l.orc_dmm_spmd_bfs.restype = C.c_int32
@@ -168,6 +174,30 @@ def pr_f64(g, max_iters=20, eps=1e-4, exact_constants=False):
     it = lib().orc_pr_pull_f64(g.n, _p(g.oo, C.c_int64), _p(g.io, C.c_int64), _p(g.inn, C.c_int32),
                                max_iters, eps, 1 if exact_constants else 0, _p(s, C.c_double),
                                _p(tr, C.c_double))
+    return s, it, tr[:it]
+
+
+def _threads(nthreads):
+    return min(16, os.cpu_count() or 1) if nthreads is None else nthreads
+
+
+def pr_f32_mt(g, max_iters=20, eps=1e-4, nthreads=None):
+    """pr_f32 with the vertex loops on threads (bit-identical results)."""
+    s = np.empty(g.n, np.float32)
+    tr = np.zeros(max(1, max_iters), np.float64)
+    it = lib().orc_pr_pull_f32_mt(g.n, _p(g.oo, C.c_int64), _p(g.io, C.c_int64), _p(g.inn, C.c_int32),
+                                  max_iters, eps, _threads(nthreads), _p(s, C.c_float), _p(tr, C.c_double))
+    return s, it, tr[:it]
+
+
+def pr_f64_mt(g, max_iters=20, eps=1e-4, exact_constants=False, nthreads=None):
+    """pr_f64 with the vertex loops on threads (bit-identical results)."""
+    nthreads = _threads(nthreads)
+    s = np.empty(g.n, np.float64)
+    tr = np.zeros(max(1, max_iters), np.float64)
+    it = lib().orc_pr_pull_f64_mt(g.n, _p(g.oo, C.c_int64), _p(g.io, C.c_int64), _p(g.inn, C.c_int32),
+                                  max_iters, eps, 1 if exact_constants else 0, nthreads,
+                                  _p(s, C.c_double), _p(tr, C.c_double))
     return s, it, tr[:it]
 
 

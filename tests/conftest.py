@@ -52,3 +52,48 @@ def ctx(dmm):
 
 def csr_from_pairs(dmm, pairs, n=-1, symmetrize=True):
     return dmm.from_edgelist(np.asarray(pairs, dtype=np.int32).reshape(-1, 2), n, symmetrize)
+
+
+class CapturedStdout:
+    """Captures what C code writes to file descriptor 1 (printf) while the
+    block runs; the text is in .text afterwards."""
+
+    def __enter__(self):
+        import ctypes
+        import tempfile
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        self._tmp = tempfile.TemporaryFile(mode="w+b")
+        self._saved = os.dup(1)
+        os.dup2(self._tmp.fileno(), 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        self._tmp.seek(0)
+        self.text = self._tmp.read().decode()
+        self._tmp.close()
+
+
+def depth_from_parent(parent, source):
+    """Depth of every vertex in a parent tree (-1 where parent < 0), by
+    pointer doubling; raises if the tree does not reach the source."""
+    parent = np.asarray(parent, np.int64)
+    n = parent.shape[0]
+    reached = parent >= 0
+    jump = np.where(reached, parent, np.arange(n))
+    dist = np.where(reached, 1, 0).astype(np.int64)
+    jump[source] = source
+    dist[source] = 0
+    for _ in range(64):
+        if np.all(jump[reached] == source):
+            break
+        dist = dist + dist[jump]
+        jump = jump[jump]
+    else:
+        raise AssertionError("parent tree has a cycle or does not reach the source")
+    return np.where(reached, dist, -1).astype(np.int32)

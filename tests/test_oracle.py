@@ -13,7 +13,7 @@ np = 1..6 on -u and -g graphs, like the reference's test sweep
 import numpy as np
 import pytest
 
-from conftest import csr_from_pairs
+from conftest import CapturedStdout, csr_from_pairs, depth_from_parent
 
 
 def csr(orc, dmm, pairs, n=-1):
@@ -165,18 +165,57 @@ def test_direction_schedule_invariants(orc, dmm, graph):
         assert total == int((d >= 0).sum())
 
 
+_STEP_KIND = {"td": 0, "bu": 1, "e": 2, "c": 3}
+
+
+def reference_trace(rg, source, alpha=15, beta=18):
+    """Runs the reference's DOBFS (oracle/_ref) and parses the step lines it
+    prints through PrintStep (gapbs-verify/bfs.cc:124-166, format of
+    gapbs-verify/util.h:44-49): td -> |F'|, bu -> awake, e / c -> no count."""
+    with CapturedStdout() as cap:
+        parent = rg.dobfs(source, alpha, beta)
+    steps = []
+    for line in cap.text.splitlines():
+        tok = line.split()
+        if not tok or tok[0] not in _STEP_KIND:
+            continue
+        kind = _STEP_KIND[tok[0]]
+        steps.append((kind, int(tok[1]) if kind in (0, 1) else -1))
+    return parent, steps
+
+
 def test_dobfs_trace_matches_reference_prints(orc, dmm, graph):
-    # PrintStep counts: td -> |F'|, bu -> awake; "e"/"c" carry no count.
-    g = orc.CSR.of(graph(12, 8, True))
-    s = int(dmm.pick_sources(graph(12, 8, True), 1)[0])
-    _, tr = orc.dobfs(g, s)
-    assert tr[0][0] in (0, 2)
-    for k, c in tr:
-        assert (c == -1) == (k in (2, 3))
+    """The oracle's direction schedule is pinned to the step lines the
+    reference kernel itself prints, step for step, with the counts, on
+    Kronecker and uniform graphs and under non-default alpha / beta."""
+    if not orc.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    cases = [((12, 16, False), 16, (15, 18)), ((14, 16, False), 16, (15, 18)),
+             ((16, 16, False), 8, (15, 18)), ((13, 8, True), 16, (15, 18)),
+             ((13, 16, False), 8, (2, 1000)), ((13, 16, False), 8, (100, 4))]
+    for (scale, degree, uniform), count, (alpha, beta) in cases:
+        hg = graph(scale, degree, uniform)
+        g = orc.CSR.of(hg)
+        rg = orc.RefGraph(g)
+        kinds_seen = set()
+        for s in dmm.pick_sources(hg, count):
+            s = int(s)
+            rp, rtr = reference_trace(rg, s, alpha, beta)
+            p, tr = orc.dobfs(g, s, alpha, beta)
+            assert tr == rtr, (scale, uniform, s, alpha, beta)
+            kinds_seen |= {k for k, _ in tr}
+        rg.close()
+        if (alpha, beta) == (15, 18) and not uniform:
+            assert kinds_seen == {0, 1, 2, 3}  # every step kind is exercised
 
 
 @pytest.mark.parametrize("scale,degree,uniform", [(10, 16, False), (12, 8, True), (14, 16, False)])
 def test_oracle_matches_reference_kernels(dmm, orc, graph, scale, degree, uniform):
+    """Against the reference kernels compiled in oracle/_ref: each side's
+    parent tree passes the other side's verifier, and the depths of the
+    reference's tree equal the oracle's serial BFS depths.  (Top-down parents
+    depend on which claim lands first under OpenMP, so parent arrays are not
+    compared entry by entry.)  The FP32 PageRank scores are bit-equal."""
     if not orc.ref_available():
         pytest.skip("oracle/_ref not built (reference tree absent)")
     hg = graph(scale, degree, uniform)
@@ -186,15 +225,25 @@ def test_oracle_matches_reference_kernels(dmm, orc, graph, scale, degree, unifor
     for s in srcs:
         s = int(s)
         p, _ = orc.dobfs(g, s)
-        rp = rg.dobfs(s)
+        with CapturedStdout():
+            rp = rg.dobfs(s)
         assert rg.verify(s, p), "reference verifier rejects the oracle's tree"
         assert orc.bfs_verify(g, s, rp)[0], "oracle verifier rejects the reference's tree"
-        assert np.array_equal(p >= 0, rp >= 0)
+        d = orc.bfs_depth(g, s)
+        assert np.array_equal(depth_from_parent(rp, s), d)
+        assert np.array_equal(depth_from_parent(p, s), d)
     sc, it, _ = orc.pr_f32(g, 20, 1e-4)
     assert np.array_equal(sc, rg.pagerank(20, 1e-4)), "FP32 PageRank differs from the reference"
     sc0, _, _ = orc.pr_f32(g, 20, 0.0)
     assert np.array_equal(sc0, rg.pagerank(20, 0.0))
     rg.close()
+
+
+def test_depth_from_parent_helper():
+    # path 0-1-2-3 rooted at 1, vertex 4 unreached
+    assert depth_from_parent([1, 1, 1, 2, -1], 1).tolist() == [1, 0, 1, 2, -1]
+    with pytest.raises(AssertionError):
+        depth_from_parent([1, 0, 3, 2], 0)  # 2 <-> 3 cycle never reaches 0
 
 
 # ---- PageRank closed forms ----------------------------------------------------
@@ -249,6 +298,20 @@ def test_pr_f32_close_to_f64(orc, graph):
     assert it32 == it64
     assert np.abs(s32 - s64).max() < 1e-6
     assert (np.abs(s32 - s64) / s64).max() < 1e-4
+
+
+@pytest.mark.parametrize("nthreads", [1, 3, 8])
+def test_pr_threaded_oracles_bit_identical(orc, graph, nthreads):
+    """The threaded FP64 oracle (used at full size) returns the serial
+    oracle's scores, error trace and stop iteration bit for bit."""
+    for scale, uniform in ((14, False), (13, True)):
+        g = orc.CSR.of(graph(scale, 16, uniform))
+        for eps in (1e-4, 0.0):
+            for serial, threaded in ((orc.pr_f64, orc.pr_f64_mt), (orc.pr_f32, orc.pr_f32_mt)):
+                a = serial(g, 20, eps)
+                b = threaded(g, 20, eps, nthreads=nthreads)
+                assert a[1] == b[1]
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2])
 
 
 def test_pr_residual_verifier(orc, graph):
