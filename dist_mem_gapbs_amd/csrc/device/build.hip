@@ -78,9 +78,26 @@ __global__ __launch_bounds__(kBThreads) void k_cols(const uint64_t* __restrict__
     nbr[i] = static_cast<int32_t>(static_cast<uint32_t>(keys[i]));
 }
 
+// A transient device buffer of the builder; its bytes count towards the
+// context's device memory only while it is held.
 struct DevBuf {
+  dmm_ctx* ctx = nullptr;
   void* p = nullptr;
-  ~DevBuf() { dev_free(p); }
+  size_t bytes = 0;
+  int alloc(dmm_ctx* c, size_t b, const char* what) {
+    int rc = dev_alloc(c, &p, b, what);
+    if (rc == DMM_OK) {
+      ctx = c;
+      bytes = b ? b : 8;  // dev_alloc's minimum
+    }
+    return rc;
+  }
+  ~DevBuf() {
+    if (p) {
+      dev_free(p);
+      ctx->dev_bytes -= static_cast<int64_t>(bytes);
+    }
+  }
 };
 
 int bit_length(uint64_t x) {
@@ -99,11 +116,11 @@ int build_direction(dmm_ctx* ctx, const int2* d_el, int64_t ne, int64_t n, int m
   hipStream_t s = ctx->stream;
   DevBuf keys_a, keys_b, keep, count, temp, offs, cols;
   int rc;
-  if ((rc = dev_alloc(ctx, &keys_a.p, sizeof(uint64_t) * std::max<int64_t>(1, nk), "builder keys")) ||
-      (rc = dev_alloc(ctx, &keys_b.p, sizeof(uint64_t) * std::max<int64_t>(1, nk), "builder keys")) ||
-      (rc = dev_alloc(ctx, &keep.p, std::max<int64_t>(1, nk), "builder flags")) ||
-      (rc = dev_alloc(ctx, &count.p, sizeof(int64_t), "builder count")) ||
-      (rc = dev_alloc(ctx, &offs.p, sizeof(int64_t) * (n + 1), "builder offsets")))
+  if ((rc = keys_a.alloc(ctx, sizeof(uint64_t) * std::max<int64_t>(1, nk), "builder keys")) ||
+      (rc = keys_b.alloc(ctx, sizeof(uint64_t) * std::max<int64_t>(1, nk), "builder keys")) ||
+      (rc = keep.alloc(ctx, std::max<int64_t>(1, nk), "builder flags")) ||
+      (rc = count.alloc(ctx, sizeof(int64_t), "builder count")) ||
+      (rc = offs.alloc(ctx, sizeof(int64_t) * (n + 1), "builder offsets")))
     return rc;
   uint64_t* ka = static_cast<uint64_t*>(keys_a.p);
   uint64_t* kb = static_cast<uint64_t*>(keys_b.p);
@@ -117,7 +134,7 @@ int build_direction(dmm_ctx* ctx, const int2* d_el, int64_t ne, int64_t n, int m
     DMM_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, ka, kb, nk, 0, end_bit, s));
     DMM_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, kb, static_cast<uint8_t*>(keep.p), ka,
                                                      static_cast<int64_t*>(count.p), nk, s));
-    if ((rc = dev_alloc(ctx, &temp.p, std::max<size_t>(1, std::max(sort_bytes, sel_bytes)), "builder scratch")))
+    if ((rc = temp.alloc(ctx, std::max<size_t>(1, std::max(sort_bytes, sel_bytes)), "builder scratch")))
       return rc;
     DMM_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortKeys(temp.p, sort_bytes, ka, kb, nk, 0, end_bit, s));
     hipLaunchKernelGGL(k_keep, dim3(grid_of(nk)), dim3(kBThreads), 0, s, kb, nk, static_cast<uint8_t*>(keep.p));
@@ -130,7 +147,7 @@ int build_direction(dmm_ctx* ctx, const int2* d_el, int64_t ne, int64_t n, int m
   // The compacted keys are in ka[0, m).
   hipLaunchKernelGGL(k_bounds, dim3(grid_of(m + 1)), dim3(kBThreads), 0, s, ka, m, n, static_cast<int64_t*>(offs.p));
   DMM_HIP_CHECK(ctx, hipGetLastError());
-  if ((rc = dev_alloc(ctx, &cols.p, sizeof(int32_t) * std::max<int64_t>(1, m), "builder neighbours"))) return rc;
+  if ((rc = cols.alloc(ctx, sizeof(int32_t) * std::max<int64_t>(1, m), "builder neighbours"))) return rc;
   if (m > 0) {
     hipLaunchKernelGGL(k_cols, dim3(grid_of(m)), dim3(kBThreads), 0, s, ka, m, static_cast<int32_t*>(cols.p));
     DMM_HIP_CHECK(ctx, hipGetLastError());
@@ -154,7 +171,7 @@ int build_graph_device(dmm_ctx* ctx, const int32_t* pairs, int64_t ne, int64_t n
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   DevBuf el;
   int rc;
-  if ((rc = dev_alloc(ctx, &el.p, sizeof(int2) * std::max<int64_t>(1, ne), "builder edge list"))) return rc;
+  if ((rc = el.alloc(ctx, sizeof(int2) * std::max<int64_t>(1, ne), "builder edge list"))) return rc;
   if (ne > 0) DMM_HIP_CHECK(ctx, hipMemcpy(el.p, pairs, sizeof(int2) * ne, hipMemcpyHostToDevice));
   g->num_nodes = n;
   g->directed = !symmetrize;

@@ -399,15 +399,14 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
 
 int dmm_bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts, uint64_t* succ,
                    int32_t* num_levels) {
-  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
-  if (ctx->nloc <= 0) return set_err(ctx, DMM_ERR_STATE, "no graph uploaded");
+  int rc = need_graph(ctx);
+  if (rc) return rc;
   if (ctx->relabeled)
     return set_err(ctx, DMM_ERR_STATE, "dmm_bc_forward needs DMM_LAYOUT_NATURAL (edge positions are the caller's)");
   if (ctx->vbegin != 0 || ctx->vend != ctx->n)
     return set_err(ctx, DMM_ERR_STATE, "dmm_bc_forward runs on a whole-graph upload");
   if (source < 0 || source >= ctx->n)
     return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
-  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   return dmm::bc_forward(ctx, source, depth, path_counts, succ, num_levels);
 }
 

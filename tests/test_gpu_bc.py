@@ -67,3 +67,19 @@ def test_bc_forward_needs_natural_layout(ctx, dmm, graph):
     with pytest.raises(dmm.DmmError) as e:
         ctx.bc_forward(0)
     assert e.value.code == L.DMM_ERR_STATE
+
+
+def test_bc_forward_needs_an_uploaded_graph(dmm):
+    """Without a graph (fresh context, or a rejected upload) the call
+    returns DMM_ERR_STATE and launches nothing."""
+    with dmm.Context(0) as c:
+        c.set_layout(L.DMM_LAYOUT_NATURAL)
+        with pytest.raises(dmm.DmmError) as e:
+            c._check(c._lib.dmm_bc_forward(c._h, 0, None, None, None, None))
+        assert e.value.code == L.DMM_ERR_STATE
+        with pytest.raises(dmm.DmmError):
+            c.upload(num_nodes=4, out_offsets=np.array([0, 1, 2, 2, 9], np.int64),
+                     out_neighs=np.array([1, 0], np.int32))
+        with pytest.raises(dmm.DmmError) as e:
+            c._check(c._lib.dmm_bc_forward(c._h, 0, None, None, None, None))
+        assert e.value.code == L.DMM_ERR_STATE
