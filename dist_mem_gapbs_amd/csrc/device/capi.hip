@@ -89,9 +89,9 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
                   ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
-                  ctx->d_tl_off, ctx->d_tl_nbr, ctx->d_tl_poff, ctx->d_tl_pslot, ctx->d_titems,
-                  ctx->d_tsupers, ctx->d_tpartial, ctx->d_tsrc, ctx->d_tchunks, ctx->d_twin,
-                  ctx->d_piece_part, ctx->d_bsegs, ctx->d_head,
+                  ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_meta, ctx->d_wt_src, ctx->d_wt_pbase,
+                  ctx->d_wt_brow, ctx->d_wt_ibase, ctx->d_wt_rcnt, ctx->d_wt_idx, ctx->d_wt_part,
+                  ctx->d_bsegs, ctx->d_head,
                   ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
                   ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp};
   for (void* p : bufs) dmm::dev_free(p);
@@ -129,11 +129,11 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bsegs = nullptr;
   ctx->n_bsegs = 0;
   ctx->bsegs_ready = false;
-  ctx->d_tl_off = ctx->d_tl_poff = ctx->d_twin = nullptr;
-  ctx->d_tl_nbr = ctx->d_tl_pslot = nullptr;
-  ctx->d_titems = ctx->d_tsupers = ctx->d_tpartial = ctx->d_tchunks = ctx->d_piece_part = nullptr;
-  ctx->d_tsrc = nullptr;
-  ctx->n_titems = ctx->n_tsuper = ctx->n_chunks = ctx->n_pieces = ctx->tile_edges = ctx->list_edges = 0;
+  ctx->d_wt_cwin = ctx->d_wt_brow = nullptr;
+  ctx->d_wt_wstart = ctx->d_wt_pbase = ctx->d_wt_ibase = nullptr;
+  ctx->d_wt_meta = ctx->d_wt_part = nullptr;
+  ctx->d_wt_src = ctx->d_wt_rcnt = ctx->d_wt_idx = nullptr;
+  ctx->n_chunks = ctx->n_pieces = ctx->n_row_blocks = ctx->tile_slots_filled = ctx->tiles_window = 0;
   ctx->n_windows = 0;
   ctx->relabeled = false;
   ctx->h_perm.clear();
@@ -228,8 +228,10 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
   }
   if (!ctx->d_err_part) {
     const int64_t parts =
-        std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx), dmm::pr_fused_blocks(ctx)) +
-        std::max<int64_t>(dmm::pr_super_blocks(ctx), dmm::ceil_div(ctx->n_tsuper, 256) + 1);
+        std::max<int64_t>(std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx),
+                                            dmm::pr_fused_blocks(ctx)),
+                          dmm::pr_row_blocks(ctx)) +
+        dmm::pr_super_blocks(ctx);
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
     ctx->err_part_cap = parts;
   }

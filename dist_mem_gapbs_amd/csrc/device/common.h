@@ -102,6 +102,7 @@ struct Tuning {
   int td_edges_per_block = 1024;
   int grid_per_cu = 8;                 // workgroups per CU for grid-stride steps
   dmm_batch_params batch = {20, 0, 16, 0};  // grid_per_cu 0: the one above
+  int pr_window = 32768;               // sources per PageRank tile window
   double plain_ms = 0, batch_ms = 0;
 };
 
@@ -199,23 +200,18 @@ struct dmm_ctx {
   double* d_err_part = nullptr;
   // Window-tiled compensated PageRank (pr_tiles.hip).
   bool tiles_ready = false;
-  int64_t* d_tl_off = nullptr;    // per row: hub + short-run in-edges
-  int32_t* d_tl_nbr = nullptr;
-  int64_t* d_tl_poff = nullptr;   // per row: its piece slots
-  int32_t* d_tl_pslot = nullptr;
-  void* d_titems = nullptr;       // TItem
-  int64_t n_titems = 0;
-  void* d_tsupers = nullptr;      // PrSuper of split rows
-  int64_t n_tsuper = 0;
-  void* d_tpartial = nullptr;     // double2 per split-row segment
-  uint16_t* d_tsrc = nullptr;     // 64 per chunk
-  void* d_tchunks = nullptr;      // TileChunk
-  int64_t n_chunks = 0;
-  int64_t* d_twin = nullptr;      // first chunk of each window (+ end)
+  int32_t* d_wt_cwin = nullptr;     // window of each chunk
+  int64_t* d_wt_wstart = nullptr;   // first chunk of each window (+ end)
+  void* d_wt_meta = nullptr;        // TileLane per lane
+  uint16_t* d_wt_src = nullptr;     // kChunkEdges source offsets per chunk
+  int64_t* d_wt_pbase = nullptr;    // first partial of each row block (+ end)
+  int32_t* d_wt_brow = nullptr;     // first row of each row block (+ end)
+  int64_t* d_wt_ibase = nullptr;    // first list entry of each row block (+ end)
+  uint16_t* d_wt_rcnt = nullptr;    // partials per row (multi-row blocks)
+  uint16_t* d_wt_idx = nullptr;     // per row, window order: partial offset in its block
+  void* d_wt_part = nullptr;        // float2 (sum, error) per piece
+  int64_t n_chunks = 0, n_pieces = 0, n_row_blocks = 0, tile_slots_filled = 0, tiles_window = 0;
   int32_t n_windows = 0;
-  void* d_piece_part = nullptr;   // float2 (sum, error) per piece
-  int64_t n_pieces = 0;
-  int64_t tile_edges = 0, list_edges = 0;
   // Rows of the light loop and the in-degree-0 tail (see the upload).
   int64_t pr_light_begin = 0, pr_light_end = 0, pr_zero_begin = 0;
   int64_t err_part_cap = 0;

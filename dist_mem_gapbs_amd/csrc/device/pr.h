@@ -2,6 +2,9 @@
 #ifndef DMM_DEVICE_PR_H_
 #define DMM_DEVICE_PR_H_
 
+#include <string>
+#include <vector>
+
 #include "common.h"
 
 namespace dmm {
@@ -24,34 +27,49 @@ constexpr int64_t kPrSplit = 2048;    // rows with more in-edges are split
 constexpr int64_t kPrSegment = 2048;  // edges per segment
 
 // Window-tiled compensated PageRank (degree-ordered whole graphs).
-// Sources are cut into windows of kTileWindow vertices.  Window 0 (the hub
-// vertices) stays in the row kernel's LDS copy; in any other window, a
-// row's run of at least kTileMinRun in-edges becomes pieces of at most 64
-// edges that a tile kernel sums from an LDS copy of that window; shorter
-// runs stay in the row's list and are gathered from HBM.
-constexpr int kTileWindow = 32768;
-constexpr int kTileMinRun = 2;
-constexpr int kTilePiece = 64;
-// A window is tiled only if its long runs hold at least this many edges
-// (the 128 KB LDS copy is then amortized); otherwise they stay in the lists.
-constexpr int64_t kTileMinWindowEdges = kTileWindow;
+// Sources are cut into windows of `pr_window` vertices (a tuning value:
+// 16384, 32768 or 40960 contributions, 64 to 160 KB of LDS).  Every in-edge
+// sits in a 64-lane chunk of its source's window, kLaneEdges edges per
+// lane, and is summed from an LDS copy of that window; each piece (a row's
+// edges inside one chunk) leaves one (sum, error) partial, and the row
+// kernel merges a row's partials in window order.
+constexpr int kLaneEdges = 16;
+constexpr int kChunkEdges = 64 * kLaneEdges;
+// Row blocks: at most kBlockRuns runs (a block's partials, runs plus chunk
+// cuts, are staged in LDS when they number at most kRowSlots); rows with
+// kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
+constexpr int kRowSlots = 8192;
+constexpr int kRowThreads = 512;    // threads of the row kernel
+constexpr int kBlockRuns = 6144;
+constexpr int kSoloRuns = 256;
+constexpr int64_t kSoloEdges = 4096;
 
-// 64 pieces-edges per chunk (one per lane), all in one window; lane i's
-// source is window * kTileWindow + src[chunk * 64 + i].
-struct TileChunk {
-  int32_t window;
-  int32_t first_slot;  // partial slot of the chunk's first piece
-  uint64_t head;       // bit i: lane i starts a piece (bit 0 always)
-  uint64_t valid;      // lanes holding an edge (a prefix)
+// Per lane of a chunk: x = end mask (bit j: slot j ends a piece) | filled
+// slots << 16; y = output slot of the first piece ending in the lane (the
+// lane's other pieces follow it).
+struct TileLane {
+  uint32_t x, y;
 };
-// Row-kernel work item of the tiled mode: list edges [b, e) and the
-// row's piece slots pslot[pb, pe) to merge.
-struct TItem {
-  int32_t v;
-  int32_t slot;        // < 0: whole row; else the segment's partial slot
-  int64_t b, e;
-  int64_t pb, pe;
+
+// The tile layout as built on the host.
+struct TileHost {
+  std::vector<int32_t> cwin;     // window of each chunk
+  std::vector<int64_t> wstart;   // first chunk of each window (+ end)
+  std::vector<TileLane> meta;    // 64 per chunk
+  std::vector<uint16_t> src;     // kChunkEdges per chunk
+  std::vector<int64_t> pbase;    // row blocks: first partial (+ end)
+  std::vector<int32_t> brow;     //             first row (+ end)
+  std::vector<int64_t> ibase;    //             first list entry (+ end)
+  std::vector<uint16_t> rcnt;    // per row below zb
+  std::vector<uint16_t> idx;
+  int64_t n_pieces = 0, slots_filled = 0, window = 0;
+  int32_t n_windows = 0;
 };
+
+// Builds the tile layout of an in-CSR (n rows, sources in [0, n_gather),
+// rows at and above zb without in-edges); sorts nbr row by row in place.
+int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off, int32_t* nbr, int64_t window,
+                     TileHost* out, std::string* err);
 
 // Builds the tiled structures from the device CSR (host work, once per
 // upload, at the first compensated call).  Returns DMM_OK or an error.
@@ -61,6 +79,7 @@ int pr_light_blocks(dmm_ctx* ctx);
 int pr_heavy_blocks(dmm_ctx* ctx);
 int pr_super_blocks(dmm_ctx* ctx);
 int pr_fused_blocks(dmm_ctx* ctx);  // persistent KAHAN / FP64 kernel
+int pr_row_blocks(dmm_ctx* ctx);    // row kernel of the tiled compensated mode
 // Size of the PrParams block (bytes) the ctx must hold at d_prparams.
 constexpr size_t kPrParamsBytes = 64;
 

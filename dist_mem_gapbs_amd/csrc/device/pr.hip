@@ -359,17 +359,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused(PrArgs a, const T* __re
 
 // ---- FP32_KAHAN, window-tiled (degree-ordered whole graphs) ----------------
 
-// Pieces of every window w >= 1: persistent blocks, each owning a contiguous
-// range of chunks (window-major), copy one window of contributions into LDS
-// at a time; a wavefront reduces kTileIlp chunks at once (independent
-// chains), each with a segmented inclusive scan over its 64 lanes built
-// from DPP lane moves (row_shr 1/2/4/8 inside 16-lane rows, then
-// row_bcast 15 and 31 across rows), restarted at every piece head.  A piece
-// holds at most 64 FP32 values; they accumulate in FP64 (relative error
-// <= 64 * 2^-53) and the piece enters its row's compensated FP32 sum as the
-// unevaluated pair (hi, lo) = (float(s), float(s - hi)).
-constexpr int kTileIlp = 4;
-
 template <int kCtrl, int kRowMask>
 __device__ __forceinline__ double dpp_move(double x) {
   const long long b = __double_as_longlong(x);
@@ -378,7 +367,9 @@ __device__ __forceinline__ double dpp_move(double x) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-// Inclusive scan of v over lanes [seg, lane] (seg = the lane's piece head).
+// Inclusive scan of v over lanes [seg, lane] (seg = the lane's segment
+// head), from DPP lane moves: row_shr 1/2/4/8 inside 16-lane rows, then
+// row_bcast 15 and 31 across rows.
 __device__ __forceinline__ double seg_scan(double v, int lane, int seg) {
   double t;
   t = dpp_move<0x111, 0xf>(v);  // row_shr:1
@@ -395,162 +386,191 @@ __device__ __forceinline__ double seg_scan(double v, int lane, int seg) {
   if (lane >= 32 && 31 >= seg) v += t;
   return v;
 }
-__global__ __launch_bounds__(kFusedThreads) void k_tile(const TileChunk* __restrict__ chunks, int64_t n_chunks,
-                                                        const int64_t* __restrict__ win_start,
-                                                        const uint16_t* __restrict__ src,
-                                                        const float* __restrict__ cin, int64_t n_gather,
-                                                        float2* __restrict__ piece_part, const PrCtrl* ctrl) {
-  __shared__ float win[kTileWindow];
+
+struct TileArgs {
+  const int32_t* cwin;     // window of each chunk
+  const int64_t* wstart;   // first chunk of each window (+ end)
+  const TileLane* meta;
+  const uint16_t* src;
+  int64_t n_chunks;
+  float2* part;            // (sum, error) per piece
+  const int64_t* pbase;    // row blocks: first partial (+ end)
+  const int32_t* brow;     //             first row (+ end)
+  const int64_t* ibase;    //             first list entry (+ end)
+  const uint16_t* rcnt;    // per row: its partials (multi-row blocks)
+  const uint16_t* idx;     // per row: partial offsets in its block, window order
+  int64_t n_blocks;
+};
+
+constexpr int kTileThreads = 1024;
+
+// One chunk of the tile pass, run by one wavefront.  Lane i holds slots
+// [i * kLaneEdges, (i + 1) * kLaneEdges) of the chunk; a piece may span
+// lanes but never chunks.  Each lane first sums its edges after its last
+// piece end (all of them if none ends in it); a segmented scan over the
+// lanes (reset at lanes where a piece ends) turns these into the carry each
+// lane receives from the piece still open at its start.  The lane then
+// walks its edges again, adding from the carry and emitting a partial at
+// every piece end.  Values accumulate in FP64 (a piece holds at most
+// kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53) and a
+// piece enters its row's compensated FP32 sum as the unevaluated pair
+// (hi, lo) = (float(s), float(s - hi)).
+__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, int64_t c, int lane) {
+  const TileLane mt = t.meta[c * 64 + lane];
+  const uint4* s4 = reinterpret_cast<const uint4*>(t.src + c * kChunkEdges + lane * kLaneEdges);
+  const uint4 q0 = s4[0], q1 = s4[1];
+  const uint32_t sw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+  const uint32_t ends = mt.x & 0xFFFFu;
+  const int cnt = static_cast<int>(mt.x >> 16);
+  float x[kLaneEdges];
+#pragma unroll
+  for (int j = 0; j < kLaneEdges; j++) x[j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
+#pragma unroll
+  for (int j = 0; j < kLaneEdges; j++)
+    if (j >= cnt) x[j] = 0.0f;
+  double tail = 0.0;
+#pragma unroll
+  for (int j = 0; j < kLaneEdges; j++) {
+    tail += static_cast<double>(x[j]);
+    if ((ends >> j) & 1u) tail = 0.0;
+  }
+  const unsigned long long has_end = __ballot(ends != 0u);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const unsigned long long h = has_end & upto;
+  const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
+  const double carry_out = seg_scan(tail, lane, seg);
+  double acc = __shfl_up(carry_out, 1, 64);
+  if (lane == 0) acc = 0.0;
+  uint32_t slot = mt.y;
+#pragma unroll
+  for (int j = 0; j < kLaneEdges; j++) {
+    acc += static_cast<double>(x[j]);
+    if ((ends >> j) & 1u) {
+      const float hi = static_cast<float>(acc);
+      const float lo = static_cast<float>(acc - static_cast<double>(hi));
+      t.part[slot++] = make_float2(hi, lo);
+      acc = 0.0;
+    }
+  }
+}
+
+// Tile pass: persistent workgroups (one per CU), each owning a contiguous
+// range of chunks (window-major); the workgroup copies one window of
+// contributions into LDS at a time and its wavefronts take the window's
+// chunks in turn.
+template <int kWin>
+__global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
+                                                      const PrCtrl* ctrl) {
+  __shared__ float win[kWin];
   if (ctrl->pad) return;  // converged
-  const int64_t per = (n_chunks + gridDim.x - 1) / gridDim.x;
-  const int64_t c0 = min(n_chunks, int64_t(blockIdx.x) * per), c1 = min(n_chunks, c0 + per);
+  const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
+  const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
-  const uint64_t below = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  constexpr int kWaves = kTileThreads / 64;
   for (int64_t c = c0; c < c1;) {
-    const int32_t w = chunks[c].window;
-    const int64_t wend = min(c1, win_start[w + 1]);
-    const int64_t base = int64_t(w) * kTileWindow;
-    const int cnt = static_cast<int>(min<int64_t>(kTileWindow, n_gather - base));
+    const int32_t w = t.cwin[c];
+    const int64_t wend = min(c1, t.wstart[w + 1]);
+    const int64_t base = int64_t(w) * kWin;
+    const int cnt = static_cast<int>(min<int64_t>(kWin, n_gather - base));
     __syncthreads();  // the previous window's readers are done
-    lds_fill<float, kTileWindow * 4>(win, cin + base, cnt);
-    __syncthreads();
-    constexpr int kWaves = kFusedThreads / 64;
-    for (int64_t cb = c + wave; cb < wend; cb += kWaves * kTileIlp) {
-      TileChunk ch[kTileIlp];
-      uint16_t e[kTileIlp];
+    if (cnt == kWin) {
+      const float4* s = reinterpret_cast<const float4*>(cin + base);
+      float4* d = reinterpret_cast<float4*>(win);
+      constexpr int kPer = kWin / 4 / kTileThreads;
+      float4 v[kPer];
 #pragma unroll
-      for (int j = 0; j < kTileIlp; j++) {
-        const int64_t cc = cb + j * kWaves;
-        if (cc < wend) {
-          ch[j] = chunks[cc];
-          e[j] = src[cc * 64 + lane];
-        } else {
-          ch[j] = TileChunk{0, 0, 1ull, 0ull};
-          e[j] = 0;
-        }
-      }
-      double acc[kTileIlp];
+      for (int i = 0; i < kPer; i++) v[i] = s[threadIdx.x + i * kTileThreads];
 #pragma unroll
-      for (int j = 0; j < kTileIlp; j++) {
-        const bool valid = (ch[j].valid >> lane) & 1ull;
-        const float xv = win[valid ? e[j] : 0];
-        const int seg = 63 - __clzll(static_cast<long long>(ch[j].head & below));
-        acc[j] = seg_scan(valid ? static_cast<double>(xv) : 0.0, lane, seg);
-      }
-#pragma unroll
-      for (int j = 0; j < kTileIlp; j++) {
-        const uint64_t tails = ch[j].valid & ((ch[j].head >> 1) | ~(ch[j].valid >> 1));
-        if ((tails >> lane) & 1ull) {
-          const int rank = __popcll(tails & ((1ull << lane) - 1));
-          const float hi = static_cast<float>(acc[j]);
-          const float lo = static_cast<float>(acc[j] - static_cast<double>(hi));
-          piece_part[ch[j].first_slot + rank] = make_float2(hi, lo);
-        }
-      }
+      for (int i = 0; i < kPer; i++) d[threadIdx.x + i * kTileThreads] = v[i];
+    } else {
+      for (int j = threadIdx.x; j < cnt; j += kTileThreads) win[j] = cin[base + j];
     }
+    __syncthreads();
+    for (int64_t cb = c + wave; cb < wend; cb += kWaves) tile_chunk(t, win, cb, lane);
     c = wend;
   }
 }
 
-struct TiledArgs {
-  const int64_t* off;      // row lists (hub + short runs)
-  const int32_t* nbr;
-  const int64_t* poff;     // row piece slots
-  const int32_t* pslot;
-  const float2* ppart;     // piece partials of this iteration
-  const TItem* items;
-  int64_t n_items;
-};
+// Compensated merge of partials p[ix[0..cnt)] (LDS or global) in order.
+template <typename P>
+__device__ __forceinline__ Comp merge_list(const P* p, const uint16_t* __restrict__ ix, int cnt) {
+  Comp acc{0.0f, 0.0f};
+  int k = 0;
+  for (; k + 4 <= cnt; k += 4) {
+    uint16_t q[4];
+    float2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) q[j] = ix[k + j];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = p[q[j]];
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc.merge(v[j].x, v[j].y);
+  }
+  for (; k < cnt; k++) {
+    const float2 v = p[ix[k]];
+    acc.merge(v.x, v.y);
+  }
+  return acc;
+}
 
-// Row pass of the tiled mode: k_fused over the row lists, and every row
-// (or split-row segment) merges its pieces' partials, lanes striding the
-// slots in window order, before its fixed butterfly.
-__global__ __launch_bounds__(kFusedThreads) void k_fused_tiled(PrArgs a, TiledArgs t,
-                                                               const float* __restrict__ cin,
-                                                               float* __restrict__ cout,
-                                                               float* __restrict__ score) {
-  __shared__ float hub[kHubBytes / sizeof(float)];
-  __shared__ double red[kFusedThreads / 64];
+// Row pass of the tiled mode: persistent workgroups take row blocks in
+// turn.  A one-row block merges its contiguous partials with its threads
+// striding them in order, then a fixed tree; a multi-row block is staged in
+// LDS and every thread merges one row's partials in window order.  Rows
+// without in-edges (the tail) only finish.
+__global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
+                                                     float* __restrict__ score) {
+  __shared__ float2 reg[kRowSlots];
+  __shared__ float2 wred[kRowThreads / 64];
+  __shared__ int scan[kRowThreads / 64 + 1];
+  __shared__ double red[kRowThreads / 64];
   if (a.ctrl->pad) return;
-  const int hub_n = static_cast<int>(min<int64_t>(kHubBytes / sizeof(float), a.n_gather));
-  lds_fill<float, kHubBytes>(hub, cin, hub_n);
-  __syncthreads();
   double err = 0;
   const int lane = lane_id();
-  const int64_t wave = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kFusedThreads) >> 6;
-  for (int64_t h = wave; h < t.n_items; h += nwaves) {
-    const TItem it = t.items[h];
-    FusedAcc<float> acc;
-    int64_t k = it.b + lane;
-    for (; k + 64 * (kUnroll - 1) < it.e; k += 64 * kUnroll) {
-      int32_t u[kUnroll];
-      float x[kUnroll];
-#pragma unroll
-      for (int j = 0; j < kUnroll; j++) u[j] = t.nbr[k + 64 * j];
-#pragma unroll
-      for (int j = 0; j < kUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, u[j]);
-#pragma unroll
-      for (int j = 0; j < kUnroll; j++) acc.add(x[j]);
-    }
-    for (; k < it.e; k += 64) acc.add(hub_gather(hub, hub_n, cin, t.nbr[k]));
-    for (int64_t p = it.pb + lane; p < it.pe; p += 64) acc.merge(t.ppart[t.pslot[p]]);
-    acc.reduce<64>(lane);
-    if (lane == 0) {
-      if (it.slot < 0)
-        err += finish_f32(a, it.v, acc.value(), score, cout);
-      else
-        a.partial[it.slot] = acc.partial();
-    }
-  }
-  const int sub = threadIdx.x & (kGroup - 1);
-  const int64_t grp = (int64_t(blockIdx.x) * kFusedThreads + threadIdx.x) / kGroup;
-  const int64_t ngrp = (int64_t(gridDim.x) * kFusedThreads) / kGroup;
-  const int64_t rounds = (a.zero_begin + ngrp - 1) / ngrp;
-  for (int64_t r = 0; r < rounds; r++) {
-    const int64_t u = grp + r * ngrp;
-    FusedAcc<float> acc;
-    bool mine = false;
-    if (u < a.zero_begin) {
-      const int64_t s = t.off[u], e = t.off[u + 1];
-      if (e - s < a.heavy_threshold) {
-        mine = true;
-        // The row's list edges, then its pieces, as one strided range; each
-        // lane takes its own entries in order, kLightUnroll loads in flight.
-        const int64_t pb = t.poff[u], np = t.poff[u + 1] - pb, nl = e - s;
-        int64_t k = sub;
-        for (; k + (kLightUnroll - 1) * kGroup < nl; k += kLightUnroll * kGroup) {
-          int32_t v[kLightUnroll];
-          float x[kLightUnroll];
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) v[j] = t.nbr[s + k + j * kGroup];
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) x[j] = hub_gather(hub, hub_n, cin, v[j]);
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) acc.add(x[j]);
-        }
-        for (; k < nl; k += kGroup) acc.add(hub_gather(hub, hub_n, cin, t.nbr[s + k]));
-        for (; k + (kLightUnroll - 1) * kGroup < nl + np; k += kLightUnroll * kGroup) {
-          int32_t q[kLightUnroll];
-          float2 pp[kLightUnroll];
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) q[j] = t.pslot[pb + k + j * kGroup - nl];
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) pp[j] = t.ppart[q[j]];
-#pragma unroll
-          for (int j = 0; j < kLightUnroll; j++) acc.merge(pp[j]);
-        }
-        for (; k < nl + np; k += kGroup) acc.merge(t.ppart[t.pslot[pb + k - nl]]);
+  const int wave = threadIdx.x >> 6;
+  for (int64_t b = blockIdx.x; b < t.n_blocks; b += gridDim.x) {
+    const int32_t r0 = t.brow[b], r1 = t.brow[b + 1];
+    const int64_t p0 = t.pbase[b], p1 = t.pbase[b + 1];
+    if (r1 - r0 == 1) {
+      FusedAcc<float> acc;
+      for (int64_t p = p0 + threadIdx.x; p < p1; p += kRowThreads) acc.merge(t.part[p]);
+      acc.reduce<64>(lane);
+      if (lane == 0) wred[wave] = make_float2(acc.c.s, acc.c.c);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        Comp tot = {wred[0].x, wred[0].y};
+        for (int w = 1; w < kRowThreads / 64; w++) tot.merge(wred[w].x, wred[w].y);
+        err += finish_f32(a, r0, tot.value(), score, cout);
       }
+      __syncthreads();
+      continue;
     }
-    acc.reduce<kGroup>(sub);
-    if (mine && sub == 0) err += finish_f32(a, u, acc.value(), score, cout);
+    const int64_t np = p1 - p0;
+    const bool fits = np <= kRowSlots;
+    if (fits) {
+      for (int64_t i = threadIdx.x; i < np; i += kRowThreads) reg[i] = t.part[p0 + i];
+      __syncthreads();
+    }
+    int64_t ib = t.ibase[b];
+    for (int32_t rr = r0; rr < r1; rr += kRowThreads) {
+      const int32_t r = rr + static_cast<int32_t>(threadIdx.x);
+      const int c = r < r1 ? t.rcnt[r] : 0;
+      int total;
+      const int ex = block_excl_scan<kRowThreads>(c, scan, &total);
+      if (r < r1) {
+        const uint16_t* ix = t.idx + ib + ex;
+        const Comp acc = fits ? merge_list(reg, ix, c) : merge_list(t.part + p0, ix, c);
+        err += finish_f32(a, r, acc.value(), score, cout);
+      }
+      ib += total;
+    }
+    __syncthreads();  // LDS readers done before the next block's copy
   }
-  for (int64_t u = a.zero_begin + int64_t(blockIdx.x) * kFusedThreads + threadIdx.x; u < a.nloc;
-       u += int64_t(gridDim.x) * kFusedThreads)
+  for (int64_t u = a.zero_begin + int64_t(blockIdx.x) * kRowThreads + threadIdx.x; u < a.nloc;
+       u += int64_t(gridDim.x) * kRowThreads)
     err += finish_f32(a, u, 0.0f, score, cout);
-  const double tt = block_err_sum<kFusedThreads>(err, red);
+  const double tt = block_err_sum<kRowThreads>(err, red);
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = tt;
 }
 
@@ -706,23 +726,29 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
     } else if (ctx->tiles_ready && !gcontrib) {
       const float* cin = ctx->d_contrib[it & 1];
       float* cout = ctx->d_contrib[(it + 1) & 1];
-      TiledArgs t;
-      t.off = ctx->d_tl_off;
-      t.nbr = ctx->d_tl_nbr;
-      t.poff = ctx->d_tl_poff;
-      t.pslot = ctx->d_tl_pslot;
-      t.ppart = static_cast<const float2*>(ctx->d_piece_part);
-      t.items = static_cast<const TItem*>(ctx->d_titems);
-      t.n_items = ctx->n_titems;
-      a.supers = static_cast<const PrSuper*>(ctx->d_tsupers);
-      a.partial = static_cast<double2*>(ctx->d_tpartial);
-      a.n_super = ctx->n_tsuper;
-      a.super_blocks = static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_tsuper, kThreads)));
-      hipLaunchKernelGGL(k_tile, dim3(std::max(1, ctx->cu_count)), dim3(kFusedThreads), 0, s,
-                         static_cast<const TileChunk*>(ctx->d_tchunks), ctx->n_chunks, ctx->d_twin, ctx->d_tsrc,
-                         cin, a.n_gather, static_cast<float2*>(ctx->d_piece_part), a.ctrl);
-      hipLaunchKernelGGL(k_fused_tiled, dim3(fb), dim3(kFusedThreads), 0, s, a, t, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_kahan_super, dim3(a.super_blocks), dim3(kThreads), 0, s, a, cout, ctx->d_score);
+      TileArgs t;
+      t.cwin = ctx->d_wt_cwin;
+      t.wstart = ctx->d_wt_wstart;
+      t.meta = static_cast<const TileLane*>(ctx->d_wt_meta);
+      t.src = ctx->d_wt_src;
+      t.n_chunks = ctx->n_chunks;
+      t.part = static_cast<float2*>(ctx->d_wt_part);
+      t.pbase = ctx->d_wt_pbase;
+      t.brow = ctx->d_wt_brow;
+      t.ibase = ctx->d_wt_ibase;
+      t.rcnt = ctx->d_wt_rcnt;
+      t.idx = ctx->d_wt_idx;
+      t.n_blocks = ctx->n_row_blocks;
+      const int tb = std::max(1, ctx->cu_count);
+      if (ctx->tiles_window == 16384)
+        hipLaunchKernelGGL(k_tile<16384>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
+      else if (ctx->tiles_window == 40960)
+        hipLaunchKernelGGL(k_tile<40960>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
+      else
+        hipLaunchKernelGGL(k_tile<32768>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
+      a.main_blocks = pr_row_blocks(ctx);
+      a.super_blocks = 0;
+      hipLaunchKernelGGL(k_rows, dim3(a.main_blocks), dim3(kRowThreads), 0, s, a, t, cout, ctx->d_score);
     } else {
       const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
       float* cout = ctx->d_contrib[(it + 1) & 1];
@@ -751,6 +777,8 @@ int pr_fused_blocks(dmm_ctx* ctx) {
   return static_cast<int>(
       std::max<int64_t>(1, std::min<int64_t>(std::max(1, ctx->cu_count), ceil_div(lanes, kFusedThreads))));
 }
+
+int pr_row_blocks(dmm_ctx* ctx) { return 2 * std::max(1, ctx->cu_count); }
 
 int pr_super_blocks(dmm_ctx* ctx) {
   return static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_super, kThreads)));

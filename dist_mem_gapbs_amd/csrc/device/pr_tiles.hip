@@ -1,20 +1,33 @@
 // Host-side construction of the window-tiled compensated PageRank layout.
 //
-// Input: the degree-ordered in-CSR already on the device.  For each row the
-// in-neighbours are sorted and cut by source window (kTileWindow vertices):
-//   window 0                 -> the row's list (LDS hub copy in the row
-//                               kernel)
-//   run of >= kTileMinRun    -> pieces of <= kTilePiece edges, summed by the
-//   in a window w >= 1          tile kernel from an LDS copy of window w,
-//   holding >= kTileMin-        when the window's long runs hold at least
-//   WindowEdges such edges      kTileMinWindowEdges edges
-//   other runs               -> the row's list (gathered from HBM)
-// Pieces are laid out window-major (rows ascending inside a window) and
-// packed into 64-lane chunks without crossing a chunk; a piece's partial
-// slot is its index in that order.  Each row keeps the slots of its pieces
-// in window order, so the row kernel merges them in a fixed order and every
-// iteration is bit-deterministic.
+// Input: the degree-ordered in-CSR already on the device.  Every row's
+// in-neighbours are sorted and cut by source window (window = the
+// `window` consecutive source IDs whose contributions one workgroup holds
+// in LDS).  A row's in-edges inside one window form a run; every run of
+// every window, hub window included, is summed by the tile kernel, so no
+// contribution is gathered from HBM.
+//
+// Tile layout (window-major, rows ascending inside a window):
+//   chunk    = 64 lanes x kLaneEdges edge slots, all of one window;
+//   piece    = the part of a run inside one chunk (a run that crosses a
+//              chunk boundary is cut there), its sum is one partial;
+//   per lane = kLaneEdges 16-bit source offsets, an end mask (bit j: slot j
+//              is the last edge of a piece), the count of filled slots and
+//              the output slot of the first piece ending in the lane.
+// A lane never holds edges of two row blocks (it is padded at a block
+// boundary), so the pieces ending in one lane have consecutive slots.
+//
+// Partial layout (block-major): rows are grouped into row blocks whose
+// partials fit one workgroup's LDS; a block's partials are contiguous,
+// window-major inside the block.  A row's partials within its block are
+// listed in window order (16-bit offsets), so the row kernel stages the
+// block's partials in LDS with coalesced loads and merges every row's
+// partials in a fixed order: iterations are bit-deterministic.  Rows with
+// many runs or many in-edges form blocks of their own; their partials are
+// contiguous and need no list.
 #include <algorithm>
+#include <cmath>
+#include <string>
 #include <vector>
 
 #include "../host/parallel.h"
@@ -24,10 +37,22 @@ namespace dmm {
 
 namespace {
 
-struct PieceRec {
+struct RunRec {
   int32_t row;
   int32_t len;
   int64_t begin;  // first edge in the sorted neighbour array
+};
+
+struct PieceRec {
+  int32_t row;
+  int32_t block;
+};
+
+struct WinPack {
+  std::vector<uint16_t> src;     // kChunkEdges per chunk
+  std::vector<uint32_t> mask;    // per lane: end mask | filled slots << 16
+  std::vector<int64_t> first;    // per lane: first piece ending there (-1: none)
+  std::vector<PieceRec> pieces;  // in window order
 };
 
 template <typename T>
@@ -40,186 +65,449 @@ int upload_vec(dmm_ctx* ctx, void** dst, const std::vector<T>& v, const char* wh
 
 }  // namespace
 
+// Builds the tile layout on the host from an in-CSR of n rows whose
+// sources lie in [0, n_gather); rows at and above zb have no in-edges.
+// nbr is sorted row by row in place.  Returns DMM_OK or an error code
+// (message in *err).
+int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off, int32_t* nbr, int64_t W,
+                     TileHost* out, std::string* err) {
+  const int nt = resolve_threads(0);
+  if (W != 16384 && W != 32768 && W != 40960) {
+    *err = "PageRank tile window is not 16384, 32768 or 40960";
+    return DMM_ERR_INVALID;
+  }
+  if (zb < 0 || zb > n) {
+    *err = "bad in-degree-0 tail";
+    return DMM_ERR_INVALID;
+  }
+  const int32_t nwin = static_cast<int32_t>(std::max<int64_t>(1, (n_gather + W - 1) / W));
+  std::vector<int32_t> runs(std::max<int64_t>(1, zb), 0);
+  parallel_dynamic(zb, nt, 4096, [&](int64_t r) {
+    int32_t* b = nbr + off[r];
+    int32_t* e = nbr + off[r + 1];
+    std::sort(b, e);
+    int32_t c = 0;
+    for (int32_t* p = b; p < e;) {
+      const int64_t w = *p / W;
+      while (p < e && *p / W == w) p++;
+      c++;
+    }
+    runs[r] = c;
+  });
+  // 2. Row blocks over [0, zb): consecutive rows holding at most
+  //    kBlockRuns runs; a row with kSoloRuns runs or kSoloEdges in-edges
+  //    (or more) is a block of its own.
+  std::vector<int32_t> brow{0};
+  {
+    int64_t cur = 0;
+    for (int64_t r = 0; r < zb; r++) {
+      const bool solo = runs[r] >= kSoloRuns || off[r + 1] - off[r] >= kSoloEdges;
+      if (solo) {
+        if (brow.back() != r) brow.push_back(static_cast<int32_t>(r));
+        brow.push_back(static_cast<int32_t>(r + 1));
+        cur = 0;
+        continue;
+      }
+      if (cur + runs[r] > kBlockRuns && brow.back() != r) {
+        brow.push_back(static_cast<int32_t>(r));
+        cur = 0;
+      }
+      cur += runs[r];
+    }
+    if (brow.back() != zb) brow.push_back(static_cast<int32_t>(zb));
+  }
+  const int64_t nb = static_cast<int64_t>(brow.size()) - 1;
+  std::vector<int32_t> block_of(std::max<int64_t>(1, zb));
+  for (int64_t b = 0; b < nb; b++)
+    for (int64_t r = brow[b]; r < brow[b + 1]; r++) block_of[r] = static_cast<int32_t>(b);
+  // 3. Runs per window, rows ascending (thread ranges are row ranges).
+  std::vector<std::vector<std::vector<RunRec>>> recs(nt, std::vector<std::vector<RunRec>>(nwin));
+  parallel_ranges(zb, nt, [&](int t, int64_t r0, int64_t r1) {
+    for (int64_t r = r0; r < r1; r++) {
+      const int32_t* b = nbr + off[r];
+      const int32_t* e = nbr + off[r + 1];
+      for (const int32_t* p = b; p < e;) {
+        const int64_t w = *p / W;
+        const int32_t* q = p;
+        while (q < e && *q / W == w) q++;
+        recs[t][w].push_back({static_cast<int32_t>(r), static_cast<int32_t>(q - p), p - nbr});
+        p = q;
+      }
+    }
+  });
+  // 4. Pack every window into chunks.
+  std::vector<WinPack> packs(nwin);
+  parallel_dynamic(nwin, nt, 1, [&](int64_t w) {
+    WinPack& o = packs[w];
+    const int64_t base = w * W;
+    int64_t ch = -1;
+    int lane = 64, slot = 0, prev_block = -1;
+    int64_t cur_piece = -1, last_lane = -1;
+    int last_slot = -1;
+    auto new_chunk = [&]() {
+      ch++;
+      lane = 0;
+      slot = 0;
+      o.src.resize((ch + 1) * kChunkEdges, 0);
+      o.mask.resize((ch + 1) * 64, 0u);
+      o.first.resize((ch + 1) * 64, -1);
+    };
+    auto end_piece = [&]() {
+      o.mask[last_lane] |= 1u << last_slot;
+      if (o.first[last_lane] < 0) o.first[last_lane] = cur_piece;
+    };
+    for (int t = 0; t < nt; t++)
+      for (const RunRec& rr : recs[t][w]) {
+        const int32_t blk = block_of[rr.row];
+        if (slot == kLaneEdges) {
+          lane++;
+          slot = 0;
+        }
+        if (slot > 0 && blk != prev_block) {  // a lane holds one block's edges
+          lane++;
+          slot = 0;
+        }
+        if (lane >= 64) new_chunk();
+        prev_block = blk;
+        o.pieces.push_back({rr.row, blk});
+        cur_piece = static_cast<int64_t>(o.pieces.size()) - 1;
+        for (int32_t k = 0; k < rr.len; k++) {
+          if (slot == kLaneEdges) {
+            lane++;
+            slot = 0;
+            if (lane == 64) {  // the run crosses a chunk boundary: cut it
+              end_piece();
+              new_chunk();
+              o.pieces.push_back({rr.row, blk});
+              cur_piece++;
+            }
+          }
+          const int64_t li = ch * 64 + lane;
+          o.src[li * kLaneEdges + slot] = static_cast<uint16_t>(nbr[rr.begin + k] - base);
+          o.mask[li] = (o.mask[li] & 0xFFFFu) | (static_cast<uint32_t>(slot + 1) << 16);
+          last_lane = li;
+          last_slot = slot;
+          slot++;
+        }
+        end_piece();
+      }
+  });
+  recs.clear();
+  recs.shrink_to_fit();
+  // 5. Partial slots: block-major, window-major inside a block.
+  std::vector<int64_t> btot(nb + 1, 0);
+  for (int32_t w = 0; w < nwin; w++)
+    for (const PieceRec& p : packs[w].pieces) btot[p.block]++;
+  std::vector<int64_t> pbase(nb + 1, 0);
+  for (int64_t b = 0; b < nb; b++) pbase[b + 1] = pbase[b] + btot[b];
+  const int64_t npieces = pbase[nb];
+  if (npieces >= (int64_t(1) << 31)) {
+    *err = "PageRank tile partials exceed 2^31";
+    return DMM_ERR_CAPACITY;
+  }
+  std::vector<int64_t> cursor(pbase.begin(), pbase.end() - 1);
+  std::vector<std::vector<int32_t>> ppos(nwin);  // slot of every piece
+  for (int32_t w = 0; w < nwin; w++) {
+    const std::vector<PieceRec>& pc = packs[w].pieces;
+    std::vector<int32_t>& pos = ppos[w];
+    pos.resize(pc.size());
+    for (size_t i = 0; i < pc.size(); i++) pos[i] = static_cast<int32_t>(cursor[pc[i].block]++);
+  }
+  // 6. Row lists of multi-row blocks: each row's partial offsets inside its
+  //    block, in window order.
+  std::vector<uint16_t> rcnt(std::max<int64_t>(1, zb), 0);
+  std::vector<char> multi(nb, 0);
+  for (int64_t b = 0; b < nb; b++) multi[b] = brow[b + 1] - brow[b] > 1;
+  for (int32_t w = 0; w < nwin; w++)
+    for (const PieceRec& p : packs[w].pieces)
+      if (multi[p.block]) {
+        if (rcnt[p.row] == 0xFFFF) {
+          *err = "a row has too many PageRank tile pieces";
+          return DMM_ERR_CAPACITY;
+        }
+        rcnt[p.row]++;
+      }
+  std::vector<int64_t> ibase(nb + 1, 0), rstart(std::max<int64_t>(1, zb), 0);
+  for (int64_t b = 0; b < nb; b++) {
+    int64_t c = 0;
+    if (multi[b]) {
+      if (btot[b] > 0xFFFF) {
+        *err = "a row block has too many PageRank tile pieces";
+        return DMM_ERR_CAPACITY;
+      }
+      for (int64_t r = brow[b]; r < brow[b + 1]; r++) {
+        rstart[r] = ibase[b] + c;
+        c += rcnt[r];
+      }
+    }
+    ibase[b + 1] = ibase[b] + c;
+  }
+  std::vector<uint16_t> idx(std::max<int64_t>(1, ibase[nb]));
+  for (int32_t w = 0; w < nwin; w++) {
+    const std::vector<PieceRec>& pc = packs[w].pieces;
+    for (size_t i = 0; i < pc.size(); i++)
+      if (multi[pc[i].block]) idx[rstart[pc[i].row]++] = static_cast<uint16_t>(ppos[w][i] - pbase[pc[i].block]);
+  }
+  // 7. Chunks in window order, with each lane's first output slot.
+  std::vector<int64_t> wstart(nwin + 1, 0);
+  for (int32_t w = 0; w < nwin; w++) wstart[w + 1] = wstart[w] + static_cast<int64_t>(packs[w].mask.size() / 64);
+  const int64_t nchunks = wstart[nwin];
+  std::vector<int32_t> cwin(std::max<int64_t>(1, nchunks));
+  std::vector<uint16_t> src(std::max<int64_t>(1, nchunks * kChunkEdges));
+  std::vector<TileLane> meta(std::max<int64_t>(1, nchunks * 64));
+  int64_t filled = 0;
+  parallel_dynamic(nwin, nt, 1, [&](int64_t w) {
+    const WinPack& o = packs[w];
+    const int64_t c0 = wstart[w], nc = wstart[w + 1] - c0;
+    for (int64_t c = 0; c < nc; c++) cwin[c0 + c] = static_cast<int32_t>(w);
+    std::copy(o.src.begin(), o.src.end(), src.begin() + c0 * kChunkEdges);
+    for (int64_t l = 0; l < nc * 64; l++)
+      meta[c0 * 64 + l] = TileLane{o.mask[l], o.first[l] < 0 ? 0u : static_cast<uint32_t>(ppos[w][o.first[l]])};
+  });
+  for (int32_t w = 0; w < nwin; w++)
+    for (uint32_t mk : packs[w].mask) filled += mk >> 16;
+  packs.clear();
+  packs.shrink_to_fit();
+  out->cwin.swap(cwin);
+  out->wstart.swap(wstart);
+  out->meta.swap(meta);
+  out->src.swap(src);
+  out->pbase.swap(pbase);
+  out->brow.swap(brow);
+  out->ibase.swap(ibase);
+  out->rcnt.swap(rcnt);
+  out->idx.swap(idx);
+  out->n_pieces = npieces;
+  out->slots_filled = filled;
+  out->window = W;
+  out->n_windows = nwin;
+  return DMM_OK;
+}
+
 int pr_tiles_build(dmm_ctx* ctx) {
   const int64_t n = ctx->nloc;
   const int64_t m = ctx->m_in;
-  const int nt = resolve_threads(0);
-  // 1. The device CSR back on the host, rows sorted.
   std::vector<int64_t> off(n + 1);
   std::vector<int32_t> nbr(std::max<int64_t>(1, m));
   DMM_HIP_CHECK(ctx, hipMemcpy(off.data(), ctx->d_in_off, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
   if (m > 0) DMM_HIP_CHECK(ctx, hipMemcpy(nbr.data(), ctx->d_in_nbr, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
-  const int32_t nwin = static_cast<int32_t>((ctx->n + kTileWindow - 1) / kTileWindow);
-  // Sort every row; count per window the edges of runs long enough to tile.
-  std::vector<std::vector<int64_t>> wlong(nt, std::vector<int64_t>(nwin, 0));
-  parallel_ranges(n, nt, [&](int t, int64_t r0, int64_t r1) {
-    for (int64_t r = r0; r < r1; r++) {
-      int32_t* b = nbr.data() + off[r];
-      int32_t* e = nbr.data() + off[r + 1];
-      std::sort(b, e);
-      for (int32_t* p = b; p < e;) {
-        const int32_t w = *p / kTileWindow;
-        int32_t* q = p;
-        while (q < e && *q / kTileWindow == w) q++;
-        if (w > 0 && q - p >= kTileMinRun) wlong[t][w] += q - p;
-        p = q;
-      }
-    }
-  });
-  std::vector<char> tiled(nwin, 0);
-  for (int32_t w = 1; w < nwin; w++) {
-    int64_t s = 0;
-    for (int t = 0; t < nt; t++) s += wlong[t][w];
-    tiled[w] = s >= kTileMinWindowEdges;
-  }
-  std::vector<int64_t> lcnt(n + 1, 0), pcnt(n + 1, 0);
-  parallel_dynamic(n, nt, 1024, [&](int64_t r) {
-    const int32_t* b = nbr.data() + off[r];
-    const int32_t* e = nbr.data() + off[r + 1];
-    int64_t lc = 0, pc = 0;
-    for (const int32_t* p = b; p < e;) {
-      const int32_t w = *p / kTileWindow;
-      const int32_t* q = p;
-      while (q < e && *q / kTileWindow == w) q++;
-      const int64_t len = q - p;
-      if (!tiled[w] || len < kTileMinRun)
-        lc += len;
-      else
-        pc += (len + kTilePiece - 1) / kTilePiece;
-      p = q;
-    }
-    lcnt[r] = lc;
-    pcnt[r] = pc;
-  });
-  // 2. Row offsets of the lists and of the piece slots.
-  std::vector<int64_t> loff(n + 1, 0), poff(n + 1, 0);
-  for (int64_t r = 0; r < n; r++) {
-    loff[r + 1] = loff[r] + lcnt[r];
-    poff[r + 1] = poff[r] + pcnt[r];
-  }
-  const int64_t nlist = loff[n], npieces = poff[n];
-  // 3. Lists, and per thread and window the pieces in row order.
-  std::vector<int32_t> lnbr(std::max<int64_t>(1, nlist));
-  std::vector<std::vector<std::vector<PieceRec>>> recs(nt, std::vector<std::vector<PieceRec>>(nwin));
-  parallel_ranges(n, nt, [&](int t, int64_t r0, int64_t r1) {
-    for (int64_t r = r0; r < r1; r++) {
-      const int32_t* b = nbr.data() + off[r];
-      const int32_t* e = nbr.data() + off[r + 1];
-      int64_t lo = loff[r];
-      for (const int32_t* p = b; p < e;) {
-        const int32_t w = *p / kTileWindow;
-        const int32_t* q = p;
-        while (q < e && *q / kTileWindow == w) q++;
-        const int64_t len = q - p;
-        if (!tiled[w] || len < kTileMinRun) {
-          for (const int32_t* x = p; x < q; x++) lnbr[lo++] = *x;
-        } else {
-          for (int64_t s = 0; s < len; s += kTilePiece)
-            recs[t][w].push_back({static_cast<int32_t>(r), static_cast<int32_t>(std::min<int64_t>(kTilePiece, len - s)),
-                                  (p - nbr.data()) + s});
-        }
-        p = q;
-      }
-    }
-  });
-  // 4. Window-major slots and chunk counts.
-  std::vector<int64_t> wpieces(nwin + 1, 0), wchunks(nwin + 1, 0);
-  parallel_dynamic(nwin, nt, 1, [&](int64_t w) {
-    int64_t pieces = 0, chunks = 0, fill = kTilePiece;
-    for (int t = 0; t < nt; t++)
-      for (const PieceRec& pr : recs[t][w]) {
-        pieces++;
-        if (fill + pr.len > kTilePiece) {
-          chunks++;
-          fill = 0;
-        }
-        fill += pr.len;
-      }
-    wpieces[w + 1] = pieces;
-    wchunks[w + 1] = chunks;
-  });
-  for (int32_t w = 0; w < nwin; w++) {
-    wpieces[w + 1] += wpieces[w];
-    wchunks[w + 1] += wchunks[w];
-  }
-  const int64_t nchunks = wchunks[nwin];
-  // 5. Chunks, source offsets and each row's slots (window order).
-  std::vector<TileChunk> chunks(std::max<int64_t>(1, nchunks));
-  std::vector<uint16_t> src(std::max<int64_t>(1, nchunks * 64), 0xFFFF);
-  std::vector<int32_t> pslot(std::max<int64_t>(1, npieces));
-  std::vector<int64_t> pnext(poff.begin(), poff.end() - 1);  // next free slot entry per row
-  // Rows' slot entries must be filled in window order; windows are
-  // processed in order, each row's entries appended as its pieces appear.
-  for (int32_t w = 0; w < nwin; w++) {
-    int64_t c = wchunks[w] - 1;
-    int fill = kTilePiece;
-    int64_t slot = wpieces[w];
-    const int64_t base = int64_t(w) * kTileWindow;
-    for (int t = 0; t < nt; t++)
-      for (const PieceRec& pr : recs[t][w]) {
-        if (fill + pr.len > kTilePiece) {
-          c++;
-          fill = 0;
-          chunks[c] = {w, static_cast<int32_t>(slot), 0, 0};
-        }
-        chunks[c].head |= 1ull << fill;
-        for (int j = 0; j < pr.len; j++) {
-          src[c * 64 + fill + j] = static_cast<uint16_t>(nbr[pr.begin + j] - base);
-          chunks[c].valid |= 1ull << (fill + j);
-        }
-        fill += pr.len;
-        pslot[pnext[pr.row]++] = static_cast<int32_t>(slot);
-        slot++;
-      }
-  }
-  recs.clear();
-  // 6. Row-kernel items over the lists: heavy rows whole, long rows in
-  //    kPrSegment-edge segments; a split row's pieces are spread over its
-  //    segments in contiguous ranges.
-  std::vector<TItem> items;
-  std::vector<PrSuper> supers;
-  int32_t slots = 0;
-  for (int64_t r = 0; r < n; r++) {
-    const int64_t b = loff[r], e = loff[r + 1];
-    if (e - b < ctx->heavy_threshold) continue;
-    const int64_t pb = poff[r], pe = poff[r + 1];
-    if (e - b < kPrSplit) {
-      items.push_back({static_cast<int32_t>(r), -1, b, e, pb, pe});
-      continue;
-    }
-    const int64_t k = (e - b + kPrSegment - 1) / kPrSegment;
-    PrSuper sp{static_cast<int32_t>(r), slots, static_cast<int32_t>(k), 0};
-    for (int64_t j = 0; j < k; j++) {
-      const int64_t sb = b + j * kPrSegment, se = std::min(e, sb + kPrSegment);
-      items.push_back({static_cast<int32_t>(r), slots++, sb, se, pb + (pe - pb) * j / k, pb + (pe - pb) * (j + 1) / k});
-    }
-    supers.push_back(sp);
-  }
-  // 7. Upload.
-  int rc;
-  std::vector<int64_t> twin(wchunks.begin(), wchunks.end());
-  if ((rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_tl_off), loff, "tiled PR row lists")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_tl_nbr), lnbr, "tiled PR row lists")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_tl_poff), poff, "tiled PR piece lists")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_tl_pslot), pslot, "tiled PR piece lists")) ||
-      (rc = upload_vec(ctx, &ctx->d_titems, items, "tiled PR items")) ||
-      (rc = upload_vec(ctx, &ctx->d_tsupers, supers, "tiled PR split rows")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_tsrc), src, "tiled PR chunk sources")) ||
-      (rc = upload_vec(ctx, &ctx->d_tchunks, chunks, "tiled PR chunks")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_twin), twin, "tiled PR windows")))
+  TileHost th;
+  std::string err;
+  int rc = build_tiles_host(n, ctx->pr_zero_begin, ctx->n, off.data(), nbr.data(), ctx->tuning.pr_window, &th, &err);
+  if (rc) return set_err(ctx, rc, "%s", err.c_str());
+  std::vector<int64_t>().swap(off);
+  std::vector<int32_t>().swap(nbr);
+  if ((rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_cwin), th.cwin, "PR tile chunk windows")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_wstart), th.wstart, "PR tile window starts")) ||
+      (rc = upload_vec(ctx, &ctx->d_wt_meta, th.meta, "PR tile lanes")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_src), th.src, "PR tile sources")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_pbase), th.pbase, "PR row block partials")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_brow), th.brow, "PR row blocks")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_ibase), th.ibase, "PR row block lists")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_rcnt), th.rcnt, "PR row piece counts")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")))
     return rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_tpartial, 16 * std::max<int64_t>(1, slots), "tiled PR segment partials")) ||
-      (rc = dev_alloc(ctx, &ctx->d_piece_part, 8 * std::max<int64_t>(1, npieces), "tiled PR piece partials")))
-    return rc;
-  ctx->n_titems = static_cast<int64_t>(items.size());
-  ctx->n_tsuper = static_cast<int64_t>(supers.size());
-  ctx->n_chunks = nchunks;
-  ctx->n_windows = nwin;
-  ctx->n_pieces = npieces;
-  ctx->list_edges = nlist;
-  ctx->tile_edges = m - nlist;
+  if ((rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) return rc;
+  ctx->n_chunks = static_cast<int64_t>(th.wstart.back());
+  ctx->n_windows = th.n_windows;
+  ctx->n_pieces = th.n_pieces;
+  ctx->n_row_blocks = static_cast<int64_t>(th.brow.size()) - 1;
+  ctx->tile_slots_filled = th.slots_filled;
+  ctx->tiles_window = th.window;
   ctx->tiles_ready = true;
   return DMM_OK;
 }
 
+namespace {
+
+// Host restatement of the compensated sum (the device's Comp): TwoSum of
+// the running sum and the new term, errors accumulated separately.
+struct HostComp {
+  float s = 0.0f, c = 0.0f;
+  void merge(float s2, float c2) {
+    const float t = s + s2;
+    const float bb = t - s;
+    const float e = (s - (t - bb)) + (s2 - bb);
+    s = t;
+    c = (c + c2) + e;
+  }
+  float value() const { return s + c; }
+};
+
+// The device's fixed butterfly over 64 lanes (FusedAcc<float>::reduce<64>).
+HostComp butterfly64(HostComp* v) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    HostComp nv[64];
+    for (int l = 0; l < 64; l++) {
+      const HostComp& p = v[l ^ d];
+      if (l & d) {
+        HostComp o = p;
+        o.merge(v[l].s, v[l].c);
+        nv[l] = o;
+      } else {
+        nv[l] = v[l];
+        nv[l].merge(p.s, p.c);
+      }
+    }
+    for (int l = 0; l < 64; l++) v[l] = nv[l];
+  }
+  return v[0];
+}
+
+}  // namespace
+
 }  // namespace dmm
+
+// Host check of the tile layout (no GPU): builds it, verifies every index
+// the kernels use against its buffer, and replays one iteration's sums with
+// the kernels' arithmetic (k_tile, k_rows) on the host.
+extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off, const int32_t* nbr,
+                                  int64_t window, const float* contrib, float* incoming, int64_t* stats) {
+  using namespace dmm;
+  if (n < 0 || !off || (off[n] > 0 && !nbr) || !contrib || !incoming || !stats)
+    return set_err(nullptr, DMM_ERR_INVALID, "dmm_pr_tiles_check: bad arguments");
+  const int64_t m = off[n];
+  for (int64_t e = 0; e < m; e++)
+    if (nbr[e] < 0 || nbr[e] >= n_gather) return set_err(nullptr, DMM_ERR_GRAPH, "source outside [0, n_gather)");
+  std::vector<int32_t> sorted(nbr, nbr + m);
+  TileHost th;
+  std::string err;
+  int rc = build_tiles_host(n, zb, n_gather, off, sorted.data(), window, &th, &err);
+  if (rc) return set_err(nullptr, rc, "%s", err.c_str());
+  const int64_t nchunks = th.wstart.back(), np = th.n_pieces, nb = static_cast<int64_t>(th.brow.size()) - 1;
+  auto bad = [](const char* what) { return set_err(nullptr, DMM_ERR_GRAPH, "tile layout: %s", what); };
+  if (static_cast<int64_t>(th.cwin.size()) < nchunks || static_cast<int64_t>(th.meta.size()) < nchunks * 64 ||
+      static_cast<int64_t>(th.src.size()) < nchunks * kChunkEdges || th.pbase.back() != np || th.brow[0] != 0 ||
+      th.brow.back() != zb)
+    return bad("sizes");
+  std::vector<char> written(std::max<int64_t>(1, np), 0);
+  std::vector<float> px(std::max<int64_t>(1, np)), py(std::max<int64_t>(1, np));
+  int64_t edges = 0;
+  // k_tile, chunk by chunk.
+  for (int32_t w = 0; w < th.n_windows; w++) {
+    const int64_t base = int64_t(w) * window;
+    const int64_t cnt_w = std::min<int64_t>(window, n_gather - base);
+    for (int64_t c = th.wstart[w]; c < th.wstart[w + 1]; c++) {
+      if (th.cwin[c] != w) return bad("chunk window");
+      double tail[64];
+      bool has_end[64];
+      float x[64][kLaneEdges];
+      for (int l = 0; l < 64; l++) {
+        const TileLane mt = th.meta[c * 64 + l];
+        const uint32_t ends = mt.x & 0xFFFFu;
+        const int cnt = static_cast<int>(mt.x >> 16);
+        if (cnt > kLaneEdges || (cnt < kLaneEdges && (ends >> cnt) != 0)) return bad("lane count / end mask");
+        if (ends && (int64_t(mt.y) + __builtin_popcount(ends) > np)) return bad("output slot");
+        edges += cnt;
+        tail[l] = 0.0;
+        for (int j = 0; j < kLaneEdges; j++) {
+          const uint16_t s = th.src[c * kChunkEdges + l * kLaneEdges + j];
+          if (s >= cnt_w && j < cnt) return bad("source offset");
+          x[l][j] = j < cnt ? contrib[base + s] : 0.0f;
+          tail[l] += static_cast<double>(x[l][j]);
+          if ((ends >> j) & 1u) tail[l] = 0.0;
+        }
+        has_end[l] = ends != 0;
+      }
+      if ((th.meta[c * 64].x & 0xFFFFu) == 0 && (th.meta[c * 64].x >> 16) == 0 && c == th.wstart[w])
+        return bad("empty first lane");
+      // Carries: the device's segmented scan (seg_scan in pr.hip), step for
+      // step, so the FP64 sums round identically.
+      double sc[64];
+      int seg[64];
+      {
+        int h = -1;
+        for (int l = 0; l < 64; l++) {
+          if (has_end[l]) h = l;
+          seg[l] = h < 0 ? 0 : h;
+          sc[l] = tail[l];
+        }
+        for (int d = 1; d <= 8; d <<= 1) {
+          double nv[64];
+          for (int l = 0; l < 64; l++)
+            nv[l] = ((l & 15) >= d && l - d >= seg[l]) ? sc[l] + sc[l - d] : sc[l];
+          std::copy(nv, nv + 64, sc);
+        }
+        double nv[64];
+        for (int l = 0; l < 64; l++)
+          nv[l] = (((l >> 4) & 1) && (l & ~15) - 1 >= seg[l]) ? sc[l] + sc[(l & ~15) - 1] : sc[l];
+        std::copy(nv, nv + 64, sc);
+        for (int l = 0; l < 64; l++) nv[l] = (l >= 32 && 31 >= seg[l]) ? sc[l] + sc[31] : sc[l];
+        std::copy(nv, nv + 64, sc);
+      }
+      for (int l = 0; l < 64; l++) {
+        const TileLane mt = th.meta[c * 64 + l];
+        const uint32_t ends = mt.x & 0xFFFFu;
+        double acc = l == 0 ? 0.0 : sc[l - 1];
+        uint32_t slot = mt.y;
+        for (int j = 0; j < kLaneEdges; j++) {
+          acc += static_cast<double>(x[l][j]);
+          if ((ends >> j) & 1u) {
+            if (written[slot]) return bad("slot written twice");
+            written[slot] = 1;
+            px[slot] = static_cast<float>(acc);
+            py[slot] = static_cast<float>(acc - static_cast<double>(px[slot]));
+            slot++;
+            acc = 0.0;
+          }
+        }
+      }
+      // Every piece ends inside its chunk: the last filled slot ends one,
+      // and filled slots are a prefix of every lane up to it.
+      int last = -1;
+      for (int l = 0; l < 64; l++)
+        if ((th.meta[c * 64 + l].x >> 16) != 0) last = l;
+      if (last < 0) return bad("empty chunk");
+      for (int l = 0; l < last; l++) {
+        const uint32_t cl = th.meta[c * 64 + l].x >> 16;
+        // Only a block boundary may leave a lane short, right after a piece end.
+        if (cl < kLaneEdges && cl > 0 && !((th.meta[c * 64 + l].x >> (cl - 1)) & 1u)) return bad("short lane");
+      }
+      const uint32_t cl = th.meta[c * 64 + last].x >> 16;
+      if (!((th.meta[c * 64 + last].x >> (cl - 1)) & 1u)) return bad("piece open at chunk end");
+    }
+  }
+  if (edges != m) return bad("edge count");
+  for (int64_t p = 0; p < np; p++)
+    if (!written[p]) return bad("slot never written");
+  // k_rows.
+  int64_t max_block = 0;
+  for (int64_t b = 0; b < nb; b++) {
+    const int32_t r0 = th.brow[b], r1 = th.brow[b + 1];
+    const int64_t p0 = th.pbase[b], p1 = th.pbase[b + 1];
+    if (r1 <= r0 || p1 < p0) return bad("row block bounds");
+    max_block = std::max(max_block, p1 - p0);
+    if (r1 - r0 == 1) {
+      HostComp lanes[kRowThreads];
+      for (int t = 0; t < kRowThreads; t++)
+        for (int64_t p = p0 + t; p < p1; p += kRowThreads) lanes[t].merge(px[p], py[p]);
+      HostComp wv[kRowThreads / 64];
+      for (int w = 0; w < kRowThreads / 64; w++) wv[w] = butterfly64(lanes + 64 * w);
+      HostComp tot = wv[0];
+      for (int w = 1; w < kRowThreads / 64; w++) tot.merge(wv[w].s, wv[w].c);
+      incoming[r0] = tot.value();
+      continue;
+    }
+    int64_t ib = th.ibase[b];
+    for (int32_t r = r0; r < r1; r++) {
+      HostComp acc;
+      for (int k = 0; k < th.rcnt[r]; k++) {
+        if (ib + k >= th.ibase[b + 1]) return bad("row list beyond its block");
+        const int64_t q = th.idx[ib + k];
+        if (q >= p1 - p0) return bad("partial offset beyond its block");
+        acc.merge(px[p0 + q], py[p0 + q]);
+      }
+      ib += th.rcnt[r];
+      incoming[r] = acc.value();
+    }
+    if (ib != th.ibase[b + 1]) return bad("row lists do not fill their block");
+  }
+  for (int64_t r = zb; r < n; r++) incoming[r] = 0.0f;
+  stats[0] = nchunks;
+  stats[1] = np;
+  stats[2] = nb;
+  stats[3] = th.slots_filled;
+  stats[4] = th.n_windows;
+  stats[5] = max_block;
+  return DMM_OK;
+}

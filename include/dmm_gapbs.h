@@ -375,6 +375,19 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
                            float* scores_inout, int32_t mode, double* err,
                            double* ms);
 
+/* Host check of the window-tiled layout that DMM_PR_FP32_KAHAN uses on a
+ * degree-ordered whole graph (no GPU involved; for tests).  Builds the
+ * layout of the in-CSR (n rows, sources in [0, n_gather), rows at and above
+ * zb without in-edges) with `window` sources per tile window (16384, 32768
+ * or 40960), checks every index the kernels use against its buffer, and
+ * replays one iteration's per-row incoming sums (float[n]) from the
+ * contributions (float[n_gather]) with the kernels' arithmetic.  stats[6]:
+ * chunks, pieces, row blocks, filled edge slots, windows, largest block.
+ * No reference counterpart: it checks this engine's own layout. */
+int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const int64_t* in_offsets,
+                       const int32_t* in_neighs, int64_t window, const float* contrib,
+                       float* incoming, int64_t* stats);
+
 /* ======================================================================== */
 /* Autotuner                                                                 */
 /* ======================================================================== */

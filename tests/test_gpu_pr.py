@@ -156,3 +156,43 @@ def test_pr_kahan_tiled_and_untiled_layouts(dmm, orc, graph):
             out[layout] = s1
     a, b = out[L.DMM_LAYOUT_DEGREE].astype(np.float64), out[L.DMM_LAYOUT_NATURAL].astype(np.float64)
     assert (np.abs(a - b) / ref64).max() < 4e-7
+
+
+@pytest.mark.parametrize("scale,uniform", [(16, False), (18, False), (15, True)])
+def test_pr_kahan_tiled_equals_host_replay(dmm, graph, scale, uniform):
+    """One compensated iteration on a degree-ordered whole graph (the tile
+    and row kernels) equals, bit for bit, the host replay of the same
+    layout and arithmetic (dmm_pr_tiles_check, itself checked against exact
+    sums in tests/test_pr_tiles.py)."""
+    import ctypes as C
+    import dist_mem_gapbs_amd._lib as L
+    from test_pr_tiles import degree_order
+    g = graph(scale, 16, uniform)
+    n = g.num_nodes
+    oo = np.asarray(g.out_offsets)
+    deg = np.diff(oo)
+    noo, non, zb = degree_order(oo, np.asarray(g.out_neighs))
+    bl = np.zeros(n, np.int64)
+    d = deg.copy()
+    while (d > 0).any():
+        bl += d > 0
+        d >>= 1
+    order = np.lexsort((np.arange(n), -bl))  # internal row -> original ID
+    init = np.float32(1.0) / np.float32(n)
+    ndeg = deg[order].astype(np.float32)
+    with np.errstate(divide="ignore"):
+        contrib = (init / ndeg).astype(np.float32)
+    inc = np.empty(n, np.float32)
+    st = np.zeros(6, np.int64)
+    lib = L.load()
+    assert lib.dmm_pr_tiles_check(n, zb, n, L.ptr(noo, C.c_int64), L.ptr(non, C.c_int32), 32768,
+                                  L.ptr(contrib, C.c_float), L.ptr(inc, C.c_float), L.ptr(st, C.c_int64)) == 0
+    base = (np.float32(1.0) - np.float32(0.85)) / np.float32(n)
+    expect_internal = (base + np.float32(0.85) * inc).astype(np.float32)
+    expect = np.empty(n, np.float32)
+    expect[order] = expect_internal
+    with dmm.Context(0) as c:
+        c.upload(g)
+        got, pst = c.PageRankPull(1, 0.0, MODES["kahan"])
+    assert pst.iters == 1
+    assert np.array_equal(got, expect)
