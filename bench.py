@@ -395,9 +395,13 @@ def only_leg(dmm, args):
             out["levels"] = [(st.level_kind[i], st.level_active[i]) for i in range(st.num_levels - 1)]
             out["sweeps"] = st.sweeps
         else:
-            for mode in (L.DMM_PR_FP32_KAHAN, L.DMM_PR_FP32_SERIAL, L.DMM_PR_FP64):
+            names = {"kahan": L.DMM_PR_FP32_KAHAN, "serial": L.DMM_PR_FP32_SERIAL, "fp64": L.DMM_PR_FP64}
+            for name in args.pr_modes.split(","):
+                mode = names[name]
+                ctx.PageRankPull(20, 0.0, mode, read=False)
                 _, pst = ctx.PageRankPull(20, 0.0, mode, read=False)
-                out[f"pr_mode{mode}_iter_ms"] = pst.iter_ms
+                out[f"pr_{name}_iter_ms"] = pst.iter_ms
+                out[f"pr_{name}_hbm_frac"] = pst.alg_bytes_per_iter / (pst.iter_ms / 1e3) / HBM_PEAK
     print(json.dumps(out), flush=True)
 
 
@@ -420,6 +424,8 @@ def main():
     ap.add_argument("--pe-scale", type=int, default=27)
     ap.add_argument("--only", choices=["bfs", "batch", "pr", "bc", "build"],
                     help="run one leg alone on Kronecker --big-scale (for profiling)")
+    ap.add_argument("--pr-modes", default="kahan,serial,fp64",
+                    help="PageRank modes the --only pr leg times (kahan, serial, fp64)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

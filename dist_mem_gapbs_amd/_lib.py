@@ -109,6 +109,8 @@ class Tuning(C.Structure):
         ("batch", BatchParams),
         ("plain_ms", C.c_double),
         ("batch_ms", C.c_double),
+        ("pr_window", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

@@ -79,7 +79,28 @@ void drop_graphs(dmm_ctx* ctx) {
   ctx->batch_graph_key = -1;
 }
 
+// Frees the compensated PageRank tile layout (rebuilt at the next
+// compensated call) and the captured graphs that use it.
+void drop_tiles(dmm_ctx* ctx) {
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  void* bufs[] = {ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_meta, ctx->d_wt_src, ctx->d_wt_pbase,
+                  ctx->d_wt_brow, ctx->d_wt_ibase, ctx->d_wt_rcnt, ctx->d_wt_idx, ctx->d_wt_part};
+  const int64_t held = ctx->tiles_bytes;
+  for (void* p : bufs) dmm::dev_free(p);
+  ctx->dev_bytes -= held;
+  ctx->tiles_bytes = 0;
+  ctx->d_wt_cwin = ctx->d_wt_brow = nullptr;
+  ctx->d_wt_wstart = ctx->d_wt_pbase = ctx->d_wt_ibase = nullptr;
+  ctx->d_wt_meta = ctx->d_wt_part = nullptr;
+  ctx->d_wt_src = ctx->d_wt_rcnt = ctx->d_wt_idx = nullptr;
+  ctx->n_chunks = ctx->n_pieces = ctx->n_row_blocks = ctx->tile_slots_filled = ctx->tiles_window = 0;
+  ctx->n_windows = 0;
+  if (ctx->tiles_ready) drop_graphs(ctx);
+  ctx->tiles_ready = false;
+}
+
 void free_graph_state(dmm_ctx* ctx) {
+  drop_tiles(ctx);
   drop_graphs(ctx);
   void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
                   ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],
@@ -89,8 +110,6 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
                   ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
-                  ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_meta, ctx->d_wt_src, ctx->d_wt_pbase,
-                  ctx->d_wt_brow, ctx->d_wt_ibase, ctx->d_wt_rcnt, ctx->d_wt_idx, ctx->d_wt_part,
                   ctx->d_bsegs, ctx->d_head,
                   ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
                   ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp};
@@ -129,12 +148,6 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bsegs = nullptr;
   ctx->n_bsegs = 0;
   ctx->bsegs_ready = false;
-  ctx->d_wt_cwin = ctx->d_wt_brow = nullptr;
-  ctx->d_wt_wstart = ctx->d_wt_pbase = ctx->d_wt_ibase = nullptr;
-  ctx->d_wt_meta = ctx->d_wt_part = nullptr;
-  ctx->d_wt_src = ctx->d_wt_rcnt = ctx->d_wt_idx = nullptr;
-  ctx->n_chunks = ctx->n_pieces = ctx->n_row_blocks = ctx->tile_slots_filled = ctx->tiles_window = 0;
-  ctx->n_windows = 0;
   ctx->relabeled = false;
   ctx->h_perm.clear();
   ctx->h_perm.shrink_to_fit();
@@ -981,6 +994,12 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
   if (!ctx || !t) return set_err(ctx, DMM_ERR_INVALID, "NULL argument");
   if (t->alpha <= 0 || t->beta <= 0 || t->batch.td_div <= 0 || t->batch.graph_levels <= 0)
     return set_err(ctx, DMM_ERR_INVALID, "tuning values must be positive");
+  if (t->pr_window != 0 && t->pr_window != 16384 && t->pr_window != 32768 && t->pr_window != 40960)
+    return set_err(ctx, DMM_ERR_INVALID, "pr_window must be 16384, 32768 or 40960");
+  if (t->pr_window != 0 && t->pr_window != ctx->tuning.pr_window) {
+    ctx->tuning.pr_window = t->pr_window;
+    drop_tiles(ctx);  // rebuilt with the new window at the next compensated call
+  }
   ctx->tuning.alpha = t->alpha;
   ctx->tuning.beta = t->beta;
   ctx->tuning.batch = t->batch;
@@ -1000,6 +1019,7 @@ int dmm_get_tuning(const dmm_ctx* ctx, dmm_tuning* t) {
   t->batch = ctx->tuning.batch;
   t->plain_ms = ctx->tuning.plain_ms;
   t->batch_ms = ctx->tuning.batch_ms;
+  t->pr_window = ctx->tuning.pr_window;
   return DMM_OK;
 }
 

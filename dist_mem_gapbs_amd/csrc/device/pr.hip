@@ -404,60 +404,79 @@ struct TileArgs {
 
 constexpr int kTileThreads = 1024;
 
+// What one lane reads of a chunk from HBM: its TileLane and its
+// kLaneEdges 16-bit source offsets.
+struct ChunkLoad {
+  TileLane mt;
+  uint4 q0, q1;
+};
+
+__device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, int lane) {
+  ChunkLoad l;
+  l.mt = t.meta[c * 64 + lane];
+  const uint4* s4 = reinterpret_cast<const uint4*>(t.src + c * kChunkEdges + lane * kLaneEdges);
+  l.q0 = s4[0];
+  l.q1 = s4[1];
+  return l;
+}
+
+__device__ __forceinline__ void emit_piece(float2* part, uint32_t slot, double s) {
+  const float hi = static_cast<float>(s);
+  const float lo = static_cast<float>(s - static_cast<double>(hi));
+  part[slot] = make_float2(hi, lo);
+}
+
 // One chunk of the tile pass, run by one wavefront.  Lane i holds slots
 // [i * kLaneEdges, (i + 1) * kLaneEdges) of the chunk; a piece may span
-// lanes but never chunks.  Each lane first sums its edges after its last
-// piece end (all of them if none ends in it); a segmented scan over the
-// lanes (reset at lanes where a piece ends) turns these into the carry each
-// lane receives from the piece still open at its start.  The lane then
-// walks its edges again, adding from the carry and emitting a partial at
-// every piece end.  Values accumulate in FP64 (a piece holds at most
-// kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53) and a
-// piece enters its row's compensated FP32 sum as the unevaluated pair
+// lanes but never chunks.  A lane sums its edges in order, restarting at
+// every piece end: the pieces that start and end inside the lane are
+// emitted at once, the sum up to its first end (`head`) waits for the
+// carry, and the sum after its last end (`tail`; all of its edges if none
+// ends in it) is what it passes on.  A segmented scan of the tails over
+// the lanes (restarted at lanes where a piece ends) gives each lane the
+// carry from the piece still open at its start; the lane then emits its
+// first piece as carry + head.  Values accumulate in FP64 (a piece holds at
+// most kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53) and
+// a piece enters its row's compensated FP32 sum as the unevaluated pair
 // (hi, lo) = (float(s), float(s - hi)).
-__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, int64_t c, int lane) {
-  const TileLane mt = t.meta[c * 64 + lane];
-  const uint4* s4 = reinterpret_cast<const uint4*>(t.src + c * kChunkEdges + lane * kLaneEdges);
-  const uint4 q0 = s4[0], q1 = s4[1];
-  const uint32_t sw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-  const uint32_t ends = mt.x & 0xFFFFu;
-  const int cnt = static_cast<int>(mt.x >> 16);
+__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane) {
+  const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
+  const uint32_t ends = l.mt.x & 0xFFFFu;
+  const int cnt = static_cast<int>(l.mt.x >> 16);
   float x[kLaneEdges];
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++) x[j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++)
     if (j >= cnt) x[j] = 0.0f;
-  double tail = 0.0;
+  const int first_end = ends ? __builtin_ctz(ends) : kLaneEdges;
+  double acc = 0.0, head = 0.0;
+  uint32_t slot = l.mt.y + 1;
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++) {
-    tail += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) tail = 0.0;
+    acc += static_cast<double>(x[j]);
+    if ((ends >> j) & 1u) {
+      if (j == first_end)
+        head = acc;
+      else
+        emit_piece(t.part, slot++, acc);
+      acc = 0.0;
+    }
   }
   const unsigned long long has_end = __ballot(ends != 0u);
   const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
   const unsigned long long h = has_end & upto;
   const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
-  const double carry_out = seg_scan(tail, lane, seg);
-  double acc = __shfl_up(carry_out, 1, 64);
-  if (lane == 0) acc = 0.0;
-  uint32_t slot = mt.y;
-#pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) {
-    acc += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) {
-      const float hi = static_cast<float>(acc);
-      const float lo = static_cast<float>(acc - static_cast<double>(hi));
-      t.part[slot++] = make_float2(hi, lo);
-      acc = 0.0;
-    }
-  }
+  const double carry_out = seg_scan(acc, lane, seg);
+  double carry = __shfl_up(carry_out, 1, 64);
+  if (lane == 0) carry = 0.0;
+  if (ends) emit_piece(t.part, l.mt.y, carry + head);
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
 // range of chunks (window-major); the workgroup copies one window of
 // contributions into LDS at a time and its wavefronts take the window's
-// chunks in turn.
+// chunks in turn, each loading its next chunk before it sums the current.
 template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
@@ -487,7 +506,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
       for (int j = threadIdx.x; j < cnt; j += kTileThreads) win[j] = cin[base + j];
     }
     __syncthreads();
-    for (int64_t cb = c + wave; cb < wend; cb += kWaves) tile_chunk(t, win, cb, lane);
+    int64_t cb = c + wave;
+    if (cb < wend) {
+      ChunkLoad cur = load_chunk(t, cb, lane);
+      for (; cb < wend; cb += kWaves) {
+        // Unconditional (the last one reloads a chunk of the range), so
+        // the wait for `cur` leaves this load in flight.
+        const ChunkLoad nxt = load_chunk(t, min(cb + kWaves, wend - 1), lane);
+        tile_chunk(t, win, cur, lane);
+        cur = nxt;
+      }
+    }
     c = wend;
   }
 }

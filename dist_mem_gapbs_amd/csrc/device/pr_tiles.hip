@@ -293,6 +293,7 @@ int pr_tiles_build(dmm_ctx* ctx) {
   if (m > 0) DMM_HIP_CHECK(ctx, hipMemcpy(nbr.data(), ctx->d_in_nbr, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
   TileHost th;
   std::string err;
+  const int64_t held = ctx->dev_bytes;
   int rc = build_tiles_host(n, ctx->pr_zero_begin, ctx->n, off.data(), nbr.data(), ctx->tuning.pr_window, &th, &err);
   if (rc) return set_err(ctx, rc, "%s", err.c_str());
   std::vector<int64_t>().swap(off);
@@ -305,15 +306,18 @@ int pr_tiles_build(dmm_ctx* ctx) {
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_brow), th.brow, "PR row blocks")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_ibase), th.ibase, "PR row block lists")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_rcnt), th.rcnt, "PR row piece counts")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")))
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")) ||
+      (rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) {
+    ctx->tiles_bytes = ctx->dev_bytes - held;  // what the failed build holds, freed with the layout
     return rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) return rc;
+  }
   ctx->n_chunks = static_cast<int64_t>(th.wstart.back());
   ctx->n_windows = th.n_windows;
   ctx->n_pieces = th.n_pieces;
   ctx->n_row_blocks = static_cast<int64_t>(th.brow.size()) - 1;
   ctx->tile_slots_filled = th.slots_filled;
   ctx->tiles_window = th.window;
+  ctx->tiles_bytes = ctx->dev_bytes - held;
   ctx->tiles_ready = true;
   return DMM_OK;
 }
@@ -389,7 +393,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
     const int64_t cnt_w = std::min<int64_t>(window, n_gather - base);
     for (int64_t c = th.wstart[w]; c < th.wstart[w + 1]; c++) {
       if (th.cwin[c] != w) return bad("chunk window");
-      double tail[64];
+      double tail[64], headv[64];
       bool has_end[64];
       float x[64][kLaneEdges];
       for (int l = 0; l < 64; l++) {
@@ -399,14 +403,32 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         if (cnt > kLaneEdges || (cnt < kLaneEdges && (ends >> cnt) != 0)) return bad("lane count / end mask");
         if (ends && (int64_t(mt.y) + __builtin_popcount(ends) > np)) return bad("output slot");
         edges += cnt;
-        tail[l] = 0.0;
+        // The lane's own pass (tile_chunk in pr.hip): pieces inside the lane
+        // are emitted, the sum up to the first end waits for the carry.
+        double acc = 0.0;
+        headv[l] = 0.0;
+        uint32_t slot = mt.y + 1;
+        bool first = true;
         for (int j = 0; j < kLaneEdges; j++) {
           const uint16_t s = th.src[c * kChunkEdges + l * kLaneEdges + j];
           if (s >= cnt_w && j < cnt) return bad("source offset");
           x[l][j] = j < cnt ? contrib[base + s] : 0.0f;
-          tail[l] += static_cast<double>(x[l][j]);
-          if ((ends >> j) & 1u) tail[l] = 0.0;
+          acc += static_cast<double>(x[l][j]);
+          if ((ends >> j) & 1u) {
+            if (first) {
+              headv[l] = acc;
+              first = false;
+            } else {
+              if (written[slot]) return bad("slot written twice");
+              written[slot] = 1;
+              px[slot] = static_cast<float>(acc);
+              py[slot] = static_cast<float>(acc - static_cast<double>(px[slot]));
+              slot++;
+            }
+            acc = 0.0;
+          }
         }
+        tail[l] = acc;
         has_end[l] = ends != 0;
       }
       if ((th.meta[c * 64].x & 0xFFFFu) == 0 && (th.meta[c * 64].x >> 16) == 0 && c == th.wstart[w])
@@ -437,20 +459,13 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       }
       for (int l = 0; l < 64; l++) {
         const TileLane mt = th.meta[c * 64 + l];
-        const uint32_t ends = mt.x & 0xFFFFu;
-        double acc = l == 0 ? 0.0 : sc[l - 1];
-        uint32_t slot = mt.y;
-        for (int j = 0; j < kLaneEdges; j++) {
-          acc += static_cast<double>(x[l][j]);
-          if ((ends >> j) & 1u) {
-            if (written[slot]) return bad("slot written twice");
-            written[slot] = 1;
-            px[slot] = static_cast<float>(acc);
-            py[slot] = static_cast<float>(acc - static_cast<double>(px[slot]));
-            slot++;
-            acc = 0.0;
-          }
-        }
+        if ((mt.x & 0xFFFFu) == 0) continue;
+        const double s = (l == 0 ? 0.0 : sc[l - 1]) + headv[l];
+        const uint32_t slot = mt.y;
+        if (written[slot]) return bad("slot written twice");
+        written[slot] = 1;
+        px[slot] = static_cast<float>(s);
+        py[slot] = static_cast<float>(s - static_cast<double>(px[slot]));
       }
       // Every piece ends inside its chunk: the last filled slot ends one,
       // and filled slots are a prefix of every lane up to it.

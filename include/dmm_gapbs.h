@@ -400,6 +400,9 @@ typedef struct dmm_tuning {
   dmm_batch_params batch;
   double plain_ms;         /* measured mean DOBFS time with these values   */
   double batch_ms;         /* measured batch time with these values        */
+  int32_t pr_window;       /* compensated PageRank tile window: sources per
+                              LDS copy (16384, 32768 or 40960; 0 = keep)  */
+  int32_t pad;
 } dmm_tuning;
 
 /* Searches alpha/beta and launch shapes on this ctx's graph using the given
