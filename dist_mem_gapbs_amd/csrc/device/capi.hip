@@ -994,8 +994,8 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
   if (!ctx || !t) return set_err(ctx, DMM_ERR_INVALID, "NULL argument");
   if (t->alpha <= 0 || t->beta <= 0 || t->batch.td_div <= 0 || t->batch.graph_levels <= 0)
     return set_err(ctx, DMM_ERR_INVALID, "tuning values must be positive");
-  if (t->pr_window != 0 && t->pr_window != 16384 && t->pr_window != 32768 && t->pr_window != 40960)
-    return set_err(ctx, DMM_ERR_INVALID, "pr_window must be 16384, 32768 or 40960");
+  if (t->pr_window != 0 && t->pr_window != 16384 && t->pr_window != 32768)
+    return set_err(ctx, DMM_ERR_INVALID, "pr_window must be 16384 or 32768");
   if (t->pr_window != 0 && t->pr_window != ctx->tuning.pr_window) {
     ctx->tuning.pr_window = t->pr_window;
     drop_tiles(ctx);  // rebuilt with the new window at the next compensated call

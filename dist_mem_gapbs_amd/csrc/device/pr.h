@@ -28,7 +28,7 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 
 // Window-tiled compensated PageRank (degree-ordered whole graphs).
 // Sources are cut into windows of `pr_window` vertices (a tuning value:
-// 16384, 32768 or 40960 contributions, 64 to 160 KB of LDS).  Every in-edge
+// 16384 or 32768 contributions, 64 or 128 KB of LDS).  Every in-edge
 // sits in a 64-lane chunk of its source's window, kLaneEdges edges per
 // lane, and is summed from an LDS copy of that window; each piece (a row's
 // edges inside one chunk) leaves one (sum, error) partial, and the row

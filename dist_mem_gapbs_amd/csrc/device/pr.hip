@@ -420,26 +420,33 @@ __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, in
   return l;
 }
 
-__device__ __forceinline__ void emit_piece(float2* part, uint32_t slot, double s) {
+// Partials a wavefront stages in LDS before writing them out: (hi, lo,
+// slot) per piece, kStage per wavefront.
+constexpr int kStage = 120;
+
+__device__ __forceinline__ float2 split_pair(double s) {
   const float hi = static_cast<float>(s);
-  const float lo = static_cast<float>(s - static_cast<double>(hi));
-  part[slot] = make_float2(hi, lo);
+  return make_float2(hi, static_cast<float>(s - static_cast<double>(hi)));
 }
 
 // One chunk of the tile pass, run by one wavefront.  Lane i holds slots
 // [i * kLaneEdges, (i + 1) * kLaneEdges) of the chunk; a piece may span
-// lanes but never chunks.  A lane sums its edges in order, restarting at
-// every piece end: the pieces that start and end inside the lane are
-// emitted at once, the sum up to its first end (`head`) waits for the
-// carry, and the sum after its last end (`tail`; all of its edges if none
-// ends in it) is what it passes on.  A segmented scan of the tails over
-// the lanes (restarted at lanes where a piece ends) gives each lane the
-// carry from the piece still open at its start; the lane then emits its
-// first piece as carry + head.  Values accumulate in FP64 (a piece holds at
-// most kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53) and
-// a piece enters its row's compensated FP32 sum as the unevaluated pair
+// lanes but never chunks.  First pass: each lane sums its edges after its
+// last piece end (all of them if none ends in it).  A segmented scan of
+// these tails over the lanes (restarted at lanes where a piece ends) gives
+// each lane the carry of the piece still open at its start.  Second pass:
+// the lane sums its edges again from the carry, restarting at every piece
+// end, where a piece is complete.  Values accumulate in FP64 (a piece holds
+// at most kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53)
+// and a piece enters its row's compensated FP32 sum as the unevaluated pair
 // (hi, lo) = (float(s), float(s - hi)).
-__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane) {
+//
+// The chunk's pieces, numbered in lane order, are staged kStage at a time
+// in the wavefront's LDS area with their output slots (the second pass is
+// repeated per round) and written back by consecutive lanes, so each store
+// instruction writes neighbouring slots.
+__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane,
+                                           float4* stage) {
   const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
   const uint32_t ends = l.mt.x & 0xFFFFu;
   const int cnt = static_cast<int>(l.mt.x >> 16);
@@ -449,19 +456,11 @@ __device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, 
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++)
     if (j >= cnt) x[j] = 0.0f;
-  const int first_end = ends ? __builtin_ctz(ends) : kLaneEdges;
-  double acc = 0.0, head = 0.0;
-  uint32_t slot = l.mt.y + 1;
+  double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++) {
     acc += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) {
-      if (j == first_end)
-        head = acc;
-      else
-        emit_piece(t.part, slot++, acc);
-      acc = 0.0;
-    }
+    if ((ends >> j) & 1u) acc = 0.0;
   }
   const unsigned long long has_end = __ballot(ends != 0u);
   const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
@@ -470,7 +469,41 @@ __device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, 
   const double carry_out = seg_scan(acc, lane, seg);
   double carry = __shfl_up(carry_out, 1, 64);
   if (lane == 0) carry = 0.0;
-  if (ends) emit_piece(t.part, l.mt.y, carry + head);
+  // Piece numbers in lane order; the chunk's total.
+  const uint32_t k_i = __builtin_popcount(ends);
+  const uint32_t incl = wave_incl_scan(k_i);
+  const uint32_t o_i = incl - k_i;
+  const uint32_t total = __shfl(incl, 63, 64);
+  for (uint32_t base = 0; base < total; base += kStage) {
+    // The lane's pieces again (its first one with the carry), each staged
+    // if its number falls in this round.
+    double a = carry;  // the lane's first piece starts from the carry
+    uint32_t q = o_i - base, slot = l.mt.y;
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++) {
+      a += static_cast<double>(x[j]);
+      if ((ends >> j) & 1u) {
+        if (q < static_cast<uint32_t>(kStage)) {
+          const float2 v = split_pair(a);
+          stage[q] = make_float4(v.x, v.y, __uint_as_float(slot), 0.0f);
+        }
+        q++;
+        slot++;
+        a = 0.0;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t n = min(static_cast<uint32_t>(kStage), total - base);
+    for (uint32_t e = lane; e < n; e += 64) {
+      const float4 v = stage[e];
+      t.part[__float_as_uint(v.z)] = make_float2(v.x, v.y);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
@@ -481,11 +514,13 @@ template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
+  __shared__ float4 stage_all[kTileThreads / 64][kStage];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
+  float4* stage = stage_all[wave];
   constexpr int kWaves = kTileThreads / 64;
   for (int64_t c = c0; c < c1;) {
     const int32_t w = t.cwin[c];
@@ -513,7 +548,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
         // Unconditional (the last one reloads a chunk of the range), so
         // the wait for `cur` leaves this load in flight.
         const ChunkLoad nxt = load_chunk(t, min(cb + kWaves, wend - 1), lane);
-        tile_chunk(t, win, cur, lane);
+        tile_chunk(t, win, cur, lane, stage);
         cur = nxt;
       }
     }
@@ -771,8 +806,6 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       const int tb = std::max(1, ctx->cu_count);
       if (ctx->tiles_window == 16384)
         hipLaunchKernelGGL(k_tile<16384>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
-      else if (ctx->tiles_window == 40960)
-        hipLaunchKernelGGL(k_tile<40960>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
       else
         hipLaunchKernelGGL(k_tile<32768>, dim3(tb), dim3(kTileThreads), 0, s, t, cin, a.n_gather, a.ctrl);
       a.main_blocks = pr_row_blocks(ctx);

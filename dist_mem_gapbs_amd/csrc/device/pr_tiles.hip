@@ -72,8 +72,8 @@ int upload_vec(dmm_ctx* ctx, void** dst, const std::vector<T>& v, const char* wh
 int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off, int32_t* nbr, int64_t W,
                      TileHost* out, std::string* err) {
   const int nt = resolve_threads(0);
-  if (W != 16384 && W != 32768 && W != 40960) {
-    *err = "PageRank tile window is not 16384, 32768 or 40960";
+  if (W != 16384 && W != 32768) {
+    *err = "PageRank tile window is not 16384 or 32768";
     return DMM_ERR_INVALID;
   }
   if (zb < 0 || zb > n) {
@@ -393,7 +393,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
     const int64_t cnt_w = std::min<int64_t>(window, n_gather - base);
     for (int64_t c = th.wstart[w]; c < th.wstart[w + 1]; c++) {
       if (th.cwin[c] != w) return bad("chunk window");
-      double tail[64], headv[64];
+      double tail[64];
       bool has_end[64];
       float x[64][kLaneEdges];
       for (int l = 0; l < 64; l++) {
@@ -403,30 +403,15 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         if (cnt > kLaneEdges || (cnt < kLaneEdges && (ends >> cnt) != 0)) return bad("lane count / end mask");
         if (ends && (int64_t(mt.y) + __builtin_popcount(ends) > np)) return bad("output slot");
         edges += cnt;
-        // The lane's own pass (tile_chunk in pr.hip): pieces inside the lane
-        // are emitted, the sum up to the first end waits for the carry.
+        // The lane's first pass (tile_chunk in pr.hip): its sum after its
+        // last piece end.
         double acc = 0.0;
-        headv[l] = 0.0;
-        uint32_t slot = mt.y + 1;
-        bool first = true;
         for (int j = 0; j < kLaneEdges; j++) {
           const uint16_t s = th.src[c * kChunkEdges + l * kLaneEdges + j];
           if (s >= cnt_w && j < cnt) return bad("source offset");
           x[l][j] = j < cnt ? contrib[base + s] : 0.0f;
           acc += static_cast<double>(x[l][j]);
-          if ((ends >> j) & 1u) {
-            if (first) {
-              headv[l] = acc;
-              first = false;
-            } else {
-              if (written[slot]) return bad("slot written twice");
-              written[slot] = 1;
-              px[slot] = static_cast<float>(acc);
-              py[slot] = static_cast<float>(acc - static_cast<double>(px[slot]));
-              slot++;
-            }
-            acc = 0.0;
-          }
+          if ((ends >> j) & 1u) acc = 0.0;
         }
         tail[l] = acc;
         has_end[l] = ends != 0;
@@ -457,15 +442,23 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         for (int l = 0; l < 64; l++) nv[l] = (l >= 32 && 31 >= seg[l]) ? sc[l] + sc[31] : sc[l];
         std::copy(nv, nv + 64, sc);
       }
+      // The lane's second pass: its pieces, the first one from the carry.
       for (int l = 0; l < 64; l++) {
         const TileLane mt = th.meta[c * 64 + l];
-        if ((mt.x & 0xFFFFu) == 0) continue;
-        const double s = (l == 0 ? 0.0 : sc[l - 1]) + headv[l];
-        const uint32_t slot = mt.y;
-        if (written[slot]) return bad("slot written twice");
-        written[slot] = 1;
-        px[slot] = static_cast<float>(s);
-        py[slot] = static_cast<float>(s - static_cast<double>(px[slot]));
+        const uint32_t ends = mt.x & 0xFFFFu;
+        double a = l == 0 ? 0.0 : sc[l - 1];
+        uint32_t slot = mt.y;
+        for (int j = 0; j < kLaneEdges; j++) {
+          a += static_cast<double>(x[l][j]);
+          if ((ends >> j) & 1u) {
+            if (written[slot]) return bad("slot written twice");
+            written[slot] = 1;
+            px[slot] = static_cast<float>(a);
+            py[slot] = static_cast<float>(a - static_cast<double>(px[slot]));
+            slot++;
+            a = 0.0;
+          }
+        }
       }
       // Every piece ends inside its chunk: the last filled slot ends one,
       // and filled slots are a prefix of every lane up to it.

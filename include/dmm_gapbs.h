@@ -378,8 +378,8 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
 /* Host check of the window-tiled layout that DMM_PR_FP32_KAHAN uses on a
  * degree-ordered whole graph (no GPU involved; for tests).  Builds the
  * layout of the in-CSR (n rows, sources in [0, n_gather), rows at and above
- * zb without in-edges) with `window` sources per tile window (16384, 32768
- * or 40960), checks every index the kernels use against its buffer, and
+ * zb without in-edges) with `window` sources per tile window (16384 or
+ * 32768), checks every index the kernels use against its buffer, and
  * replays one iteration's per-row incoming sums (float[n]) from the
  * contributions (float[n_gather]) with the kernels' arithmetic.  stats[6]:
  * chunks, pieces, row blocks, filled edge slots, windows, largest block.
@@ -401,7 +401,7 @@ typedef struct dmm_tuning {
   double plain_ms;         /* measured mean DOBFS time with these values   */
   double batch_ms;         /* measured batch time with these values        */
   int32_t pr_window;       /* compensated PageRank tile window: sources per
-                              LDS copy (16384, 32768 or 40960; 0 = keep)  */
+                              LDS copy (16384 or 32768; 0 = keep)         */
   int32_t pad;
 } dmm_tuning;
 

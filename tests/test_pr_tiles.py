@@ -67,7 +67,7 @@ def check(n, oo, on, zb, window, n_gather=None, seed=1):
 
 
 @pytest.mark.parametrize("scale,uniform", [(12, False), (14, False), (13, True)])
-@pytest.mark.parametrize("window", [16384, 32768, 40960])
+@pytest.mark.parametrize("window", [16384, 32768])
 def test_tiles_degree_layout(dmm, scale, uniform, window):
     g = dmm.build_graph(scale, 16, uniform)
     oo, on, zb = degree_order(np.asarray(g.out_offsets), np.asarray(g.out_neighs))
