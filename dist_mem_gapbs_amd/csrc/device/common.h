@@ -202,13 +202,16 @@ struct dmm_ctx {
   bool tiles_ready = false;
   int32_t* d_wt_cwin = nullptr;     // window of each chunk
   int64_t* d_wt_wstart = nullptr;   // first chunk of each window (+ end)
-  void* d_wt_meta = nullptr;        // TileLane per lane
+  int32_t* d_wt_cbase = nullptr;    // first partial of each chunk (+ end)
+  uint32_t* d_wt_meta = nullptr;    // per lane: end mask | filled slots << 16
   uint16_t* d_wt_src = nullptr;     // kChunkEdges source offsets per chunk
-  int64_t* d_wt_pbase = nullptr;    // first partial of each row block (+ end)
   int32_t* d_wt_brow = nullptr;     // first row of each row block (+ end)
+  int64_t* d_wt_bseg = nullptr;     // first segment of each row block (+ end)
+  int32_t* d_wt_seg_start = nullptr;  // segments: first partial
+  int32_t* d_wt_seg_len = nullptr;    //           partials
   int64_t* d_wt_ibase = nullptr;    // first list entry of each row block (+ end)
   uint16_t* d_wt_rcnt = nullptr;    // partials per row (multi-row blocks)
-  uint16_t* d_wt_idx = nullptr;     // per row, window order: partial offset in its block
+  uint16_t* d_wt_idx = nullptr;     // per row, window order: partial offset in its block's region
   void* d_wt_part = nullptr;        // float2 (sum, error) per piece
   int64_t n_chunks = 0, n_pieces = 0, n_row_blocks = 0, tile_slots_filled = 0, tiles_window = 0;
   int64_t tiles_bytes = 0;          // device bytes the layout holds

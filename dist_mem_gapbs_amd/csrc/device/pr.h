@@ -28,40 +28,41 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 
 // Window-tiled compensated PageRank (degree-ordered whole graphs).
 // Sources are cut into windows of `pr_window` vertices (a tuning value:
-// 16384 or 32768 contributions, 64 or 128 KB of LDS).  Every in-edge
-// sits in a 64-lane chunk of its source's window, kLaneEdges edges per
-// lane, and is summed from an LDS copy of that window; each piece (a row's
-// edges inside one chunk) leaves one (sum, error) partial, and the row
-// kernel merges a row's partials in window order.
+// 16384 or 32768 contributions, 64 or 128 KB of LDS).  Every in-edge sits
+// in a 64-lane chunk of its source's window, kLaneEdges edges per lane, and
+// is summed from an LDS copy of that window; each piece (a row's edges
+// inside one chunk) leaves one (sum, error) partial, and the row kernel
+// merges a row's partials in window order.
 constexpr int kLaneEdges = 16;
 constexpr int kChunkEdges = 64 * kLaneEdges;
-// Row blocks: at most kBlockRuns runs (a block's partials, runs plus chunk
-// cuts, are staged in LDS when they number at most kRowSlots); rows with
+constexpr int kChunkPieces = 256;   // pieces one chunk may hold
+// Row blocks: consecutive rows holding at most kBlockRuns runs; rows with
 // kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
+// A block's partials (its runs plus chunk cuts) must number at most
+// kRowSlots: the row kernel gathers them into LDS.
 constexpr int kRowSlots = 8192;
 constexpr int kRowThreads = 512;    // threads of the row kernel
 constexpr int kBlockRuns = 6144;
 constexpr int kSoloRuns = 256;
 constexpr int64_t kSoloEdges = 4096;
 
-// Per lane of a chunk: x = end mask (bit j: slot j ends a piece) | filled
-// slots << 16; y = output slot of the first piece ending in the lane (the
-// lane's other pieces follow it).
-struct TileLane {
-  uint32_t x, y;
-};
-
-// The tile layout as built on the host.
+// The tile layout as built on the host.  Partials are numbered window-major
+// in chunk order (chunk c writes [cbase[c], cbase[c + 1])); a row block's
+// partials in one window are contiguous there (a segment), and its region
+// is the concatenation of its segments in window order.
 struct TileHost {
   std::vector<int32_t> cwin;     // window of each chunk
   std::vector<int64_t> wstart;   // first chunk of each window (+ end)
-  std::vector<TileLane> meta;    // 64 per chunk
+  std::vector<int32_t> cbase;    // first partial of each chunk (+ end)
+  std::vector<uint32_t> meta;    // per lane: end mask (bit j: slot j ends a piece) | filled slots << 16
   std::vector<uint16_t> src;     // kChunkEdges per chunk
-  std::vector<int64_t> pbase;    // row blocks: first partial (+ end)
-  std::vector<int32_t> brow;     //             first row (+ end)
-  std::vector<int64_t> ibase;    //             first list entry (+ end)
-  std::vector<uint16_t> rcnt;    // per row below zb
-  std::vector<uint16_t> idx;
+  std::vector<int32_t> brow;     // row blocks: first row (+ end)
+  std::vector<int64_t> bseg;     // row blocks: first segment (+ end)
+  std::vector<int32_t> seg_start;  // segments: first partial
+  std::vector<int32_t> seg_len;    //           partials
+  std::vector<int64_t> ibase;    // row blocks: first list entry (+ end)
+  std::vector<uint16_t> rcnt;    // per row below zb: its partials (multi-row blocks)
+  std::vector<uint16_t> idx;     // per row, window order: partial offsets in its block's region
   int64_t n_pieces = 0, slots_filled = 0, window = 0;
   int32_t n_windows = 0;
 };

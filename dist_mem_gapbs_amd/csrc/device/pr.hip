@@ -390,25 +390,30 @@ __device__ __forceinline__ double seg_scan(double v, int lane, int seg) {
 struct TileArgs {
   const int32_t* cwin;     // window of each chunk
   const int64_t* wstart;   // first chunk of each window (+ end)
-  const TileLane* meta;
+  const int32_t* cbase;    // first partial of each chunk (+ end)
+  const uint32_t* meta;    // per lane: end mask | filled slots << 16
   const uint16_t* src;
   int64_t n_chunks;
   float2* part;            // (sum, error) per piece
-  const int64_t* pbase;    // row blocks: first partial (+ end)
-  const int32_t* brow;     //             first row (+ end)
-  const int64_t* ibase;    //             first list entry (+ end)
+  const int32_t* brow;     // row blocks: first row (+ end)
+  const int64_t* bseg;     //             first segment (+ end)
+  const int32_t* seg_start;  // segments: first partial
+  const int32_t* seg_len;    //           partials
+  const int64_t* ibase;    // row blocks: first list entry (+ end)
   const uint16_t* rcnt;    // per row: its partials (multi-row blocks)
-  const uint16_t* idx;     // per row: partial offsets in its block, window order
+  const uint16_t* idx;     // per row: partial offsets in its block's region, window order
   int64_t n_blocks;
 };
 
 constexpr int kTileThreads = 1024;
+constexpr int kBatch = 4;  // chunks a wavefront loads together
 
-// What one lane reads of a chunk from HBM: its TileLane and its
-// kLaneEdges 16-bit source offsets.
+// What one lane reads of a chunk from HBM: its end mask / slot count, its
+// kLaneEdges 16-bit source offsets and the chunk's partial range.
 struct ChunkLoad {
-  TileLane mt;
+  uint32_t mt;
   uint4 q0, q1;
+  int32_t c0, c1;
 };
 
 __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, int lane) {
@@ -417,12 +422,10 @@ __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, in
   const uint4* s4 = reinterpret_cast<const uint4*>(t.src + c * kChunkEdges + lane * kLaneEdges);
   l.q0 = s4[0];
   l.q1 = s4[1];
+  l.c0 = t.cbase[c];
+  l.c1 = t.cbase[c + 1];
   return l;
 }
-
-// Partials a wavefront stages in LDS before writing them out: (hi, lo,
-// slot) per piece, kStage per wavefront.
-constexpr int kStage = 120;
 
 __device__ __forceinline__ float2 split_pair(double s) {
   const float hi = static_cast<float>(s);
@@ -441,15 +444,17 @@ __device__ __forceinline__ float2 split_pair(double s) {
 // and a piece enters its row's compensated FP32 sum as the unevaluated pair
 // (hi, lo) = (float(s), float(s - hi)).
 //
-// The chunk's pieces, numbered in lane order, are staged kStage at a time
-// in the wavefront's LDS area with their output slots (the second pass is
-// repeated per round) and written back by consecutive lanes, so each store
-// instruction writes neighbouring slots.
+// The chunk's pieces (at most kChunkPieces), numbered in lane order, are
+// staged in the wavefront's LDS area and written to the chunk's contiguous
+// partial range by kChunkPieces / 64 store instructions of consecutive
+// lanes.  With that fixed count, a chunk whose loads were issued before
+// these stores is waited for without waiting for the stores (vector memory
+// operations complete in issue order).
 __device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane,
-                                           float4* stage) {
+                                           float2* stage) {
   const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
-  const uint32_t ends = l.mt.x & 0xFFFFu;
-  const int cnt = static_cast<int>(l.mt.x >> 16);
+  const uint32_t ends = l.mt & 0xFFFFu;
+  const int cnt = static_cast<int>(l.mt >> 16);
   float x[kLaneEdges];
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++) x[j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
@@ -467,43 +472,31 @@ __device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, 
   const unsigned long long h = has_end & upto;
   const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
   const double carry_out = seg_scan(acc, lane, seg);
-  double carry = __shfl_up(carry_out, 1, 64);
-  if (lane == 0) carry = 0.0;
-  // Piece numbers in lane order; the chunk's total.
+  double a = __shfl_up(carry_out, 1, 64);
+  if (lane == 0) a = 0.0;
   const uint32_t k_i = __builtin_popcount(ends);
-  const uint32_t incl = wave_incl_scan(k_i);
-  const uint32_t o_i = incl - k_i;
-  const uint32_t total = __shfl(incl, 63, 64);
-  for (uint32_t base = 0; base < total; base += kStage) {
-    // The lane's pieces again (its first one with the carry), each staged
-    // if its number falls in this round.
-    double a = carry;  // the lane's first piece starts from the carry
-    uint32_t q = o_i - base, slot = l.mt.y;
+  uint32_t q = wave_incl_scan(k_i) - k_i;  // the lane's first piece number
 #pragma unroll
-    for (int j = 0; j < kLaneEdges; j++) {
-      a += static_cast<double>(x[j]);
-      if ((ends >> j) & 1u) {
-        if (q < static_cast<uint32_t>(kStage)) {
-          const float2 v = split_pair(a);
-          stage[q] = make_float4(v.x, v.y, __uint_as_float(slot), 0.0f);
-        }
-        q++;
-        slot++;
-        a = 0.0;
-      }
+  for (int j = 0; j < kLaneEdges; j++) {
+    a += static_cast<double>(x[j]);
+    if ((ends >> j) & 1u) {
+      stage[q] = split_pair(a);
+      q++;
+      a = 0.0;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t n = min(static_cast<uint32_t>(kStage), total - base);
-    for (uint32_t e = lane; e < n; e += 64) {
-      const float4 v = stage[e];
-      t.part[__float_as_uint(v.z)] = make_float2(v.x, v.y);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  // The staged pieces are read back by other lanes of this wavefront (LDS
+  // operations of one wavefront complete in order).
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  // Always kChunkPieces / 64 stores, with no branch around them: lanes past
+  // the chunk's last piece write that piece again.
+  const int32_t c0 = l.c0, n = l.c1 - l.c0;
+#pragma unroll
+  for (int r = 0; r < kChunkPieces / 64; r++) {
+    const int e = min(lane + 64 * r, n - 1);
+    t.part[c0 + e] = stage[e];
+  }
+  __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
@@ -514,13 +507,13 @@ template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
-  __shared__ float4 stage_all[kTileThreads / 64][kStage];
+  __shared__ float2 stage_all[kTileThreads / 64][kChunkPieces];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  float4* stage = stage_all[wave];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar chunk indices
+  float2* stage = stage_all[wave];
   constexpr int kWaves = kTileThreads / 64;
   for (int64_t c = c0; c < c1;) {
     const int32_t w = t.cwin[c];
@@ -541,24 +534,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
       for (int j = threadIdx.x; j < cnt; j += kTileThreads) win[j] = cin[base + j];
     }
     __syncthreads();
-    int64_t cb = c + wave;
-    if (cb < wend) {
-      ChunkLoad cur = load_chunk(t, cb, lane);
-      for (; cb < wend; cb += kWaves) {
-        // Unconditional (the last one reloads a chunk of the range), so
-        // the wait for `cur` leaves this load in flight.
-        const ChunkLoad nxt = load_chunk(t, min(cb + kWaves, wend - 1), lane);
-        tile_chunk(t, win, cur, lane, stage);
-        cur = nxt;
-      }
+    // kBatch chunks per round: all their loads first, then each chunk in
+    // turn, so only the first waits for earlier stores.  Chunks past the
+    // range are the range's last chunk again (the same partials rewritten
+    // with the same values), keeping every round's memory operations fixed.
+    for (int64_t cb = c + wave; cb < wend; cb += kBatch * kWaves) {
+      ChunkLoad l[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) l[k] = load_chunk(t, min(cb + k * kWaves, wend - 1), lane);
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) tile_chunk(t, win, l[k], lane, stage);
     }
     c = wend;
   }
 }
 
-// Compensated merge of partials p[ix[0..cnt)] (LDS or global) in order.
-template <typename P>
-__device__ __forceinline__ Comp merge_list(const P* p, const uint16_t* __restrict__ ix, int cnt) {
+// Compensated merge of partials p[ix[0..cnt)] in order.
+__device__ __forceinline__ Comp merge_list(const float2* p, const uint16_t* __restrict__ ix, int cnt) {
   Comp acc{0.0f, 0.0f};
   int k = 0;
   for (; k + 4 <= cnt; k += 4) {
@@ -578,14 +570,23 @@ __device__ __forceinline__ Comp merge_list(const P* p, const uint16_t* __restric
   return acc;
 }
 
+constexpr int kSegBatch = kRowThreads;  // segments staged per round
+constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per thread
+
 // Row pass of the tiled mode: persistent workgroups take row blocks in
-// turn.  A one-row block merges its contiguous partials with its threads
-// striding them in order, then a fixed tree; a multi-row block is staged in
-// LDS and every thread merges one row's partials in window order.  Rows
-// without in-edges (the tail) only finish.
+// turn.  A block's region (its partials, window order) is gathered into LDS
+// from its segments: the segments' offsets in the region come from a block
+// scan of their lengths, and every thread then loads its region entries
+// (all loads in flight before the LDS stores), finding each entry's segment
+// by binary search.  A one-row block merges its region with its threads
+// striding it in order, then a fixed tree; in a multi-row block every
+// thread merges one row's partials in window order.  Rows without in-edges
+// (the tail) only finish.
 __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
                                                      float* __restrict__ score) {
   __shared__ float2 reg[kRowSlots];
+  __shared__ int32_t soff[kSegBatch + 1];
+  __shared__ int32_t sstart[kSegBatch];
   __shared__ float2 wred[kRowThreads / 64];
   __shared__ int scan[kRowThreads / 64 + 1];
   __shared__ double red[kRowThreads / 64];
@@ -595,10 +596,43 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, floa
   const int wave = threadIdx.x >> 6;
   for (int64_t b = blockIdx.x; b < t.n_blocks; b += gridDim.x) {
     const int32_t r0 = t.brow[b], r1 = t.brow[b + 1];
-    const int64_t p0 = t.pbase[b], p1 = t.pbase[b + 1];
+    const int64_t s0 = t.bseg[b], s1 = t.bseg[b + 1];
+    // Gather the region, kSegBatch segments per round.
+    int32_t done = 0;  // region entries gathered so far
+    for (int64_t sb = s0; sb < s1; sb += kSegBatch) {
+      const int ns = static_cast<int>(min<int64_t>(kSegBatch, s1 - sb));
+      const int k = static_cast<int>(threadIdx.x);
+      const int len = k < ns ? t.seg_len[sb + k] : 0;
+      if (k < ns) sstart[k] = t.seg_start[sb + k];
+      int total;
+      const int ex = block_excl_scan<kRowThreads>(len, scan, &total);
+      soff[k] = ex;
+      if (k == 0) soff[ns] = total;
+      __syncthreads();
+      float2 v[kRegionPer];
+#pragma unroll
+      for (int i = 0; i < kRegionPer; i++) {
+        const int e = k + i * kRowThreads;
+        if (e < total) {
+          int lo = 0, hi = ns;  // the last segment with soff <= e
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (soff[mid] <= e) lo = mid; else hi = mid;
+          }
+          v[i] = t.part[sstart[lo] + (e - soff[lo])];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kRegionPer; i++) {
+        const int e = k + i * kRowThreads;
+        if (e < total) reg[done + e] = v[i];
+      }
+      done += total;
+      __syncthreads();
+    }
     if (r1 - r0 == 1) {
       FusedAcc<float> acc;
-      for (int64_t p = p0 + threadIdx.x; p < p1; p += kRowThreads) acc.merge(t.part[p]);
+      for (int p = threadIdx.x; p < done; p += kRowThreads) acc.merge(reg[p]);
       acc.reduce<64>(lane);
       if (lane == 0) wred[wave] = make_float2(acc.c.s, acc.c.c);
       __syncthreads();
@@ -610,12 +644,6 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, floa
       __syncthreads();
       continue;
     }
-    const int64_t np = p1 - p0;
-    const bool fits = np <= kRowSlots;
-    if (fits) {
-      for (int64_t i = threadIdx.x; i < np; i += kRowThreads) reg[i] = t.part[p0 + i];
-      __syncthreads();
-    }
     int64_t ib = t.ibase[b];
     for (int32_t rr = r0; rr < r1; rr += kRowThreads) {
       const int32_t r = rr + static_cast<int32_t>(threadIdx.x);
@@ -623,13 +651,12 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, floa
       int total;
       const int ex = block_excl_scan<kRowThreads>(c, scan, &total);
       if (r < r1) {
-        const uint16_t* ix = t.idx + ib + ex;
-        const Comp acc = fits ? merge_list(reg, ix, c) : merge_list(t.part + p0, ix, c);
+        const Comp acc = merge_list(reg, t.idx + ib + ex, c);
         err += finish_f32(a, r, acc.value(), score, cout);
       }
       ib += total;
     }
-    __syncthreads();  // LDS readers done before the next block's copy
+    __syncthreads();  // LDS readers done before the next block's gather
   }
   for (int64_t u = a.zero_begin + int64_t(blockIdx.x) * kRowThreads + threadIdx.x; u < a.nloc;
        u += int64_t(gridDim.x) * kRowThreads)
@@ -793,12 +820,15 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       TileArgs t;
       t.cwin = ctx->d_wt_cwin;
       t.wstart = ctx->d_wt_wstart;
-      t.meta = static_cast<const TileLane*>(ctx->d_wt_meta);
+      t.cbase = ctx->d_wt_cbase;
+      t.meta = ctx->d_wt_meta;
       t.src = ctx->d_wt_src;
       t.n_chunks = ctx->n_chunks;
       t.part = static_cast<float2*>(ctx->d_wt_part);
-      t.pbase = ctx->d_wt_pbase;
       t.brow = ctx->d_wt_brow;
+      t.bseg = ctx->d_wt_bseg;
+      t.seg_start = ctx->d_wt_seg_start;
+      t.seg_len = ctx->d_wt_seg_len;
       t.ibase = ctx->d_wt_ibase;
       t.rcnt = ctx->d_wt_rcnt;
       t.idx = ctx->d_wt_idx;

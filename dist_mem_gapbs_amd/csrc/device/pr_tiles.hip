@@ -8,23 +8,24 @@
 // contribution is gathered from HBM.
 //
 // Tile layout (window-major, rows ascending inside a window):
-//   chunk    = 64 lanes x kLaneEdges edge slots, all of one window;
+//   chunk    = 64 lanes x kLaneEdges edge slots, all of one window, holding
+//              at most kChunkPieces pieces;
 //   piece    = the part of a run inside one chunk (a run that crosses a
-//              chunk boundary is cut there), its sum is one partial;
-//   per lane = kLaneEdges 16-bit source offsets, an end mask (bit j: slot j
-//              is the last edge of a piece), the count of filled slots and
-//              the output slot of the first piece ending in the lane.
-// A lane never holds edges of two row blocks (it is padded at a block
-// boundary), so the pieces ending in one lane have consecutive slots.
+//              chunk boundary is cut there); its sum is one partial;
+//   per lane = kLaneEdges 16-bit source offsets and an end mask (bit j:
+//              slot j is the last edge of a piece) with the count of
+//              filled slots.
+// Partials are numbered in chunk order, so a chunk writes one contiguous
+// range of them and every partial's slot follows from the chunk's first
+// slot and the piece's rank in lane order.
 //
-// Partial layout (block-major): rows are grouped into row blocks whose
-// partials fit one workgroup's LDS; a block's partials are contiguous,
-// window-major inside the block.  A row's partials within its block are
-// listed in window order (16-bit offsets), so the row kernel stages the
-// block's partials in LDS with coalesced loads and merges every row's
-// partials in a fixed order: iterations are bit-deterministic.  Rows with
-// many runs or many in-edges form blocks of their own; their partials are
-// contiguous and need no list.
+// Row blocks: rows are grouped into blocks whose partials fit one
+// workgroup's LDS.  A block's partials in one window are contiguous (a
+// segment); the row kernel gathers the block's segments, in window order,
+// into LDS (the block's region) and merges every row's partials through a
+// list of region offsets kept in window order, so iterations are
+// bit-deterministic.  Rows with many runs or many in-edges form blocks of
+// their own.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -51,8 +52,8 @@ struct PieceRec {
 struct WinPack {
   std::vector<uint16_t> src;     // kChunkEdges per chunk
   std::vector<uint32_t> mask;    // per lane: end mask | filled slots << 16
-  std::vector<int64_t> first;    // per lane: first piece ending there (-1: none)
-  std::vector<PieceRec> pieces;  // in window order
+  std::vector<int32_t> cpieces;  // pieces per chunk
+  std::vector<PieceRec> pieces;  // in chunk order
 };
 
 template <typename T>
@@ -81,6 +82,7 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
     return DMM_ERR_INVALID;
   }
   const int32_t nwin = static_cast<int32_t>(std::max<int64_t>(1, (n_gather + W - 1) / W));
+  // 1. Rows sorted; runs per row.
   std::vector<int32_t> runs(std::max<int64_t>(1, zb), 0);
   parallel_dynamic(zb, nt, 4096, [&](int64_t r) {
     int32_t* b = nbr + off[r];
@@ -94,9 +96,7 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
     }
     runs[r] = c;
   });
-  // 2. Row blocks over [0, zb): consecutive rows holding at most
-  //    kBlockRuns runs; a row with kSoloRuns runs or kSoloEdges in-edges
-  //    (or more) is a block of its own.
+  // 2. Row blocks over [0, zb).
   std::vector<int32_t> brow{0};
   {
     int64_t cur = 0;
@@ -141,8 +141,8 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
     WinPack& o = packs[w];
     const int64_t base = w * W;
     int64_t ch = -1;
-    int lane = 64, slot = 0, prev_block = -1;
-    int64_t cur_piece = -1, last_lane = -1;
+    int lane = 64, slot = 0;
+    int64_t last_lane = -1;
     int last_slot = -1;
     auto new_chunk = [&]() {
       ch++;
@@ -150,11 +150,11 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
       slot = 0;
       o.src.resize((ch + 1) * kChunkEdges, 0);
       o.mask.resize((ch + 1) * 64, 0u);
-      o.first.resize((ch + 1) * 64, -1);
+      o.cpieces.push_back(0);
     };
-    auto end_piece = [&]() {
-      o.mask[last_lane] |= 1u << last_slot;
-      if (o.first[last_lane] < 0) o.first[last_lane] = cur_piece;
+    auto start_piece = [&](int32_t row, int32_t blk) {
+      o.pieces.push_back({row, blk});
+      o.cpieces[ch]++;
     };
     for (int t = 0; t < nt; t++)
       for (const RunRec& rr : recs[t][w]) {
@@ -163,23 +163,16 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
           lane++;
           slot = 0;
         }
-        if (slot > 0 && blk != prev_block) {  // a lane holds one block's edges
-          lane++;
-          slot = 0;
-        }
-        if (lane >= 64) new_chunk();
-        prev_block = blk;
-        o.pieces.push_back({rr.row, blk});
-        cur_piece = static_cast<int64_t>(o.pieces.size()) - 1;
+        if (lane >= 64 || o.cpieces[ch] == kChunkPieces) new_chunk();
+        start_piece(rr.row, blk);
         for (int32_t k = 0; k < rr.len; k++) {
           if (slot == kLaneEdges) {
             lane++;
             slot = 0;
             if (lane == 64) {  // the run crosses a chunk boundary: cut it
-              end_piece();
+              o.mask[last_lane] |= 1u << last_slot;
               new_chunk();
-              o.pieces.push_back({rr.row, blk});
-              cur_piece++;
+              start_piece(rr.row, blk);
             }
           }
           const int64_t li = ch * 64 + lane;
@@ -189,91 +182,114 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
           last_slot = slot;
           slot++;
         }
-        end_piece();
+        o.mask[last_lane] |= 1u << last_slot;  // the piece ends here
       }
   });
   recs.clear();
   recs.shrink_to_fit();
-  // 5. Partial slots: block-major, window-major inside a block.
-  std::vector<int64_t> btot(nb + 1, 0);
-  for (int32_t w = 0; w < nwin; w++)
-    for (const PieceRec& p : packs[w].pieces) btot[p.block]++;
-  std::vector<int64_t> pbase(nb + 1, 0);
-  for (int64_t b = 0; b < nb; b++) pbase[b + 1] = pbase[b] + btot[b];
-  const int64_t npieces = pbase[nb];
-  if (npieces >= (int64_t(1) << 31)) {
-    *err = "PageRank tile partials exceed 2^31";
-    return DMM_ERR_CAPACITY;
+  // 5. Chunks in window order and their first partials.
+  std::vector<int64_t> wstart(nwin + 1, 0);
+  for (int32_t w = 0; w < nwin; w++) wstart[w + 1] = wstart[w] + static_cast<int64_t>(packs[w].cpieces.size());
+  const int64_t nchunks = wstart[nwin];
+  std::vector<int32_t> cbase(nchunks + 1, 0);
+  {
+    int64_t g = 0;
+    for (int32_t w = 0; w < nwin; w++)
+      for (size_t k = 0; k < packs[w].cpieces.size(); k++) {
+        cbase[wstart[w] + k] = static_cast<int32_t>(g);
+        g += packs[w].cpieces[k];
+        if (g >= (int64_t(1) << 31)) {
+          *err = "PageRank tile partials exceed 2^31";
+          return DMM_ERR_CAPACITY;
+        }
+      }
+    cbase[nchunks] = static_cast<int32_t>(g);
   }
-  std::vector<int64_t> cursor(pbase.begin(), pbase.end() - 1);
-  std::vector<std::vector<int32_t>> ppos(nwin);  // slot of every piece
-  for (int32_t w = 0; w < nwin; w++) {
-    const std::vector<PieceRec>& pc = packs[w].pieces;
-    std::vector<int32_t>& pos = ppos[w];
-    pos.resize(pc.size());
-    for (size_t i = 0; i < pc.size(); i++) pos[i] = static_cast<int32_t>(cursor[pc[i].block]++);
-  }
-  // 6. Row lists of multi-row blocks: each row's partial offsets inside its
-  //    block, in window order.
+  const int64_t npieces = cbase[nchunks];
+  // 6. Segments per block, in window order; region offsets; row lists.
+  std::vector<std::vector<int32_t>> bs_start(nb), bs_len(nb);
+  std::vector<int64_t> btot(nb, 0);
   std::vector<uint16_t> rcnt(std::max<int64_t>(1, zb), 0);
   std::vector<char> multi(nb, 0);
   for (int64_t b = 0; b < nb; b++) multi[b] = brow[b + 1] - brow[b] > 1;
-  for (int32_t w = 0; w < nwin; w++)
-    for (const PieceRec& p : packs[w].pieces)
-      if (multi[p.block]) {
-        if (rcnt[p.row] == 0xFFFF) {
-          *err = "a row has too many PageRank tile pieces";
-          return DMM_ERR_CAPACITY;
+  std::vector<int32_t> rpos(npieces > 0 ? npieces : 1);  // region offset of every partial
+  for (int32_t w = 0; w < nwin; w++) {
+    const std::vector<PieceRec>& pc = packs[w].pieces;
+    const int64_t g0 = cbase[wstart[w]];
+    for (size_t i = 0; i < pc.size();) {
+      const int32_t b = pc[i].block;
+      size_t j = i;
+      while (j < pc.size() && pc[j].block == b) j++;
+      bs_start[b].push_back(static_cast<int32_t>(g0 + i));
+      bs_len[b].push_back(static_cast<int32_t>(j - i));
+      for (size_t k = i; k < j; k++) {
+        rpos[g0 + k] = static_cast<int32_t>(btot[b]++);
+        if (multi[b]) {
+          if (rcnt[pc[k].row] == 0xFFFF) {
+            *err = "a row has too many PageRank tile pieces";
+            return DMM_ERR_CAPACITY;
+          }
+          rcnt[pc[k].row]++;
         }
-        rcnt[p.row]++;
       }
-  std::vector<int64_t> ibase(nb + 1, 0), rstart(std::max<int64_t>(1, zb), 0);
+      i = j;
+    }
+  }
+  for (int64_t b = 0; b < nb; b++)
+    if (btot[b] > kRowSlots) {
+      *err = "a row block has more PageRank tile partials than the row kernel stages";
+      return DMM_ERR_CAPACITY;
+    }
+  std::vector<int64_t> bseg(nb + 1, 0), ibase(nb + 1, 0), rstart(std::max<int64_t>(1, zb), 0);
   for (int64_t b = 0; b < nb; b++) {
+    bseg[b + 1] = bseg[b] + static_cast<int64_t>(bs_start[b].size());
     int64_t c = 0;
-    if (multi[b]) {
-      if (btot[b] > 0xFFFF) {
-        *err = "a row block has too many PageRank tile pieces";
-        return DMM_ERR_CAPACITY;
-      }
+    if (multi[b])
       for (int64_t r = brow[b]; r < brow[b + 1]; r++) {
         rstart[r] = ibase[b] + c;
         c += rcnt[r];
       }
-    }
     ibase[b + 1] = ibase[b] + c;
   }
+  std::vector<int32_t> seg_start(std::max<int64_t>(1, bseg[nb])), seg_len(std::max<int64_t>(1, bseg[nb]));
+  for (int64_t b = 0; b < nb; b++) {
+    std::copy(bs_start[b].begin(), bs_start[b].end(), seg_start.begin() + bseg[b]);
+    std::copy(bs_len[b].begin(), bs_len[b].end(), seg_len.begin() + bseg[b]);
+  }
+  bs_start.clear();
+  bs_len.clear();
   std::vector<uint16_t> idx(std::max<int64_t>(1, ibase[nb]));
   for (int32_t w = 0; w < nwin; w++) {
     const std::vector<PieceRec>& pc = packs[w].pieces;
+    const int64_t g0 = cbase[wstart[w]];
     for (size_t i = 0; i < pc.size(); i++)
-      if (multi[pc[i].block]) idx[rstart[pc[i].row]++] = static_cast<uint16_t>(ppos[w][i] - pbase[pc[i].block]);
+      if (multi[pc[i].block]) idx[rstart[pc[i].row]++] = static_cast<uint16_t>(rpos[g0 + i]);
   }
-  // 7. Chunks in window order, with each lane's first output slot.
-  std::vector<int64_t> wstart(nwin + 1, 0);
-  for (int32_t w = 0; w < nwin; w++) wstart[w + 1] = wstart[w] + static_cast<int64_t>(packs[w].mask.size() / 64);
-  const int64_t nchunks = wstart[nwin];
+  // 7. Chunk arrays.
   std::vector<int32_t> cwin(std::max<int64_t>(1, nchunks));
   std::vector<uint16_t> src(std::max<int64_t>(1, nchunks * kChunkEdges));
-  std::vector<TileLane> meta(std::max<int64_t>(1, nchunks * 64));
-  int64_t filled = 0;
+  std::vector<uint32_t> meta(std::max<int64_t>(1, nchunks * 64));
   parallel_dynamic(nwin, nt, 1, [&](int64_t w) {
     const WinPack& o = packs[w];
     const int64_t c0 = wstart[w], nc = wstart[w + 1] - c0;
     for (int64_t c = 0; c < nc; c++) cwin[c0 + c] = static_cast<int32_t>(w);
     std::copy(o.src.begin(), o.src.end(), src.begin() + c0 * kChunkEdges);
-    for (int64_t l = 0; l < nc * 64; l++)
-      meta[c0 * 64 + l] = TileLane{o.mask[l], o.first[l] < 0 ? 0u : static_cast<uint32_t>(ppos[w][o.first[l]])};
+    std::copy(o.mask.begin(), o.mask.end(), meta.begin() + c0 * 64);
   });
+  int64_t filled = 0;
   for (int32_t w = 0; w < nwin; w++)
     for (uint32_t mk : packs[w].mask) filled += mk >> 16;
   packs.clear();
   packs.shrink_to_fit();
   out->cwin.swap(cwin);
   out->wstart.swap(wstart);
+  out->cbase.swap(cbase);
   out->meta.swap(meta);
   out->src.swap(src);
-  out->pbase.swap(pbase);
   out->brow.swap(brow);
+  out->bseg.swap(bseg);
+  out->seg_start.swap(seg_start);
+  out->seg_len.swap(seg_len);
   out->ibase.swap(ibase);
   out->rcnt.swap(rcnt);
   out->idx.swap(idx);
@@ -300,10 +316,13 @@ int pr_tiles_build(dmm_ctx* ctx) {
   std::vector<int32_t>().swap(nbr);
   if ((rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_cwin), th.cwin, "PR tile chunk windows")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_wstart), th.wstart, "PR tile window starts")) ||
-      (rc = upload_vec(ctx, &ctx->d_wt_meta, th.meta, "PR tile lanes")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_cbase), th.cbase, "PR tile chunk partials")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_meta), th.meta, "PR tile lanes")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_src), th.src, "PR tile sources")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_pbase), th.pbase, "PR row block partials")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_brow), th.brow, "PR row blocks")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_bseg), th.bseg, "PR row block segments")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_start), th.seg_start, "PR segments")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_len), th.seg_len, "PR segments")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_ibase), th.ibase, "PR row block lists")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_rcnt), th.rcnt, "PR row piece counts")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")) ||
@@ -358,6 +377,28 @@ HostComp butterfly64(HostComp* v) {
   return v[0];
 }
 
+// The device's segmented inclusive scan (seg_scan in pr.hip), step for
+// step, so the FP64 sums round identically.
+void host_seg_scan(const double* tail, const bool* has_end, double* sc) {
+  int seg[64];
+  int h = -1;
+  for (int l = 0; l < 64; l++) {
+    if (has_end[l]) h = l;
+    seg[l] = h < 0 ? 0 : h;
+    sc[l] = tail[l];
+  }
+  double nv[64];
+  for (int d = 1; d <= 8; d <<= 1) {
+    for (int l = 0; l < 64; l++) nv[l] = ((l & 15) >= d && l - d >= seg[l]) ? sc[l] + sc[l - d] : sc[l];
+    std::copy(nv, nv + 64, sc);
+  }
+  for (int l = 0; l < 64; l++)
+    nv[l] = (((l >> 4) & 1) && (l & ~15) - 1 >= seg[l]) ? sc[l] + sc[(l & ~15) - 1] : sc[l];
+  std::copy(nv, nv + 64, sc);
+  for (int l = 0; l < 64; l++) nv[l] = (l >= 32 && 31 >= seg[l]) ? sc[l] + sc[31] : sc[l];
+  std::copy(nv, nv + 64, sc);
+}
+
 }  // namespace
 
 }  // namespace dmm
@@ -381,7 +422,8 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
   const int64_t nchunks = th.wstart.back(), np = th.n_pieces, nb = static_cast<int64_t>(th.brow.size()) - 1;
   auto bad = [](const char* what) { return set_err(nullptr, DMM_ERR_GRAPH, "tile layout: %s", what); };
   if (static_cast<int64_t>(th.cwin.size()) < nchunks || static_cast<int64_t>(th.meta.size()) < nchunks * 64 ||
-      static_cast<int64_t>(th.src.size()) < nchunks * kChunkEdges || th.pbase.back() != np || th.brow[0] != 0 ||
+      static_cast<int64_t>(th.src.size()) < nchunks * kChunkEdges ||
+      static_cast<int64_t>(th.cbase.size()) != nchunks + 1 || th.cbase[nchunks] != np || th.brow[0] != 0 ||
       th.brow.back() != zb)
     return bad("sizes");
   std::vector<char> written(std::max<int64_t>(1, np), 0);
@@ -396,13 +438,14 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       double tail[64];
       bool has_end[64];
       float x[64][kLaneEdges];
+      int total = 0;
       for (int l = 0; l < 64; l++) {
-        const TileLane mt = th.meta[c * 64 + l];
-        const uint32_t ends = mt.x & 0xFFFFu;
-        const int cnt = static_cast<int>(mt.x >> 16);
+        const uint32_t mt = th.meta[c * 64 + l];
+        const uint32_t ends = mt & 0xFFFFu;
+        const int cnt = static_cast<int>(mt >> 16);
         if (cnt > kLaneEdges || (cnt < kLaneEdges && (ends >> cnt) != 0)) return bad("lane count / end mask");
-        if (ends && (int64_t(mt.y) + __builtin_popcount(ends) > np)) return bad("output slot");
         edges += cnt;
+        total += __builtin_popcount(ends);
         // The lane's first pass (tile_chunk in pr.hip): its sum after its
         // last piece end.
         double acc = 0.0;
@@ -416,42 +459,19 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         tail[l] = acc;
         has_end[l] = ends != 0;
       }
-      if ((th.meta[c * 64].x & 0xFFFFu) == 0 && (th.meta[c * 64].x >> 16) == 0 && c == th.wstart[w])
-        return bad("empty first lane");
-      // Carries: the device's segmented scan (seg_scan in pr.hip), step for
-      // step, so the FP64 sums round identically.
+      if (total > kChunkPieces || total != th.cbase[c + 1] - th.cbase[c]) return bad("chunk piece count");
       double sc[64];
-      int seg[64];
-      {
-        int h = -1;
-        for (int l = 0; l < 64; l++) {
-          if (has_end[l]) h = l;
-          seg[l] = h < 0 ? 0 : h;
-          sc[l] = tail[l];
-        }
-        for (int d = 1; d <= 8; d <<= 1) {
-          double nv[64];
-          for (int l = 0; l < 64; l++)
-            nv[l] = ((l & 15) >= d && l - d >= seg[l]) ? sc[l] + sc[l - d] : sc[l];
-          std::copy(nv, nv + 64, sc);
-        }
-        double nv[64];
-        for (int l = 0; l < 64; l++)
-          nv[l] = (((l >> 4) & 1) && (l & ~15) - 1 >= seg[l]) ? sc[l] + sc[(l & ~15) - 1] : sc[l];
-        std::copy(nv, nv + 64, sc);
-        for (int l = 0; l < 64; l++) nv[l] = (l >= 32 && 31 >= seg[l]) ? sc[l] + sc[31] : sc[l];
-        std::copy(nv, nv + 64, sc);
-      }
-      // The lane's second pass: its pieces, the first one from the carry.
+      host_seg_scan(tail, has_end, sc);
+      // The lane's second pass: its pieces, the first one from the carry,
+      // numbered in lane order from the chunk's first partial.
+      int64_t slot = th.cbase[c];
       for (int l = 0; l < 64; l++) {
-        const TileLane mt = th.meta[c * 64 + l];
-        const uint32_t ends = mt.x & 0xFFFFu;
+        const uint32_t ends = th.meta[c * 64 + l] & 0xFFFFu;
         double a = l == 0 ? 0.0 : sc[l - 1];
-        uint32_t slot = mt.y;
         for (int j = 0; j < kLaneEdges; j++) {
           a += static_cast<double>(x[l][j]);
           if ((ends >> j) & 1u) {
-            if (written[slot]) return bad("slot written twice");
+            if (slot >= th.cbase[c + 1] || written[slot]) return bad("output slot");
             written[slot] = 1;
             px[slot] = static_cast<float>(a);
             py[slot] = static_cast<float>(a - static_cast<double>(px[slot]));
@@ -461,34 +481,46 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         }
       }
       // Every piece ends inside its chunk: the last filled slot ends one,
-      // and filled slots are a prefix of every lane up to it.
+      // and every lane before it is full or ends a piece at its last slot.
       int last = -1;
       for (int l = 0; l < 64; l++)
-        if ((th.meta[c * 64 + l].x >> 16) != 0) last = l;
+        if ((th.meta[c * 64 + l] >> 16) != 0) last = l;
       if (last < 0) return bad("empty chunk");
-      for (int l = 0; l < last; l++) {
-        const uint32_t cl = th.meta[c * 64 + l].x >> 16;
-        // Only a block boundary may leave a lane short, right after a piece end.
-        if (cl < kLaneEdges && cl > 0 && !((th.meta[c * 64 + l].x >> (cl - 1)) & 1u)) return bad("short lane");
+      for (int l = 0; l <= last; l++) {
+        const uint32_t mt = th.meta[c * 64 + l];
+        const uint32_t cl = mt >> 16;
+        if (cl == 0 || (cl < kLaneEdges && !((mt >> (cl - 1)) & 1u))) return bad("short lane");
       }
-      const uint32_t cl = th.meta[c * 64 + last].x >> 16;
-      if (!((th.meta[c * 64 + last].x >> (cl - 1)) & 1u)) return bad("piece open at chunk end");
+      const uint32_t cl = th.meta[c * 64 + last] >> 16;
+      if (!((th.meta[c * 64 + last] >> (cl - 1)) & 1u)) return bad("piece open at chunk end");
     }
   }
   if (edges != m) return bad("edge count");
   for (int64_t p = 0; p < np; p++)
     if (!written[p]) return bad("slot never written");
-  // k_rows.
+  // k_rows: each block's region gathered from its segments, window order.
   int64_t max_block = 0;
+  std::vector<float> rx, ry;
   for (int64_t b = 0; b < nb; b++) {
     const int32_t r0 = th.brow[b], r1 = th.brow[b + 1];
-    const int64_t p0 = th.pbase[b], p1 = th.pbase[b + 1];
-    if (r1 <= r0 || p1 < p0) return bad("row block bounds");
-    max_block = std::max(max_block, p1 - p0);
+    if (r1 <= r0) return bad("row block bounds");
+    rx.clear();
+    ry.clear();
+    for (int64_t s = th.bseg[b]; s < th.bseg[b + 1]; s++) {
+      if (th.seg_start[s] < 0 || th.seg_len[s] <= 0 || th.seg_start[s] + int64_t(th.seg_len[s]) > np)
+        return bad("segment bounds");
+      for (int32_t k = 0; k < th.seg_len[s]; k++) {
+        rx.push_back(px[th.seg_start[s] + k]);
+        ry.push_back(py[th.seg_start[s] + k]);
+      }
+    }
+    const int64_t rn = static_cast<int64_t>(rx.size());
+    if (rn > kRowSlots) return bad("block region beyond the row kernel's LDS");
+    max_block = std::max(max_block, rn);
     if (r1 - r0 == 1) {
       HostComp lanes[kRowThreads];
       for (int t = 0; t < kRowThreads; t++)
-        for (int64_t p = p0 + t; p < p1; p += kRowThreads) lanes[t].merge(px[p], py[p]);
+        for (int64_t p = t; p < rn; p += kRowThreads) lanes[t].merge(rx[p], ry[p]);
       HostComp wv[kRowThreads / 64];
       for (int w = 0; w < kRowThreads / 64; w++) wv[w] = butterfly64(lanes + 64 * w);
       HostComp tot = wv[0];
@@ -502,8 +534,8 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       for (int k = 0; k < th.rcnt[r]; k++) {
         if (ib + k >= th.ibase[b + 1]) return bad("row list beyond its block");
         const int64_t q = th.idx[ib + k];
-        if (q >= p1 - p0) return bad("partial offset beyond its block");
-        acc.merge(px[p0 + q], py[p0 + q]);
+        if (q >= rn) return bad("partial offset beyond its block");
+        acc.merge(rx[q], ry[q]);
       }
       ib += th.rcnt[r];
       incoming[r] = acc.value();
