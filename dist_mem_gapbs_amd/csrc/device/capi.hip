@@ -84,17 +84,18 @@ void drop_graphs(dmm_ctx* ctx) {
 void drop_tiles(dmm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* bufs[] = {ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_cbase, ctx->d_wt_meta, ctx->d_wt_src,
-                  ctx->d_wt_brow, ctx->d_wt_bseg, ctx->d_wt_seg_start, ctx->d_wt_seg_len, ctx->d_wt_ibase,
-                  ctx->d_wt_rcnt, ctx->d_wt_idx, ctx->d_wt_part};
+                  ctx->d_wt_brow, ctx->d_wt_bseg, ctx->d_wt_bsize, ctx->d_wt_seg_start, ctx->d_wt_seg_off,
+                  ctx->d_wt_istart, ctx->d_wt_idx, ctx->d_wt_part};
   const int64_t held = ctx->tiles_bytes;
   for (void* p : bufs) dmm::dev_free(p);
   ctx->dev_bytes -= held;
   ctx->tiles_bytes = 0;
-  ctx->d_wt_cwin = ctx->d_wt_cbase = ctx->d_wt_brow = ctx->d_wt_seg_start = ctx->d_wt_seg_len = nullptr;
-  ctx->d_wt_wstart = ctx->d_wt_bseg = ctx->d_wt_ibase = nullptr;
+  ctx->d_wt_cwin = ctx->d_wt_cbase = ctx->d_wt_brow = ctx->d_wt_seg_start = ctx->d_wt_seg_off = nullptr;
+  ctx->d_wt_bsize = ctx->d_wt_istart = nullptr;
+  ctx->d_wt_wstart = ctx->d_wt_bseg = nullptr;
   ctx->d_wt_meta = nullptr;
   ctx->d_wt_part = nullptr;
-  ctx->d_wt_src = ctx->d_wt_rcnt = ctx->d_wt_idx = nullptr;
+  ctx->d_wt_src = ctx->d_wt_idx = nullptr;
   ctx->n_chunks = ctx->n_pieces = ctx->n_row_blocks = ctx->tile_slots_filled = ctx->tiles_window = 0;
   ctx->n_windows = 0;
   if (ctx->tiles_ready) drop_graphs(ctx);

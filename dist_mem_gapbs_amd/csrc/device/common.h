@@ -207,10 +207,10 @@ struct dmm_ctx {
   uint16_t* d_wt_src = nullptr;     // kChunkEdges source offsets per chunk
   int32_t* d_wt_brow = nullptr;     // first row of each row block (+ end)
   int64_t* d_wt_bseg = nullptr;     // first segment of each row block (+ end)
+  int32_t* d_wt_bsize = nullptr;    // partials of each row block
   int32_t* d_wt_seg_start = nullptr;  // segments: first partial
-  int32_t* d_wt_seg_len = nullptr;    //           partials
-  int64_t* d_wt_ibase = nullptr;    // first list entry of each row block (+ end)
-  uint16_t* d_wt_rcnt = nullptr;    // partials per row (multi-row blocks)
+  int32_t* d_wt_seg_off = nullptr;    //           offset in the block's region
+  int32_t* d_wt_istart = nullptr;   // per row: first list entry (+ end)
   uint16_t* d_wt_idx = nullptr;     // per row, window order: partial offset in its block's region
   void* d_wt_part = nullptr;        // float2 (sum, error) per piece
   int64_t n_chunks = 0, n_pieces = 0, n_row_blocks = 0, tile_slots_filled = 0, tiles_window = 0;

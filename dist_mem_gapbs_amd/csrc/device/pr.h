@@ -58,10 +58,10 @@ struct TileHost {
   std::vector<uint16_t> src;     // kChunkEdges per chunk
   std::vector<int32_t> brow;     // row blocks: first row (+ end)
   std::vector<int64_t> bseg;     // row blocks: first segment (+ end)
+  std::vector<int32_t> bsize;    // row blocks: partials (the region's size)
   std::vector<int32_t> seg_start;  // segments: first partial
-  std::vector<int32_t> seg_len;    //           partials
-  std::vector<int64_t> ibase;    // row blocks: first list entry (+ end)
-  std::vector<uint16_t> rcnt;    // per row below zb: its partials (multi-row blocks)
+  std::vector<int32_t> seg_off;    //           offset in the block's region
+  std::vector<int32_t> istart;   // rows [0, zb]: first list entry (rows of one-row blocks have none)
   std::vector<uint16_t> idx;     // per row, window order: partial offsets in its block's region
   int64_t n_pieces = 0, slots_filled = 0, window = 0;
   int32_t n_windows = 0;

@@ -387,6 +387,18 @@ __device__ __forceinline__ double seg_scan(double v, int lane, int seg) {
   return v;
 }
 
+// Inclusive prefix sum over the 64 lanes from DPP lane moves (the
+// unsegmented form of seg_scan): no LDS traffic.
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1 and 3
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 struct TileArgs {
   const int32_t* cwin;     // window of each chunk
   const int64_t* wstart;   // first chunk of each window (+ end)
@@ -397,10 +409,10 @@ struct TileArgs {
   float2* part;            // (sum, error) per piece
   const int32_t* brow;     // row blocks: first row (+ end)
   const int64_t* bseg;     //             first segment (+ end)
+  const int32_t* bsize;    //             partials (the region's size)
   const int32_t* seg_start;  // segments: first partial
-  const int32_t* seg_len;    //           partials
-  const int64_t* ibase;    // row blocks: first list entry (+ end)
-  const uint16_t* rcnt;    // per row: its partials (multi-row blocks)
+  const int32_t* seg_off;    //           offset in the block's region
+  const int32_t* istart;   // per row: first list entry (+ end)
   const uint16_t* idx;     // per row: partial offsets in its block's region, window order
   int64_t n_blocks;
 };
@@ -475,7 +487,7 @@ __device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, 
   double a = __shfl_up(carry_out, 1, 64);
   if (lane == 0) a = 0.0;
   const uint32_t k_i = __builtin_popcount(ends);
-  uint32_t q = wave_incl_scan(k_i) - k_i;  // the lane's first piece number
+  uint32_t q = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
 #pragma unroll
   for (int j = 0; j < kLaneEdges; j++) {
     a += static_cast<double>(x[j]);
@@ -549,23 +561,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
   }
 }
 
-// Compensated merge of partials p[ix[0..cnt)] in order.
-__device__ __forceinline__ Comp merge_list(const float2* p, const uint16_t* __restrict__ ix, int cnt) {
+// Compensated merge of the region's partials reg[ix[0..cnt)] in order; the
+// offsets are loaded kMergeAhead at a time, all before their LDS reads.
+constexpr int kMergeAhead = 8;
+__device__ __forceinline__ Comp merge_list(const float2* reg, const uint16_t* __restrict__ ix, int cnt) {
   Comp acc{0.0f, 0.0f};
-  int k = 0;
-  for (; k + 4 <= cnt; k += 4) {
-    uint16_t q[4];
-    float2 v[4];
+  for (int k = 0; k < cnt; k += kMergeAhead) {
+    uint16_t q[kMergeAhead];
 #pragma unroll
-    for (int j = 0; j < 4; j++) q[j] = ix[k + j];
+    for (int j = 0; j < kMergeAhead; j++) q[j] = k + j < cnt ? ix[k + j] : 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = p[q[j]];
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc.merge(v[j].x, v[j].y);
-  }
-  for (; k < cnt; k++) {
-    const float2 v = p[ix[k]];
-    acc.merge(v.x, v.y);
+    for (int j = 0; j < kMergeAhead; j++) {
+      const float2 v = reg[q[j]];
+      if (k + j < cnt) acc.merge(v.x, v.y);
+    }
   }
   return acc;
 }
@@ -575,45 +584,43 @@ constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per threa
 
 // Row pass of the tiled mode: persistent workgroups take row blocks in
 // turn.  A block's region (its partials, window order) is gathered into LDS
-// from its segments: the segments' offsets in the region come from a block
-// scan of their lengths, and every thread then loads its region entries
-// (all loads in flight before the LDS stores), finding each entry's segment
-// by binary search.  A one-row block merges its region with its threads
-// striding it in order, then a fixed tree; in a multi-row block every
-// thread merges one row's partials in window order.  Rows without in-edges
-// (the tail) only finish.
+// from its segments: every thread loads its region entries (all loads in
+// flight before the LDS stores), finding each entry's segment by binary
+// search over the segments' region offsets.  A one-row block merges its
+// region with its threads striding it in order, then a fixed tree; in a
+// multi-row block every thread merges one row's partials in window order.
+// Rows without in-edges (the tail) only finish.
 __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
                                                      float* __restrict__ score) {
   __shared__ float2 reg[kRowSlots];
   __shared__ int32_t soff[kSegBatch + 1];
   __shared__ int32_t sstart[kSegBatch];
   __shared__ float2 wred[kRowThreads / 64];
-  __shared__ int scan[kRowThreads / 64 + 1];
   __shared__ double red[kRowThreads / 64];
   if (a.ctrl->pad) return;
   double err = 0;
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
+  const int k = static_cast<int>(threadIdx.x);
   for (int64_t b = blockIdx.x; b < t.n_blocks; b += gridDim.x) {
     const int32_t r0 = t.brow[b], r1 = t.brow[b + 1];
     const int64_t s0 = t.bseg[b], s1 = t.bseg[b + 1];
+    const int32_t size = t.bsize[b];
     // Gather the region, kSegBatch segments per round.
-    int32_t done = 0;  // region entries gathered so far
     for (int64_t sb = s0; sb < s1; sb += kSegBatch) {
       const int ns = static_cast<int>(min<int64_t>(kSegBatch, s1 - sb));
-      const int k = static_cast<int>(threadIdx.x);
-      const int len = k < ns ? t.seg_len[sb + k] : 0;
-      if (k < ns) sstart[k] = t.seg_start[sb + k];
-      int total;
-      const int ex = block_excl_scan<kRowThreads>(len, scan, &total);
-      soff[k] = ex;
-      if (k == 0) soff[ns] = total;
+      if (k < ns) {
+        sstart[k] = t.seg_start[sb + k];
+        soff[k] = t.seg_off[sb + k];
+      }
+      if (k == 0) soff[ns] = sb + ns < s1 ? t.seg_off[sb + ns] : size;
       __syncthreads();
+      const int lo_e = soff[0], hi_e = soff[ns];
       float2 v[kRegionPer];
 #pragma unroll
       for (int i = 0; i < kRegionPer; i++) {
-        const int e = k + i * kRowThreads;
-        if (e < total) {
+        const int e = lo_e + k + i * kRowThreads;
+        if (e < hi_e) {
           int lo = 0, hi = ns;  // the last segment with soff <= e
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -624,19 +631,18 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, floa
       }
 #pragma unroll
       for (int i = 0; i < kRegionPer; i++) {
-        const int e = k + i * kRowThreads;
-        if (e < total) reg[done + e] = v[i];
+        const int e = lo_e + k + i * kRowThreads;
+        if (e < hi_e) reg[e] = v[i];
       }
-      done += total;
       __syncthreads();
     }
     if (r1 - r0 == 1) {
       FusedAcc<float> acc;
-      for (int p = threadIdx.x; p < done; p += kRowThreads) acc.merge(reg[p]);
+      for (int p = k; p < size; p += kRowThreads) acc.merge(reg[p]);
       acc.reduce<64>(lane);
       if (lane == 0) wred[wave] = make_float2(acc.c.s, acc.c.c);
       __syncthreads();
-      if (threadIdx.x == 0) {
+      if (k == 0) {
         Comp tot = {wred[0].x, wred[0].y};
         for (int w = 1; w < kRowThreads / 64; w++) tot.merge(wred[w].x, wred[w].y);
         err += finish_f32(a, r0, tot.value(), score, cout);
@@ -644,17 +650,10 @@ __global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, floa
       __syncthreads();
       continue;
     }
-    int64_t ib = t.ibase[b];
-    for (int32_t rr = r0; rr < r1; rr += kRowThreads) {
-      const int32_t r = rr + static_cast<int32_t>(threadIdx.x);
-      const int c = r < r1 ? t.rcnt[r] : 0;
-      int total;
-      const int ex = block_excl_scan<kRowThreads>(c, scan, &total);
-      if (r < r1) {
-        const Comp acc = merge_list(reg, t.idx + ib + ex, c);
-        err += finish_f32(a, r, acc.value(), score, cout);
-      }
-      ib += total;
+    for (int32_t r = r0 + k; r < r1; r += kRowThreads) {
+      const int32_t i0 = t.istart[r], i1 = t.istart[r + 1];
+      const Comp acc = merge_list(reg, t.idx + i0, i1 - i0);
+      err += finish_f32(a, r, acc.value(), score, cout);
     }
     __syncthreads();  // LDS readers done before the next block's gather
   }
@@ -827,10 +826,10 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       t.part = static_cast<float2*>(ctx->d_wt_part);
       t.brow = ctx->d_wt_brow;
       t.bseg = ctx->d_wt_bseg;
+      t.bsize = ctx->d_wt_bsize;
       t.seg_start = ctx->d_wt_seg_start;
-      t.seg_len = ctx->d_wt_seg_len;
-      t.ibase = ctx->d_wt_ibase;
-      t.rcnt = ctx->d_wt_rcnt;
+      t.seg_off = ctx->d_wt_seg_off;
+      t.istart = ctx->d_wt_istart;
       t.idx = ctx->d_wt_idx;
       t.n_blocks = ctx->n_row_blocks;
       const int tb = std::max(1, ctx->cu_count);

@@ -207,9 +207,9 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
   }
   const int64_t npieces = cbase[nchunks];
   // 6. Segments per block, in window order; region offsets; row lists.
-  std::vector<std::vector<int32_t>> bs_start(nb), bs_len(nb);
+  std::vector<std::vector<int32_t>> bs_start(nb), bs_off(nb);
   std::vector<int64_t> btot(nb, 0);
-  std::vector<uint16_t> rcnt(std::max<int64_t>(1, zb), 0);
+  std::vector<int32_t> rcnt(std::max<int64_t>(1, zb), 0);
   std::vector<char> multi(nb, 0);
   for (int64_t b = 0; b < nb; b++) multi[b] = brow[b + 1] - brow[b] > 1;
   std::vector<int32_t> rpos(npieces > 0 ? npieces : 1);  // region offset of every partial
@@ -221,16 +221,10 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
       size_t j = i;
       while (j < pc.size() && pc[j].block == b) j++;
       bs_start[b].push_back(static_cast<int32_t>(g0 + i));
-      bs_len[b].push_back(static_cast<int32_t>(j - i));
+      bs_off[b].push_back(static_cast<int32_t>(btot[b]));
       for (size_t k = i; k < j; k++) {
         rpos[g0 + k] = static_cast<int32_t>(btot[b]++);
-        if (multi[b]) {
-          if (rcnt[pc[k].row] == 0xFFFF) {
-            *err = "a row has too many PageRank tile pieces";
-            return DMM_ERR_CAPACITY;
-          }
-          rcnt[pc[k].row]++;
-        }
+        if (multi[b]) rcnt[pc[k].row]++;
       }
       i = j;
     }
@@ -240,30 +234,37 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
       *err = "a row block has more PageRank tile partials than the row kernel stages";
       return DMM_ERR_CAPACITY;
     }
-  std::vector<int64_t> bseg(nb + 1, 0), ibase(nb + 1, 0), rstart(std::max<int64_t>(1, zb), 0);
+  std::vector<int64_t> bseg(nb + 1, 0);
+  std::vector<int32_t> bsize(std::max<int64_t>(1, nb), 0);
   for (int64_t b = 0; b < nb; b++) {
     bseg[b + 1] = bseg[b] + static_cast<int64_t>(bs_start[b].size());
-    int64_t c = 0;
-    if (multi[b])
-      for (int64_t r = brow[b]; r < brow[b + 1]; r++) {
-        rstart[r] = ibase[b] + c;
-        c += rcnt[r];
-      }
-    ibase[b + 1] = ibase[b] + c;
+    bsize[b] = static_cast<int32_t>(btot[b]);
   }
-  std::vector<int32_t> seg_start(std::max<int64_t>(1, bseg[nb])), seg_len(std::max<int64_t>(1, bseg[nb]));
+  std::vector<int32_t> istart(zb + 1, 0);
+  for (int64_t r = 0; r < zb; r++) {
+    const int64_t next = int64_t(istart[r]) + rcnt[r];
+    if (next >= (int64_t(1) << 31)) {
+      *err = "PageRank tile row lists exceed 2^31 entries";
+      return DMM_ERR_CAPACITY;
+    }
+    istart[r + 1] = static_cast<int32_t>(next);
+  }
+  std::vector<int32_t> seg_start(std::max<int64_t>(1, bseg[nb])), seg_off(std::max<int64_t>(1, bseg[nb]));
   for (int64_t b = 0; b < nb; b++) {
     std::copy(bs_start[b].begin(), bs_start[b].end(), seg_start.begin() + bseg[b]);
-    std::copy(bs_len[b].begin(), bs_len[b].end(), seg_len.begin() + bseg[b]);
+    std::copy(bs_off[b].begin(), bs_off[b].end(), seg_off.begin() + bseg[b]);
   }
   bs_start.clear();
-  bs_len.clear();
-  std::vector<uint16_t> idx(std::max<int64_t>(1, ibase[nb]));
-  for (int32_t w = 0; w < nwin; w++) {
-    const std::vector<PieceRec>& pc = packs[w].pieces;
-    const int64_t g0 = cbase[wstart[w]];
-    for (size_t i = 0; i < pc.size(); i++)
-      if (multi[pc[i].block]) idx[rstart[pc[i].row]++] = static_cast<uint16_t>(rpos[g0 + i]);
+  bs_off.clear();
+  std::vector<uint16_t> idx(std::max<int64_t>(1, istart[zb]));
+  {
+    std::vector<int32_t> fill(istart.begin(), istart.end() - 1);
+    for (int32_t w = 0; w < nwin; w++) {
+      const std::vector<PieceRec>& pc = packs[w].pieces;
+      const int64_t g0 = cbase[wstart[w]];
+      for (size_t i = 0; i < pc.size(); i++)
+        if (multi[pc[i].block]) idx[fill[pc[i].row]++] = static_cast<uint16_t>(rpos[g0 + i]);
+    }
   }
   // 7. Chunk arrays.
   std::vector<int32_t> cwin(std::max<int64_t>(1, nchunks));
@@ -288,10 +289,10 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
   out->src.swap(src);
   out->brow.swap(brow);
   out->bseg.swap(bseg);
+  out->bsize.swap(bsize);
   out->seg_start.swap(seg_start);
-  out->seg_len.swap(seg_len);
-  out->ibase.swap(ibase);
-  out->rcnt.swap(rcnt);
+  out->seg_off.swap(seg_off);
+  out->istart.swap(istart);
   out->idx.swap(idx);
   out->n_pieces = npieces;
   out->slots_filled = filled;
@@ -321,10 +322,10 @@ int pr_tiles_build(dmm_ctx* ctx) {
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_src), th.src, "PR tile sources")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_brow), th.brow, "PR row blocks")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_bseg), th.bseg, "PR row block segments")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_bsize), th.bsize, "PR row block sizes")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_start), th.seg_start, "PR segments")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_len), th.seg_len, "PR segments")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_ibase), th.ibase, "PR row block lists")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_rcnt), th.rcnt, "PR row piece counts")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_off), th.seg_off, "PR segments")) ||
+      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_istart), th.istart, "PR row lists")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")) ||
       (rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) {
     ctx->tiles_bytes = ctx->dev_bytes - held;  // what the failed build holds, freed with the layout
@@ -507,13 +508,16 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
     rx.clear();
     ry.clear();
     for (int64_t s = th.bseg[b]; s < th.bseg[b + 1]; s++) {
-      if (th.seg_start[s] < 0 || th.seg_len[s] <= 0 || th.seg_start[s] + int64_t(th.seg_len[s]) > np)
+      const int64_t len = (s + 1 < th.bseg[b + 1] ? th.seg_off[s + 1] : th.bsize[b]) - th.seg_off[s];
+      if (th.seg_off[s] != static_cast<int64_t>(rx.size()) || th.seg_start[s] < 0 || len <= 0 ||
+          th.seg_start[s] + len > np)
         return bad("segment bounds");
-      for (int32_t k = 0; k < th.seg_len[s]; k++) {
+      for (int64_t k = 0; k < len; k++) {
         rx.push_back(px[th.seg_start[s] + k]);
         ry.push_back(py[th.seg_start[s] + k]);
       }
     }
+    if (static_cast<int64_t>(rx.size()) != th.bsize[b]) return bad("block size");
     const int64_t rn = static_cast<int64_t>(rx.size());
     if (rn > kRowSlots) return bad("block region beyond the row kernel's LDS");
     max_block = std::max(max_block, rn);
@@ -528,19 +532,17 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       incoming[r0] = tot.value();
       continue;
     }
-    int64_t ib = th.ibase[b];
     for (int32_t r = r0; r < r1; r++) {
       HostComp acc;
-      for (int k = 0; k < th.rcnt[r]; k++) {
-        if (ib + k >= th.ibase[b + 1]) return bad("row list beyond its block");
-        const int64_t q = th.idx[ib + k];
+      if (th.istart[r + 1] < th.istart[r] || th.istart[r + 1] > static_cast<int64_t>(th.idx.size()))
+        return bad("row list bounds");
+      for (int64_t k = th.istart[r]; k < th.istart[r + 1]; k++) {
+        const int64_t q = th.idx[k];
         if (q >= rn) return bad("partial offset beyond its block");
         acc.merge(rx[q], ry[q]);
       }
-      ib += th.rcnt[r];
       incoming[r] = acc.value();
     }
-    if (ib != th.ibase[b + 1]) return bad("row lists do not fill their block");
   }
   for (int64_t r = zb; r < n; r++) incoming[r] = 0.0f;
   stats[0] = nchunks;
