@@ -418,7 +418,8 @@ struct TileArgs {
 };
 
 constexpr int kTileThreads = 1024;
-constexpr int kBatch = 4;  // chunks a wavefront loads together
+constexpr int kBatch = 2;  // chunks a wavefront loads together
+constexpr int kPair = 2;   // chunks a wavefront sums side by side
 
 // What one lane reads of a chunk from HBM: its end mask / slot count, its
 // kLaneEdges 16-bit source offsets and the chunk's partial range.
@@ -462,53 +463,70 @@ __device__ __forceinline__ float2 split_pair(double s) {
 // lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
-__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane,
-                                           float2* stage) {
-  const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
-  const uint32_t ends = l.mt & 0xFFFFu;
-  const int cnt = static_cast<int>(l.mt >> 16);
-  float x[kLaneEdges];
+template <int N>
+__device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win, const ChunkLoad* l, int lane,
+                                            float2* stage) {
+  // N chunks side by side: their gathers, first passes and scans are
+  // independent chains the scheduler interleaves.
+  float x[N][kLaneEdges];
+  uint32_t ends[N];
 #pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) x[j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
+  for (int n = 0; n < N; n++) {
+    const uint32_t sw[8] = {l[n].q0.x, l[n].q0.y, l[n].q0.z, l[n].q0.w, l[n].q1.x, l[n].q1.y, l[n].q1.z, l[n].q1.w};
+    ends[n] = l[n].mt & 0xFFFFu;
 #pragma unroll
-  for (int j = 0; j < kLaneEdges; j++)
-    if (j >= cnt) x[j] = 0.0f;
-  double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) {
-    acc += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) acc = 0.0;
+    for (int j = 0; j < kLaneEdges; j++) x[n][j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
   }
-  const unsigned long long has_end = __ballot(ends != 0u);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  const unsigned long long h = has_end & upto;
-  const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
-  const double carry_out = seg_scan(acc, lane, seg);
-  double a = __shfl_up(carry_out, 1, 64);
-  if (lane == 0) a = 0.0;
-  const uint32_t k_i = __builtin_popcount(ends);
-  uint32_t q = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
+  double carry[N];
+  uint32_t q[N];
 #pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) {
-    a += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) {
-      stage[q] = split_pair(a);
-      q++;
-      a = 0.0;
+  for (int n = 0; n < N; n++) {
+    const int cnt = static_cast<int>(l[n].mt >> 16);
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++)
+      if (j >= cnt) x[n][j] = 0.0f;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++) {
+      acc += static_cast<double>(x[n][j]);
+      if ((ends[n] >> j) & 1u) acc = 0.0;
     }
+    const unsigned long long has_end = __ballot(ends[n] != 0u);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const unsigned long long h = has_end & upto;
+    const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
+    const double carry_out = seg_scan(acc, lane, seg);
+    carry[n] = __shfl_up(carry_out, 1, 64);
+    if (lane == 0) carry[n] = 0.0;
+    const uint32_t k_i = __builtin_popcount(ends[n]);
+    q[n] = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
   }
-  // The staged pieces are read back by other lanes of this wavefront (LDS
-  // operations of one wavefront complete in order).
-  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  // Always kChunkPieces / 64 stores, with no branch around them: lanes past
-  // the chunk's last piece write that piece again.
-  const int32_t c0 = l.c0, n = l.c1 - l.c0;
 #pragma unroll
-  for (int r = 0; r < kChunkPieces / 64; r++) {
-    const int e = min(lane + 64 * r, n - 1);
-    t.part[c0 + e] = stage[e];
+  for (int n = 0; n < N; n++) {
+    double a = carry[n];
+    uint32_t qq = q[n];
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++) {
+      a += static_cast<double>(x[n][j]);
+      if ((ends[n] >> j) & 1u) {
+        stage[qq] = split_pair(a);
+        qq++;
+        a = 0.0;
+      }
+    }
+    // The staged pieces are read back by other lanes of this wavefront (LDS
+    // operations of one wavefront complete in order).
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // Always kChunkPieces / 64 stores, with no branch around them: lanes
+    // past the chunk's last piece write that piece again.
+    const int32_t c0 = l[n].c0, np = l[n].c1 - l[n].c0;
+#pragma unroll
+    for (int r = 0; r < kChunkPieces / 64; r++) {
+      const int e = min(lane + 64 * r, np - 1);
+      t.part[c0 + e] = stage[e];
+    }
+    __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
   }
-  __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
@@ -555,7 +573,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
 #pragma unroll
       for (int k = 0; k < kBatch; k++) l[k] = load_chunk(t, min(cb + k * kWaves, wend - 1), lane);
 #pragma unroll
-      for (int k = 0; k < kBatch; k++) tile_chunk(t, win, l[k], lane, stage);
+      for (int k = 0; k < kBatch; k += kPair) tile_chunks<kPair>(t, win, l + k, lane, stage);
     }
     c = wend;
   }
