@@ -104,6 +104,7 @@ void drop_tiles(dmm_ctx* ctx) {
 
 void free_graph_state(dmm_ctx* ctx) {
   drop_tiles(ctx);
+  ctx->tiles_unusable = false;
   drop_graphs(ctx);
   void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
                   ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],
@@ -236,8 +237,18 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
     if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_contrib[1], sizeof(float) * nl, "contrib"))) return rc;
   }
   // Compensated mode on a degree-ordered whole graph: window-tiled layout.
-  if (mode == DMM_PR_FP32_KAHAN && ctx->relabeled && ctx->nloc == ctx->n && !ctx->tiles_ready) {
-    if ((rc = dmm::pr_tiles_build(ctx))) return rc;
+  // A graph whose rows do not fit the tiled layout (a row with more
+  // partials than the row kernel stages, or more than 2^31 partials) keeps
+  // the untiled compensated kernels.
+  if (mode == DMM_PR_FP32_KAHAN && ctx->relabeled && ctx->nloc == ctx->n && !ctx->tiles_ready &&
+      !ctx->tiles_unusable) {
+    rc = dmm::pr_tiles_build(ctx);
+    if (rc == DMM_ERR_CAPACITY) {
+      drop_tiles(ctx);
+      ctx->tiles_unusable = true;
+    } else if (rc) {
+      return rc;
+    }
     if (ctx->d_err_part) dmm::dev_free(ctx->d_err_part);  // resized below
     ctx->d_err_part = nullptr;
     drop_graphs(ctx);
@@ -1002,6 +1013,7 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
   if (t->pr_window != 0 && t->pr_window != ctx->tuning.pr_window) {
     ctx->tuning.pr_window = t->pr_window;
     drop_tiles(ctx);  // rebuilt with the new window at the next compensated call
+    ctx->tiles_unusable = false;
   }
   ctx->tuning.alpha = t->alpha;
   ctx->tuning.beta = t->beta;

@@ -215,6 +215,7 @@ struct dmm_ctx {
   void* d_wt_part = nullptr;        // float2 (sum, error) per piece
   int64_t n_chunks = 0, n_pieces = 0, n_row_blocks = 0, tile_slots_filled = 0, tiles_window = 0;
   int64_t tiles_bytes = 0;          // device bytes the layout holds
+  bool tiles_unusable = false;      // the graph does not fit the layout: untiled kernels
   int32_t n_windows = 0;
   // Rows of the light loop and the in-degree-0 tail (see the upload).
   int64_t pr_light_begin = 0, pr_light_end = 0, pr_zero_begin = 0;

@@ -35,14 +35,15 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 // merges a row's partials in window order.
 constexpr int kLaneEdges = 16;
 constexpr int kChunkEdges = 64 * kLaneEdges;
-constexpr int kChunkPieces = 256;   // pieces one chunk may hold
+constexpr int kChunkPieces = 192;   // pieces one chunk may hold
 // Row blocks: consecutive rows holding at most kBlockRuns runs; rows with
 // kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
 // A block's partials (its runs plus chunk cuts) must number at most
 // kRowSlots: the row kernel gathers them into LDS.
-constexpr int kRowSlots = 8192;
+constexpr int kRowSlots = 4096;
 constexpr int kRowThreads = 512;    // threads of the row kernel
-constexpr int kBlockRuns = 6144;
+constexpr int kRowGroupsPerCU = 4;  // row kernel workgroups resident per CU
+constexpr int kBlockRuns = 3072;
 constexpr int kSoloRuns = 256;
 constexpr int64_t kSoloEdges = 4096;
 

@@ -458,14 +458,14 @@ __device__ __forceinline__ float2 split_pair(double s) {
 // (hi, lo) = (float(s), float(s - hi)).
 //
 // The chunk's pieces (at most kChunkPieces), numbered in lane order, are
-// staged in the wavefront's LDS area and written to the chunk's contiguous
-// partial range by kChunkPieces / 64 store instructions of consecutive
-// lanes.  With that fixed count, a chunk whose loads were issued before
+// staged (as FP64 sums) in the wavefront's LDS area and written, split
+// into pairs, to the chunk's contiguous partial range by kChunkPieces / 64
+// store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
 template <int N>
 __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win, const ChunkLoad* l, int lane,
-                                            float2* stage) {
+                                            double* stage) {
   // N chunks side by side: their gathers, first passes and scans are
   // independent chains the scheduler interleaves.
   float x[N][kLaneEdges];
@@ -503,16 +503,18 @@ __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win,
   }
 #pragma unroll
   for (int n = 0; n < N; n++) {
+    // Second pass, branch-free: every slot writes its running sum, to the
+    // piece's staging entry where a piece ends and to the lane's own spare
+    // entry elsewhere.
     double a = carry[n];
     uint32_t qq = q[n];
 #pragma unroll
     for (int j = 0; j < kLaneEdges; j++) {
       a += static_cast<double>(x[n][j]);
-      if ((ends[n] >> j) & 1u) {
-        stage[qq] = split_pair(a);
-        qq++;
-        a = 0.0;
-      }
+      const uint32_t e = (ends[n] >> j) & 1u;
+      stage[e ? qq : kChunkPieces + lane] = a;
+      qq += e;
+      a = e ? 0.0 : a;
     }
     // The staged pieces are read back by other lanes of this wavefront (LDS
     // operations of one wavefront complete in order).
@@ -523,7 +525,7 @@ __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win,
 #pragma unroll
     for (int r = 0; r < kChunkPieces / 64; r++) {
       const int e = min(lane + 64 * r, np - 1);
-      t.part[c0 + e] = stage[e];
+      t.part[c0 + e] = split_pair(stage[e]);
     }
     __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
   }
@@ -537,13 +539,13 @@ template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
-  __shared__ float2 stage_all[kTileThreads / 64][kChunkPieces];
+  __shared__ double stage_all[kTileThreads / 64][kChunkPieces + 64];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar chunk indices
-  float2* stage = stage_all[wave];
+  double* stage = stage_all[wave];
   constexpr int kWaves = kTileThreads / 64;
   for (int64_t c = c0; c < c1;) {
     const int32_t w = t.cwin[c];
@@ -608,7 +610,7 @@ constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per threa
 // region with its threads striding it in order, then a fixed tree; in a
 // multi-row block every thread merges one row's partials in window order.
 // Rows without in-edges (the tail) only finish.
-__global__ __launch_bounds__(kRowThreads) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
+__global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
                                                      float* __restrict__ score) {
   __shared__ float2 reg[kRowSlots];
   __shared__ int32_t soff[kSegBatch + 1];
@@ -887,7 +889,7 @@ int pr_fused_blocks(dmm_ctx* ctx) {
       std::max<int64_t>(1, std::min<int64_t>(std::max(1, ctx->cu_count), ceil_div(lanes, kFusedThreads))));
 }
 
-int pr_row_blocks(dmm_ctx* ctx) { return 2 * std::max(1, ctx->cu_count); }
+int pr_row_blocks(dmm_ctx* ctx) { return kRowGroupsPerCU * std::max(1, ctx->cu_count); }
 
 int pr_super_blocks(dmm_ctx* ctx) {
   return static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_super, kThreads)));
