@@ -83,7 +83,7 @@ void drop_graphs(dmm_ctx* ctx) {
 // compensated call) and the captured graphs that use it.
 void drop_tiles(dmm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* bufs[] = {ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_wrounds, ctx->d_wt_cbase, ctx->d_wt_meta, ctx->d_wt_src,
+  void* bufs[] = {ctx->d_wt_cwin, ctx->d_wt_wstart, ctx->d_wt_cbase, ctx->d_wt_meta, ctx->d_wt_src,
                   ctx->d_wt_brow, ctx->d_wt_bseg, ctx->d_wt_bsize, ctx->d_wt_seg_start, ctx->d_wt_seg_off,
                   ctx->d_wt_istart, ctx->d_wt_idx, ctx->d_wt_part};
   const int64_t held = ctx->tiles_bytes;
@@ -91,7 +91,7 @@ void drop_tiles(dmm_ctx* ctx) {
   ctx->dev_bytes -= held;
   ctx->tiles_bytes = 0;
   ctx->d_wt_cwin = ctx->d_wt_cbase = ctx->d_wt_brow = ctx->d_wt_seg_start = ctx->d_wt_seg_off = nullptr;
-  ctx->d_wt_bsize = ctx->d_wt_istart = ctx->d_wt_wrounds = nullptr;
+  ctx->d_wt_bsize = ctx->d_wt_istart = nullptr;
   ctx->d_wt_wstart = ctx->d_wt_bseg = nullptr;
   ctx->d_wt_meta = nullptr;
   ctx->d_wt_part = nullptr;

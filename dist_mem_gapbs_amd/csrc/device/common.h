@@ -202,7 +202,6 @@ struct dmm_ctx {
   bool tiles_ready = false;
   int32_t* d_wt_cwin = nullptr;     // window of each chunk
   int64_t* d_wt_wstart = nullptr;   // first chunk of each window (+ end)
-  int32_t* d_wt_wrounds = nullptr;  // staging rounds of each window's chunks
   int32_t* d_wt_cbase = nullptr;    // first partial of each chunk (+ end)
   uint32_t* d_wt_meta = nullptr;    // per lane: end mask | filled slots << 16
   uint16_t* d_wt_src = nullptr;     // kChunkEdges source offsets per chunk

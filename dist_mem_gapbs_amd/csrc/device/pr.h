@@ -32,13 +32,10 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 // in a 64-lane chunk of its source's window, kLaneEdges edges per lane, and
 // is summed from an LDS copy of that window; each piece (a row's edges
 // inside one chunk) leaves one (sum, error) partial, and the row kernel
-// merges a row's partials in window order.  A window whose runs are short
-// gets more staging rounds per chunk, so its chunks stay full.
+// merges a row's partials in window order.
 constexpr int kLaneEdges = 16;
 constexpr int kChunkEdges = 64 * kLaneEdges;
-constexpr int kRoundPieces = 192;   // pieces staged per round (a wavefront's LDS area)
-constexpr int kMaxRounds = 4;       // rounds per chunk: a window's chunks hold up to
-                                    // rounds * kRoundPieces pieces
+constexpr int kChunkPieces = 192;   // pieces one chunk may hold
 // Row blocks: consecutive rows holding at most kBlockRuns runs; rows with
 // kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
 // A block's partials (its runs plus chunk cuts) must number at most
@@ -57,7 +54,6 @@ constexpr int64_t kSoloEdges = 4096;
 struct TileHost {
   std::vector<int32_t> cwin;     // window of each chunk
   std::vector<int64_t> wstart;   // first chunk of each window (+ end)
-  std::vector<int32_t> wrounds;  // staging rounds of each window's chunks
   std::vector<int32_t> cbase;    // first partial of each chunk (+ end)
   std::vector<uint32_t> meta;    // per lane: end mask (bit j: slot j ends a piece) | filled slots << 16
   std::vector<uint16_t> src;     // kChunkEdges per chunk

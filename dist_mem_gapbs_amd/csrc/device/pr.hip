@@ -402,13 +402,11 @@ __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t v) {
 struct TileArgs {
   const int32_t* cwin;     // window of each chunk
   const int64_t* wstart;   // first chunk of each window (+ end)
-  const int32_t* wrounds;  // staging rounds of each window's chunks
   const int32_t* cbase;    // first partial of each chunk (+ end)
   const uint32_t* meta;    // per lane: end mask | filled slots << 16
   const uint16_t* src;
   int64_t n_chunks;
-  float2* part;            // (sum, error) per piece, then kRoundPieces spare entries
-  int64_t n_pieces;
+  float2* part;            // (sum, error) per piece
   const int32_t* brow;     // row blocks: first row (+ end)
   const int64_t* bseg;     //             first segment (+ end)
   const int32_t* bsize;    //             partials (the region's size)
@@ -420,6 +418,8 @@ struct TileArgs {
 };
 
 constexpr int kTileThreads = 1024;
+constexpr int kBatch = 2;  // chunks a wavefront loads together
+constexpr int kPair = 2;   // chunks a wavefront sums side by side
 
 // What one lane reads of a chunk from HBM: its end mask / slot count, its
 // kLaneEdges 16-bit source offsets and the chunk's partial range.
@@ -457,133 +457,89 @@ __device__ __forceinline__ float2 split_pair(double s) {
 // and a piece enters its row's compensated FP32 sum as the unevaluated pair
 // (hi, lo) = (float(s), float(s - hi)).
 //
-// The chunk's pieces, numbered in lane order, are staged (as FP64 sums) in
-// the wavefront's LDS area and written, split into pairs, to the chunk's
-// contiguous partial range by store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
+// The chunk's pieces (at most kChunkPieces), numbered in lane order, are
+// staged (as FP64 sums) in the wavefront's LDS area and written, split
+// into pairs, to the chunk's contiguous partial range by kChunkPieces / 64
+// store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
-// One chunk (see tile_chunk's notes above), with its pieces staged and
-// written in R rounds of kRoundPieces: round r holds pieces [r *
-// kRoundPieces, (r + 1) * kRoundPieces) and ends with kRoundPieces / 64
-// stores, so every chunk of a window issues the same stores.
-template <int R>
-__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, const ChunkLoad& l, int lane,
-                                           double* stage) {
-  const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
-  const uint32_t ends = l.mt & 0xFFFFu;
-  const int cnt = static_cast<int>(l.mt >> 16);
-  float x[kLaneEdges];
+template <int N>
+__device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win, const ChunkLoad* l, int lane,
+                                            double* stage) {
+  // N chunks side by side: their gathers, first passes and scans are
+  // independent chains the scheduler interleaves.
+  float x[N][kLaneEdges];
+  uint32_t ends[N];
 #pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) x[j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
+  for (int n = 0; n < N; n++) {
+    const uint32_t sw[8] = {l[n].q0.x, l[n].q0.y, l[n].q0.z, l[n].q0.w, l[n].q1.x, l[n].q1.y, l[n].q1.z, l[n].q1.w};
+    ends[n] = l[n].mt & 0xFFFFu;
 #pragma unroll
-  for (int j = 0; j < kLaneEdges; j++)
-    if (j >= cnt) x[j] = 0.0f;
-  double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < kLaneEdges; j++) {
-    acc += static_cast<double>(x[j]);
-    if ((ends >> j) & 1u) acc = 0.0;
+    for (int j = 0; j < kLaneEdges; j++) x[n][j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
   }
-  const unsigned long long has_end = __ballot(ends != 0u);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  const unsigned long long h = has_end & upto;
-  const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
-  const double carry_out = seg_scan(acc, lane, seg);
-  double carry = __shfl_up(carry_out, 1, 64);
-  if (lane == 0) carry = 0.0;
-  const uint32_t k_i = __builtin_popcount(ends);
-  const uint32_t q0 = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
-  const int32_t c0 = l.c0, np = l.c1 - l.c0;
+  double carry[N];
+  uint32_t q[N];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    // Second pass, branch-free: every slot writes its running sum, to the
-    // piece's staging entry where a piece of this round ends and to the
-    // lane's own spare entry elsewhere.
-    double a = carry;
-    uint32_t qq = q0 - r * kRoundPieces;
+  for (int n = 0; n < N; n++) {
+    const int cnt = static_cast<int>(l[n].mt >> 16);
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++)
+      if (j >= cnt) x[n][j] = 0.0f;
+    double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < kLaneEdges; j++) {
-      a += static_cast<double>(x[j]);
-      const uint32_t e = (ends >> j) & 1u;
-      stage[(e && qq < static_cast<uint32_t>(kRoundPieces)) ? qq : kRoundPieces + lane] = a;
+      acc += static_cast<double>(x[n][j]);
+      if ((ends[n] >> j) & 1u) acc = 0.0;
+    }
+    const unsigned long long has_end = __ballot(ends[n] != 0u);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const unsigned long long h = has_end & upto;
+    const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
+    const double carry_out = seg_scan(acc, lane, seg);
+    carry[n] = __shfl_up(carry_out, 1, 64);
+    if (lane == 0) carry[n] = 0.0;
+    const uint32_t k_i = __builtin_popcount(ends[n]);
+    q[n] = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
+  }
+#pragma unroll
+  for (int n = 0; n < N; n++) {
+    // Second pass, branch-free: every slot writes its running sum, to the
+    // piece's staging entry where a piece ends and to the lane's own spare
+    // entry elsewhere.
+    double a = carry[n];
+    uint32_t qq = q[n];
+#pragma unroll
+    for (int j = 0; j < kLaneEdges; j++) {
+      a += static_cast<double>(x[n][j]);
+      const uint32_t e = (ends[n] >> j) & 1u;
+      stage[e ? qq : kChunkPieces + lane] = a;
       qq += e;
       a = e ? 0.0 : a;
     }
     // The staged pieces are read back by other lanes of this wavefront (LDS
     // operations of one wavefront complete in order).
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    // Always kRoundPieces / 64 stores, with no branch around them: lanes
-    // past the round's (or chunk's) last piece write that piece again.
-    const int nr = min(kRoundPieces, np - r * kRoundPieces);  // >= 1 unless the chunk has fewer rounds of pieces
+    // Always kChunkPieces / 64 stores, with no branch around them: lanes
+    // past the chunk's last piece write that piece again.
+    const int32_t c0 = l[n].c0, np = l[n].c1 - l[n].c0;
 #pragma unroll
-    for (int k = 0; k < kRoundPieces / 64; k++) {
-      const int e = max(0, min(lane + 64 * k, nr - 1));
-      const int slot = nr > 0 ? c0 + r * kRoundPieces + e : t.n_pieces + lane + 64 * k;
-      t.part[slot] = split_pair(stage[e]);
+    for (int r = 0; r < kChunkPieces / 64; r++) {
+      const int e = min(lane + 64 * r, np - 1);
+      t.part[c0 + e] = split_pair(stage[e]);
     }
-    __asm__ volatile("" ::: "memory");  // the next round's staging follows these reads
-  }
-}
-
-// R rounds of kRoundPieces / 64 stores into the spare area past the last
-// partial.
-template <int R>
-__device__ __forceinline__ void spare_stores(const TileArgs& t, int lane) {
-#pragma unroll
-  for (int r = 0; r < R * kRoundPieces / 64; r++) t.part[t.n_pieces + lane + 64 * (r % (kRoundPieces / 64))] = make_float2(0.0f, 0.0f);
-}
-
-// The chunks cb, cb + kWaves, ... < wend of one window, each loaded two
-// chunks ahead of its sum (three buffers in rotation; one chunk ahead with
-// two buffers when a chunk takes several rounds).  Every chunk is followed
-// by exactly R * kRoundPieces / 64 stores; spare-area store groups after
-// the loads ahead of the loop give its first iteration the same sequence of
-// memory operations as the later ones, so a chunk's wait never covers the
-// stores issued after its loads.  Chunks past the
-// range are the range's last chunk again (its partials rewritten with the
-// same values).
-template <int R>
-__device__ __forceinline__ void tile_range(const TileArgs& t, const float* win, int64_t cb, int64_t wend, int lane,
-                                           double* stage) {
-  constexpr int kWaves = kTileThreads / 64;
-  ChunkLoad A = load_chunk(t, cb, lane);
-  spare_stores<R>(t, lane);
-  if (R == 1) {
-    ChunkLoad B = load_chunk(t, min(cb + kWaves, wend - 1), lane);
-    spare_stores<R>(t, lane);
-    for (;; cb += 3 * kWaves) {
-      ChunkLoad C = load_chunk(t, min(cb + 2 * kWaves, wend - 1), lane);
-      tile_chunk<R>(t, win, A, lane, stage);
-      if (cb + kWaves >= wend) break;
-      A = load_chunk(t, min(cb + 3 * kWaves, wend - 1), lane);
-      tile_chunk<R>(t, win, B, lane, stage);
-      if (cb + 2 * kWaves >= wend) break;
-      B = load_chunk(t, min(cb + 4 * kWaves, wend - 1), lane);
-      tile_chunk<R>(t, win, C, lane, stage);
-      if (cb + 3 * kWaves >= wend) break;
-    }
-  } else {
-    // Denser work per chunk: one chunk ahead (two buffers).
-    for (;; cb += 2 * kWaves) {
-      ChunkLoad B = load_chunk(t, min(cb + kWaves, wend - 1), lane);
-      tile_chunk<R>(t, win, A, lane, stage);
-      if (cb + kWaves >= wend) break;
-      A = load_chunk(t, min(cb + 2 * kWaves, wend - 1), lane);
-      tile_chunk<R>(t, win, B, lane, stage);
-      if (cb + 2 * kWaves >= wend) break;
-    }
+    __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
   }
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
 // range of chunks (window-major); the workgroup copies one window of
 // contributions into LDS at a time and its wavefronts take the window's
-// chunks in turn.
+// chunks in turn, each loading its next chunk before it sums the current.
 template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
-  __shared__ double stage_all[kTileThreads / 64][kRoundPieces + 64];
+  __shared__ double stage_all[kTileThreads / 64][kChunkPieces + 64];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
@@ -594,7 +550,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
   for (int64_t c = c0; c < c1;) {
     const int32_t w = t.cwin[c];
     const int64_t wend = min(c1, t.wstart[w + 1]);
-    const int rounds = t.wrounds[w];
     const int64_t base = int64_t(w) * kWin;
     const int cnt = static_cast<int>(min<int64_t>(kWin, n_gather - base));
     __syncthreads();  // the previous window's readers are done
@@ -611,19 +566,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
       for (int j = threadIdx.x; j < cnt; j += kTileThreads) win[j] = cin[base + j];
     }
     __syncthreads();
-    const int64_t cb = c + wave;
-    if (cb < wend) {
-      if (rounds == 1)
-        tile_range<1>(t, win, cb, wend, lane, stage);
-      else if (rounds == 2)
-        tile_range<2>(t, win, cb, wend, lane, stage);
-      else if (rounds == 3)
-        tile_range<3>(t, win, cb, wend, lane, stage);
-      else
-        tile_range<4>(t, win, cb, wend, lane, stage);
+    // kBatch chunks per round: all their loads first, then each chunk in
+    // turn, so only the first waits for earlier stores.  Chunks past the
+    // range are the range's last chunk again (the same partials rewritten
+    // with the same values), keeping every round's memory operations fixed.
+    for (int64_t cb = c + wave; cb < wend; cb += kBatch * kWaves) {
+      ChunkLoad l[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) l[k] = load_chunk(t, min(cb + k * kWaves, wend - 1), lane);
+#pragma unroll
+      for (int k = 0; k < kBatch; k += kPair) tile_chunks<kPair>(t, win, l + k, lane, stage);
     }
     c = wend;
-    (void)kWaves;
   }
 }
 
@@ -885,13 +839,11 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       TileArgs t;
       t.cwin = ctx->d_wt_cwin;
       t.wstart = ctx->d_wt_wstart;
-      t.wrounds = ctx->d_wt_wrounds;
       t.cbase = ctx->d_wt_cbase;
       t.meta = ctx->d_wt_meta;
       t.src = ctx->d_wt_src;
       t.n_chunks = ctx->n_chunks;
       t.part = static_cast<float2*>(ctx->d_wt_part);
-      t.n_pieces = ctx->n_pieces;
       t.brow = ctx->d_wt_brow;
       t.bseg = ctx->d_wt_bseg;
       t.bsize = ctx->d_wt_bsize;

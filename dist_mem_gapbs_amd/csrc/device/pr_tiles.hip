@@ -9,9 +9,7 @@
 //
 // Tile layout (window-major, rows ascending inside a window):
 //   chunk    = 64 lanes x kLaneEdges edge slots, all of one window, holding
-//              at most rounds * kRoundPieces pieces (rounds per window: the
-//              window's pieces per full chunk over kRoundPieces, rounded
-//              up, at most kMaxRounds);
+//              at most kChunkPieces pieces;
 //   piece    = the part of a run inside one chunk (a run that crosses a
 //              chunk boundary is cut there); its sum is one partial;
 //   per lane = kLaneEdges 16-bit source offsets and an end mask (bit j:
@@ -139,20 +137,8 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
   });
   // 4. Pack every window into chunks.
   std::vector<WinPack> packs(nwin);
-  std::vector<int32_t> wrounds(nwin, 1);
   parallel_dynamic(nwin, nt, 1, [&](int64_t w) {
     WinPack& o = packs[w];
-    int64_t wedges = 0, wruns = 0;
-    for (int t = 0; t < nt; t++)
-      for (const RunRec& rr : recs[t][w]) {
-        wedges += rr.len;
-        wruns++;
-      }
-    const int64_t per_chunk = wedges > 0 ? (wruns * kChunkEdges + wedges - 1) / wedges : 1;
-    const int rounds = static_cast<int>(
-        std::min<int64_t>(kMaxRounds, std::max<int64_t>(1, (per_chunk + kRoundPieces - 1) / kRoundPieces)));
-    wrounds[w] = rounds;
-    const int32_t cap = rounds * kRoundPieces;
     const int64_t base = w * W;
     int64_t ch = -1;
     int lane = 64, slot = 0;
@@ -177,7 +163,7 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
           lane++;
           slot = 0;
         }
-        if (lane >= 64 || o.cpieces[ch] == cap) new_chunk();
+        if (lane >= 64 || o.cpieces[ch] == kChunkPieces) new_chunk();
         start_piece(rr.row, blk);
         for (int32_t k = 0; k < rr.len; k++) {
           if (slot == kLaneEdges) {
@@ -298,7 +284,6 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
   packs.shrink_to_fit();
   out->cwin.swap(cwin);
   out->wstart.swap(wstart);
-  out->wrounds.swap(wrounds);
   out->cbase.swap(cbase);
   out->meta.swap(meta);
   out->src.swap(src);
@@ -332,7 +317,6 @@ int pr_tiles_build(dmm_ctx* ctx) {
   std::vector<int32_t>().swap(nbr);
   if ((rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_cwin), th.cwin, "PR tile chunk windows")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_wstart), th.wstart, "PR tile window starts")) ||
-      (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_wrounds), th.wrounds, "PR tile window rounds")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_cbase), th.cbase, "PR tile chunk partials")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_meta), th.meta, "PR tile lanes")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_src), th.src, "PR tile sources")) ||
@@ -343,7 +327,7 @@ int pr_tiles_build(dmm_ctx* ctx) {
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_off), th.seg_off, "PR segments")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_istart), th.istart, "PR row lists")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")) ||
-      (rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * (th.n_pieces + kMaxRounds * kRoundPieces), "PR tile partials"))) {
+      (rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) {
     ctx->tiles_bytes = ctx->dev_bytes - held;  // what the failed build holds, freed with the layout
     return rc;
   }
@@ -476,9 +460,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         tail[l] = acc;
         has_end[l] = ends != 0;
       }
-      if (th.wrounds[w] < 1 || th.wrounds[w] > kMaxRounds || total > th.wrounds[w] * kRoundPieces ||
-          total != th.cbase[c + 1] - th.cbase[c])
-        return bad("chunk piece count");
+      if (total > kChunkPieces || total != th.cbase[c + 1] - th.cbase[c]) return bad("chunk piece count");
       double sc[64];
       host_seg_scan(tail, has_end, sc);
       // The lane's second pass: its pieces, the first one from the carry,
