@@ -172,6 +172,7 @@ SIGNATURES = {
     "dmm_batch_block_level": (_I32, [_P, _PU64, _PU64, _PU64, C.POINTER(BlockLevelStats)]),
     "dmm_pr_pull": (_I32, [_P, _I32, C.c_double, _I32, _P, C.POINTER(PrStats)]),
     "dmm_pr_block_iteration": (_I32, [_P, _PF32, _PF32, _I32, _PD, _PD]),
+    "dmm_pr_block_iteration_noalloc": (_I32, [_P, _P, _P, _P, _I32, _P]),
     "dmm_pr_tiles_check": (_I32, [_I64, _I64, _I64, _PI64, _PI32, _I64, _PF32, _PF32, _PI64]),
     "dmm_autotune": (_I32, [_P, _PI32, _I32, _I32, C.POINTER(Tuning)]),
     "dmm_set_tuning": (_I32, [_P, C.POINTER(Tuning)]),

@@ -368,6 +368,8 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
     return bail(set_err(nullptr, DMM_ERR_HIP, "hipGetDeviceProperties failed"));
   ctx->cu_count = prop.multiProcessorCount;
   ctx->arch = prop.gcnArchName;
+  int rc0 = dmm::scratch_init(ctx);
+  if (rc0) return bail(rc0);
   // Default-priority, non-blocking stream.
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(set_err(nullptr, DMM_ERR_HIP, "stream creation failed"));
@@ -998,6 +1000,18 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib, float* sco
     return set_err(ctx, DMM_ERR_INVALID, "block iteration supports the FP32 modes");
   if ((rc = ensure_pr_state(ctx, mode))) return rc;
   return dmm::pr_block_iteration(ctx, global_contrib, scores_inout, mode, err, ms);
+}
+
+int dmm_pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_global_contrib, float* d_scores,
+                                   float* d_contrib_out, int32_t mode, double* d_err) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!d_global_contrib || !d_scores || !d_contrib_out || !d_err) return set_err(ctx, DMM_ERR_INVALID, "NULL buffer");
+  if (mode != DMM_PR_FP32_SERIAL && mode != DMM_PR_FP32_KAHAN)
+    return set_err(ctx, DMM_ERR_INVALID, "block iteration supports the FP32 modes");
+  if (ctx->relabeled)
+    return set_err(ctx, DMM_ERR_STATE, "block iteration needs DMM_LAYOUT_NATURAL (contributions in the caller's IDs)");
+  return dmm::pr_block_iteration_noalloc(ctx, d_global_contrib, d_scores, d_contrib_out, mode, d_err);
 }
 
 // ---------------------------------------------------------------------------

@@ -123,6 +123,7 @@ struct dmm_ctx {
   std::string arch;
   int cu_count = 0;
   int64_t dev_bytes = 0;    // device memory held
+  unsigned char* scratch = nullptr;  // this device's scratch array (allocation-free calls)
   int64_t fixed_bytes = 0;  // ... of it by the ctx itself (not the graph)
 
   // Graph (one PE's block).

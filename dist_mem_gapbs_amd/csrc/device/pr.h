@@ -88,6 +88,13 @@ constexpr size_t kPrParamsBytes = 64;
 int pr_run(dmm_ctx* ctx, int32_t max_iters, double epsilon, int32_t mode, dmm_pr_stats* stats);
 int pr_block_iteration(dmm_ctx* ctx, const float* gcontrib_host, float* scores_inout, int32_t mode,
                        double* err, double* ms);
+// The context's device address of this GPU's scratch array (once per
+// device), kept in ctx->scratch.
+int scratch_init(dmm_ctx* ctx);
+// pr_block_iteration on device buffers with every temporary in the scratch
+// array: no device allocation.
+int pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_gcontrib, float* d_scores, float* d_contrib_out,
+                               int32_t mode, double* d_err);
 
 }  // namespace dmm
 

@@ -25,6 +25,7 @@
 #include "pr.h"
 
 #include <algorithm>
+#include <mutex>
 
 #include "layout.h"
 
@@ -815,15 +816,21 @@ PrArgs make_args(dmm_ctx* ctx) {
   return a;
 }
 
-void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcontrib) {
+// One iteration on ctx->stream.  With gcontrib (a PE block's iteration from
+// the global contribution vector) the FP32 modes may write the scores and
+// the block's new contributions to score_out / contrib_out instead of the
+// context's own buffers.
+void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcontrib, float* score_out = nullptr,
+                      float* contrib_out = nullptr) {
   hipStream_t s = ctx->stream;
   const int sb = a.super_blocks;
+  float* const score_f = score_out ? score_out : ctx->d_score;
   if (mode == DMM_PR_FP32_SERIAL) {
     const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
-    float* cout = ctx->d_contrib[(it + 1) & 1];
+    float* cout = contrib_out ? contrib_out : ctx->d_contrib[(it + 1) & 1];
     a.main_blocks = a.light_blocks + a.heavy_blocks;
-    hipLaunchKernelGGL(k_serial_light, dim3(a.light_blocks), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
-    hipLaunchKernelGGL(k_serial_heavy, dim3(a.heavy_blocks), dim3(kThreads), 0, s, a, cin, cout, ctx->d_score);
+    hipLaunchKernelGGL(k_serial_light, dim3(a.light_blocks), dim3(kThreads), 0, s, a, cin, cout, score_f);
+    hipLaunchKernelGGL(k_serial_heavy, dim3(a.heavy_blocks), dim3(kThreads), 0, s, a, cin, cout, score_f);
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(kThreads), 0, s, a);
   } else {
     const int fb = pr_fused_blocks(ctx);
@@ -862,9 +869,9 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       hipLaunchKernelGGL(k_rows, dim3(a.main_blocks), dim3(kRowThreads), 0, s, a, t, cout, ctx->d_score);
     } else {
       const float* cin = gcontrib ? static_cast<const float*>(gcontrib) : ctx->d_contrib[it & 1];
-      float* cout = ctx->d_contrib[(it + 1) & 1];
-      hipLaunchKernelGGL(k_fused<float>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, ctx->d_score);
-      hipLaunchKernelGGL(k_kahan_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score);
+      float* cout = contrib_out ? contrib_out : ctx->d_contrib[(it + 1) & 1];
+      hipLaunchKernelGGL(k_fused<float>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, score_f);
+      hipLaunchKernelGGL(k_kahan_super, dim3(sb), dim3(kThreads), 0, s, a, cout, score_f);
     }
   }
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, s, a);
@@ -1007,6 +1014,108 @@ int pr_block_iteration(dmm_ctx* ctx, const float* gcontrib_host, float* scores_i
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&f, ctx->ev_a, ctx->ev_b));
     *ms = f;
   }
+  return DMM_OK;
+}
+
+// ---- allocation-free PE-block iteration -------------------------------------
+//
+// The same kernels as pr_block_iteration, with everything they need beyond
+// the caller's buffers and the uploaded graph -- constants, control block,
+// error partials, split-row segment partials -- in one file-scope device
+// array.  Each GPU holds its own copy of the array for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress
+// returns for that device (scratch_ptr), never through the array's name.
+// Calls that overlap on one GPU share it and race.
+
+namespace {
+
+constexpr size_t kScratchBytes = size_t(4) << 20;  // per GPU
+constexpr size_t kScratchParams = 0;                // PrParams
+constexpr size_t kScratchCtrl = 64;                 // PrCtrl
+constexpr size_t kScratchTrace = 96;                // one error-trace entry
+constexpr size_t kScratchErr = 256;                 // error partials
+constexpr int kScratchErrParts = 8192;
+constexpr size_t kScratchSegs = kScratchErr + sizeof(double) * kScratchErrParts;  // segment partials
+constexpr int64_t kScratchSegCap = static_cast<int64_t>((kScratchBytes - kScratchSegs) / sizeof(double2));
+static_assert(sizeof(PrParams) <= kScratchCtrl && sizeof(PrCtrl) <= kScratchTrace - kScratchCtrl, "scratch head");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+__global__ void k_scratch_prep(unsigned char* base, PrParams p) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *reinterpret_cast<PrParams*>(base + kScratchParams) = p;
+    PrCtrl* c = reinterpret_cast<PrCtrl*>(base + kScratchCtrl);
+    c->err = 0;
+    c->ticket = 0;
+    c->pad = 0;
+  }
+}
+
+__global__ void k_scratch_err(const unsigned char* base, double* err_out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *err_out = reinterpret_cast<const PrCtrl*>(base + kScratchCtrl)->err;
+}
+
+}  // namespace
+
+int scratch_init(dmm_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+    return set_err(ctx, DMM_ERR_INVALID, "device %d beyond the scratch table", ctx->device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(ctx, DMM_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = static_cast<unsigned char*>(p);
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return DMM_OK;
+}
+
+int pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_gcontrib, float* d_scores, float* d_contrib_out,
+                               int32_t mode, double* d_err) {
+  unsigned char* base = ctx->scratch;
+  if (!base) return set_err(ctx, DMM_ERR_STATE, "the context has no scratch address");
+  PrArgs a = make_args(ctx);
+  const int parts = mode == DMM_PR_FP32_SERIAL ? a.light_blocks + a.heavy_blocks + a.super_blocks
+                                               : pr_fused_blocks(ctx) + a.super_blocks;
+  if (parts > kScratchErrParts)
+    return set_err(ctx, DMM_ERR_CAPACITY, "%d error partials exceed the %d the scratch array holds", parts,
+                   kScratchErrParts);
+  if (ctx->n_slots > kScratchSegCap)
+    return set_err(ctx, DMM_ERR_CAPACITY, "%lld split-row segments exceed the %lld the scratch array holds",
+                   (long long)ctx->n_slots, (long long)kScratchSegCap);
+  a.prm = reinterpret_cast<const PrParams*>(base + kScratchParams);
+  a.ctrl = reinterpret_cast<PrCtrl*>(base + kScratchCtrl);
+  a.err_trace = reinterpret_cast<double*>(base + kScratchTrace);
+  a.err_part = reinterpret_cast<double*>(base + kScratchErr);
+  a.partial = reinterpret_cast<double2*>(base + kScratchSegs);
+  // Constants as the reference computes them in float (src/pr.cc:39, 45-46);
+  // no convergence test (one iteration).
+  PrParams p;
+  const float kDamp = 0.85f;
+  const float nf = static_cast<float>(ctx->n);
+  p.eps = -1.0;
+  p.damp_f = kDamp;
+  p.base_f = (1.0f - kDamp) / nf;
+  p.init_f = 1.0f / nf;
+  p.damp_d = static_cast<double>(kDamp);
+  p.base_d = static_cast<double>(p.base_f);
+  p.init_d = static_cast<double>(p.init_f);
+  p.pad = 0;
+  hipLaunchKernelGGL(k_scratch_prep, dim3(1), dim3(64), 0, ctx->stream, base, p);
+  launch_iteration(ctx, a, mode, 0, d_gcontrib, d_scores, d_contrib_out);
+  hipLaunchKernelGGL(k_scratch_err, dim3(1), dim3(64), 0, ctx->stream, base, d_err);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DMM_OK;
 }
 

@@ -375,6 +375,21 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
                            float* scores_inout, int32_t mode, double* err,
                            double* ms);
 
+/* dmm_pr_block_iteration on device buffers, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * d_global_contrib: N floats; d_scores: the block's vend - vbegin scores,
+ * updated in place; d_contrib_out: the block's new contributions (scores
+ * over out-degrees); d_err: one double, the block's L1 change.  All device
+ * pointers.  Every temporary lives in a 4 MiB array each GPU holds for the
+ * life of the process; two calls that overlap on one GPU share it and race.
+ * Returns DMM_ERR_CAPACITY for a block whose error partials or split-row
+ * segments do not fit that array.  mode: DMM_PR_FP32_SERIAL or
+ * DMM_PR_FP32_KAHAN; the upload must use DMM_LAYOUT_NATURAL. */
+int dmm_pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_global_contrib,
+                                   float* d_scores, float* d_contrib_out, int32_t mode,
+                                   double* d_err);
+
 /* Host check of the window-tiled layout that DMM_PR_FP32_KAHAN uses on a
  * degree-ordered whole graph (no GPU involved; for tests).  Builds the
  * layout of the in-CSR (n rows, sources in [0, n_gather), rows at and above

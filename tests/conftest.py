@@ -54,6 +54,56 @@ def csr_from_pairs(dmm, pairs, n=-1, symmetrize=True):
     return dmm.from_edgelist(np.asarray(pairs, dtype=np.int32).reshape(-1, 2), n, symmetrize)
 
 
+class HipBuffer:
+    """Device memory from the HIP runtime the library itself links (test
+    plumbing for entry points that take device pointers)."""
+
+    _rt = None
+
+    @classmethod
+    def runtime(cls):
+        if cls._rt is None:
+            import ctypes as C
+            import dist_mem_gapbs_amd._lib as L
+            L.load()  # maps libamdhip64.so.7; the dlopen below returns that same object
+            rt = C.CDLL("libamdhip64.so.7")
+            rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            rt.hipFree.argtypes = [C.c_void_p]
+            cls._rt = rt
+        return cls._rt
+
+    def __init__(self, nbytes):
+        import ctypes as C
+        self.nbytes = max(1, int(nbytes))
+        p = C.c_void_p()
+        assert self.runtime().hipMalloc(C.byref(p), self.nbytes) == 0
+        self.ptr = p
+
+    @classmethod
+    def of(cls, arr):
+        a = np.ascontiguousarray(arr)
+        b = cls(a.nbytes)
+        assert cls.runtime().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return b
+
+    def read(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert self.runtime().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
 class CapturedStdout:
     """Captures what C code writes to file descriptor 1 (printf) while the
     block runs; the text is in .text afterwards."""
