@@ -147,6 +147,7 @@ SIGNATURES = {
     "dmm_device_count": (_I32, [_PI32]),
     "dmm_ctx_create": (_I32, [_I32, C.POINTER(_P)]),
     "dmm_ctx_destroy": (None, [_P]),
+    "dmm_ctx_synchronize": (_I32, [_P]),
     "dmm_build_graph_device": (_I32, [_P, _I64, _PI32, _I64, _I32, C.POINTER(_P)]),
     "dmm_bc_forward": (_I32, [_P, _I32, _PI32, _PU64, _PU64, _PI32]),
     "dmm_graph_upload": (_I32, [_P, _I64, _I64, _I64, _PI64, _PI32, _PI64, _PI32, _I64]),
@@ -167,6 +168,9 @@ SIGNATURES = {
     "dmm_state_get_bitmap": (_I32, [_P, _I32, _PU64]),
     "dmm_state_set_bitmap": (_I32, [_P, _I32, _PU64]),
     "dmm_state_set_level": (_I32, [_P, _I32]),
+    "dmm_generate_edgelist_device": (_I32, [_P, _I32, _I32, _I32, _PI32]),
+    "dmm_make_graph_device": (_I32, [_P, _I32, _I32, _I32, C.POINTER(_P)]),
+    "dmm_batch_read_depth": (_I32, [_P, _PU8]),
     "dmm_bfs_batch64": (_I32, [_P, _PI32, _I32, _PU8, C.POINTER(BatchParams), C.POINTER(BatchStats)]),
     "dmm_batch_read_level": (_I32, [_P, _I32, _PU64]),
     "dmm_batch_block_level": (_I32, [_P, _PU64, _PU64, _PU64, C.POINTER(BlockLevelStats)]),

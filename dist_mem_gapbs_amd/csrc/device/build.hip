@@ -78,28 +78,6 @@ __global__ __launch_bounds__(kBThreads) void k_cols(const uint64_t* __restrict__
     nbr[i] = static_cast<int32_t>(static_cast<uint32_t>(keys[i]));
 }
 
-// A transient device buffer of the builder; its bytes count towards the
-// context's device memory only while it is held.
-struct DevBuf {
-  dmm_ctx* ctx = nullptr;
-  void* p = nullptr;
-  size_t bytes = 0;
-  int alloc(dmm_ctx* c, size_t b, const char* what) {
-    int rc = dev_alloc(c, &p, b, what);
-    if (rc == DMM_OK) {
-      ctx = c;
-      bytes = b ? b : 8;  // dev_alloc's minimum
-    }
-    return rc;
-  }
-  ~DevBuf() {
-    if (p) {
-      dev_free(p);
-      ctx->dev_bytes -= static_cast<int64_t>(bytes);
-    }
-  }
-};
-
 int bit_length(uint64_t x) {
   int b = 0;
   while (x) {
@@ -166,6 +144,15 @@ int build_direction(dmm_ctx* ctx, const int2* d_el, int64_t ne, int64_t n, int m
 
 }  // namespace
 
+int build_graph_device_el(dmm_ctx* ctx, const int2* d_el, int64_t ne, int64_t n, bool symmetrize, HostGraph* g) {
+  g->num_nodes = n;
+  g->directed = !symmetrize;
+  int rc;
+  if ((rc = build_direction(ctx, d_el, ne, n, symmetrize ? 0 : 1, &g->out_offsets, &g->out_neighs))) return rc;
+  if (!symmetrize && (rc = build_direction(ctx, d_el, ne, n, 2, &g->in_offsets, &g->in_neighs))) return rc;
+  return DMM_OK;
+}
+
 int build_graph_device(dmm_ctx* ctx, const int32_t* pairs, int64_t ne, int64_t n, bool symmetrize, HostGraph* g) {
   DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -173,12 +160,7 @@ int build_graph_device(dmm_ctx* ctx, const int32_t* pairs, int64_t ne, int64_t n
   int rc;
   if ((rc = el.alloc(ctx, sizeof(int2) * std::max<int64_t>(1, ne), "builder edge list"))) return rc;
   if (ne > 0) DMM_HIP_CHECK(ctx, hipMemcpy(el.p, pairs, sizeof(int2) * ne, hipMemcpyHostToDevice));
-  g->num_nodes = n;
-  g->directed = !symmetrize;
-  const int2* d_el = static_cast<const int2*>(el.p);
-  if ((rc = build_direction(ctx, d_el, ne, n, symmetrize ? 0 : 1, &g->out_offsets, &g->out_neighs))) return rc;
-  if (!symmetrize && (rc = build_direction(ctx, d_el, ne, n, 2, &g->in_offsets, &g->in_neighs))) return rc;
-  return DMM_OK;
+  return build_graph_device_el(ctx, static_cast<const int2*>(el.p), ne, n, symmetrize, g);
 }
 
 }  // namespace dmm

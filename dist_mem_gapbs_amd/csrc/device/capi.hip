@@ -15,6 +15,7 @@
 #include "bc.h"
 #include "bfs.h"
 #include "build.h"
+#include "gen.h"
 #include "common.h"
 #include "layout.h"
 #include "pr.h"
@@ -408,6 +409,13 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
   return DMM_OK;
 }
 
+int dmm_ctx_synchronize(dmm_ctx* ctx) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
+}
+
 void dmm_ctx_destroy(dmm_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
@@ -462,6 +470,53 @@ int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs
   dmm_host_graph* hg = new (std::nothrow) dmm_host_graph;
   if (!hg) return set_err(ctx, DMM_ERR_NOMEM, "host allocation failed");
   const int rc = dmm::build_graph_device(ctx, pairs, num_pairs, num_nodes, symmetrize != 0, &hg->g);
+  if (rc != DMM_OK) {
+    delete hg;
+    return rc;
+  }
+  *out = hg;
+  return DMM_OK;
+}
+
+namespace {
+
+int check_generator_args(dmm_ctx* ctx, int32_t scale, int32_t degree) {
+  if (!ctx) return set_err(nullptr, DMM_ERR_INVALID, "ctx is NULL");
+  if (scale < 1 || scale > 30) return set_err(ctx, DMM_ERR_INVALID, "scale %d outside [1, 30]", scale);
+  if (degree < 1 || degree > 1024) return set_err(ctx, DMM_ERR_INVALID, "degree %d outside [1, 1024]", degree);
+  DMM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return DMM_OK;
+}
+
+}  // namespace
+
+int dmm_generate_edgelist_device(dmm_ctx* ctx, int32_t scale, int32_t degree, int32_t uniform, int32_t* pairs) {
+  int rc = check_generator_args(ctx, scale, degree);
+  if (rc) return rc;
+  if (!pairs) return set_err(ctx, DMM_ERR_INVALID, "pairs is NULL");
+  const int64_t ne = (int64_t(1) << scale) * degree;
+  dmm::DevBuf el;
+  if ((rc = el.alloc(ctx, sizeof(int2) * ne, "generated edge list"))) return rc;
+  if ((rc = dmm::generate_edgelist_device(ctx, scale, degree, uniform != 0, static_cast<int2*>(el.p)))) return rc;
+  DMM_HIP_CHECK(ctx, hipMemcpy(pairs, el.p, sizeof(int2) * ne, hipMemcpyDeviceToHost));
+  return DMM_OK;
+}
+
+int dmm_make_graph_device(dmm_ctx* ctx, int32_t scale, int32_t degree, int32_t uniform, dmm_host_graph** out) {
+  int rc = check_generator_args(ctx, scale, degree);
+  if (rc) return rc;
+  if (!out) return set_err(ctx, DMM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  const int64_t ne = (int64_t(1) << scale) * degree;
+  dmm::DevBuf el;
+  if ((rc = el.alloc(ctx, sizeof(int2) * ne, "generated edge list"))) return rc;
+  const int2* d_el = static_cast<const int2*>(el.p);
+  if ((rc = dmm::generate_edgelist_device(ctx, scale, degree, uniform != 0, static_cast<int2*>(el.p)))) return rc;
+  int64_t max_id = -1;
+  if ((rc = dmm::max_id_device(ctx, d_el, ne, &max_id))) return rc;
+  dmm_host_graph* hg = new (std::nothrow) dmm_host_graph;
+  if (!hg) return set_err(ctx, DMM_ERR_NOMEM, "host allocation failed");
+  rc = dmm::build_graph_device_el(ctx, d_el, ne, max_id + 1, true, &hg->g);
   if (rc != DMM_OK) {
     delete hg;
     return rc;

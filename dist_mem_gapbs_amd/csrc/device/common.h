@@ -285,6 +285,31 @@ int set_err(dmm_ctx* ctx, int code, const char* fmt, ...);
 int dev_alloc(dmm_ctx* ctx, void** p, size_t bytes, const char* what);
 void dev_free(void* p);
 
+// A transient device buffer; its bytes count towards the context's device
+// memory only while it is held.
+struct DevBuf {
+  dmm_ctx* ctx = nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  int alloc(dmm_ctx* c, size_t b, const char* what) {
+    int rc = dev_alloc(c, &p, b, what);
+    if (rc == DMM_OK) {
+      ctx = c;
+      bytes = b ? b : 8;  // dev_alloc's minimum
+    }
+    return rc;
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) {
+      dev_free(p);
+      ctx->dev_bytes -= static_cast<int64_t>(bytes);
+    }
+  }
+};
+
 // ---- wave64 helpers -------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

@@ -106,6 +106,24 @@ class Context:
         the next upload.  The step API needs the natural layout."""
         self._check(self._lib.dmm_ctx_set_layout(self._h, layout))
 
+    def generate_edgelist(self, scale, degree=16, uniform=False):
+        """Generator::GenerateEL on this context's GPU: int32[num_pairs, 2]."""
+        pairs = np.empty(((1 << scale) * degree, 2), np.int32)
+        self._check(self._lib.dmm_generate_edgelist_device(self._h, scale, degree, 1 if uniform else 0,
+                                                           L.ptr(pairs, C.c_int32)))
+        return pairs
+
+    def make_graph(self, scale, degree=16, uniform=False):
+        """Generate and build the symmetrized graph on this context's GPU."""
+        from .graph import HostGraph
+        h = C.c_void_p()
+        self._check(self._lib.dmm_make_graph_device(self._h, scale, degree, 1 if uniform else 0, C.byref(h)))
+        return HostGraph(h)
+
+    def synchronize(self):
+        """Waits for all work queued on the context's stream."""
+        self._check(self._lib.dmm_ctx_synchronize(self._h))
+
     def device_bytes(self):
         b = C.c_int64()
         self._check(self._lib.dmm_ctx_device_bytes(self._h, C.byref(b)))

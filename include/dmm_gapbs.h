@@ -142,6 +142,11 @@ int dmm_device_count(int32_t* count);
 int dmm_ctx_create(int32_t device, dmm_ctx** out);
 void dmm_ctx_destroy(dmm_ctx* ctx);
 
+/* Waits until all work the ctx has queued on its stream has finished: the
+ * barrier that brackets a timed kernel in BenchmarkKernel
+ * (src/benchmark.h:118-145, shmem_barrier_all) for one PE. */
+int dmm_ctx_synchronize(dmm_ctx* ctx);
+
 /* Replaces BuilderBase::MakeGraphFromEL + SquishGraph (src/builder.h:290-312,
  * 168-226) on ctx's GPU: the same squished CSR as dmm_build_graph_from_edgelist
  * (rows sorted, duplicates and self-loops removed; num_nodes < 0 means max ID
@@ -151,6 +156,21 @@ void dmm_ctx_destroy(dmm_ctx* ctx);
  * device memory the build used is freed before the call returns. */
 int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs,
                            int64_t num_nodes, int32_t symmetrize, dmm_host_graph** out);
+
+/* Replaces Generator::GenerateEL (src/generator.h:169-180) on ctx's GPU:
+ * the same list as dmm_generate_edgelist, bit for bit (block-seeded mt19937
+ * streams, one wavefront per 2^18-pair block; the R-MAT ID permutation is
+ * drawn on the host while the GPU generates and applied on the GPU).  pairs
+ * (host) receives 2 * 2^scale * degree int32. */
+int dmm_generate_edgelist_device(dmm_ctx* ctx, int32_t scale, int32_t degree,
+                                 int32_t uniform, int32_t* pairs);
+
+/* Replaces BuilderBase::MakeGraph for synthetic input (src/builder.h:
+ * 314-338, symmetrized): dmm_generate_edgelist_device followed by the GPU
+ * build of dmm_build_graph_device, the edge list never leaving HBM
+ * (num_nodes = max ID + 1).  The same graph as dmm_build_graph. */
+int dmm_make_graph_device(dmm_ctx* ctx, int32_t scale, int32_t degree, int32_t uniform,
+                          dmm_host_graph** out);
 
 /* Replaces shmem_BFS, Brandes' forward phase (src/bc.cc:55-96), on the
  * uploaded whole graph from `source`: depth (int32[N], -1 = unreachable),

@@ -2,7 +2,13 @@
 Builder (MakeCSR + SquishCSR, src/builder.h:168-312), whose output the CPU
 tests pin against the reference's own builder.  Bit-for-bit equality of
 offsets and neighbours on Kronecker, uniform, directed and hand-made edge
-lists (duplicates, self-loops, empty rows, no edges)."""
+lists (duplicates, self-loops, empty rows, no edges).
+
+The GPU generator (dmm_generate_edgelist_device, Generator::GenerateEL of
+src/generator.h:66-180) must give the host generator's list bit for bit --
+uniform and R-MAT with the ID permutation, at scales 10 to 22 and degrees
+that leave the last 2^18-pair block partial -- and generate + build on the
+GPU (dmm_make_graph_device) the host's graph."""
 import time
 
 import numpy as np
@@ -68,3 +74,30 @@ def test_device_build_scale22_and_search(ctx, dmm, orc, graph):
     s = int(dmm.pick_sources(host, 1)[0])
     parent, depth, _ = ctx.DOBFS(s)
     assert np.array_equal(depth, orc.bfs_depth(og, s))
+
+
+@pytest.mark.parametrize("scale,degree", [(10, 16), (13, 3), (16, 16), (19, 5), (22, 16)])
+@pytest.mark.parametrize("uniform", [False, True])
+def test_device_generator_equals_host(ctx, dmm, scale, degree, uniform):
+    print(f"[generate scale {scale} degree {degree} uniform {uniform}]", flush=True)
+    host = dmm.generate_edgelist(scale, degree, uniform)
+    dev = ctx.generate_edgelist(scale, degree, uniform)
+    assert dev.shape == host.shape
+    assert np.array_equal(dev, host)
+
+
+@pytest.mark.parametrize("scale,uniform", [(12, False), (17, True), (20, False)])
+def test_device_make_graph_equals_host(ctx, dmm, scale, uniform):
+    print(f"[make graph scale {scale} uniform {uniform}]", flush=True)
+    before = ctx.device_bytes()
+    dev = ctx.make_graph(scale, 16, uniform)
+    assert ctx.device_bytes() == before  # the transient buffers are released
+    _same_csr(dmm.build_graph(scale, 16, uniform), dev)
+
+
+def test_device_generator_rejects(ctx):
+    for scale, degree in ((0, 16), (31, 16), (10, 0), (10, 2000)):
+        with pytest.raises(L.DmmError):
+            ctx.make_graph(scale, degree)
+    with pytest.raises(L.DmmError):
+        ctx.generate_edgelist(0, 16)
