@@ -111,6 +111,7 @@ class Tuning(C.Structure):
         ("batch_ms", C.c_double),
         ("pr_window", C.c_int32),
         ("pad", C.c_int32),
+        ("pr_ms", C.c_double),
     ]
 
 

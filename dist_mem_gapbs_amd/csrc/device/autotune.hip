@@ -1,9 +1,14 @@
-// Autotuner: searches the direction-switch thresholds and launch shapes on
-// the ctx's own graph and sample sources, timing the device-resident paths.
-//   plain DOBFS : alpha x beta x workgroups-per-CU
-//   batched BFS : top-down divisor x its own workgroups-per-CU
-// The best values are installed on the ctx.  (The reference fixes alpha =
-// 15, beta = 18 in the DOBFS signature, src/bfs.cc:142.)
+// Autotuner: searches the direction-switch thresholds, launch and tile
+// shapes on the ctx's own graph and sample sources, timing the
+// device-resident paths (device time from HIP events).
+//   plain DOBFS : workgroups per CU, then the top-down chunk (256 / 512 /
+//                 1024 edges), both at the reference thresholds alpha = 15,
+//                 beta = 18 (src/bfs.cc:142); then alpha x beta
+//   batched BFS : top-down divisor x its own workgroups per CU, then the
+//                 levels per captured graph
+//   PageRank    : the compensated mode's tile window (16384 / 32768
+//                 sources), on a degree-ordered whole graph
+// The best values are installed on the ctx.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -52,10 +57,25 @@ extern "C" int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, 
   const int alphas[] = {5, 10, 15, 20, 30, 60};
   const int betas[] = {9, 18, 36, 72};
   const int gpcs[] = {2, 4, 8, 16};
-  // Workgroups per CU first at the reference thresholds, then alpha x beta.
+  const int chunks[] = {256, 512, 1024};
+  // Workgroups per CU, then the top-down chunk, at the reference
+  // thresholds; then alpha x beta.
+  base.alpha = 15;
+  base.beta = 18;
   for (int g : gpcs) {
     dmm_tuning t = base;
     t.grid_per_cu = g;
+    if ((rc = dmm_set_tuning(ctx, &t))) return rc;
+    double ms;
+    if ((rc = time_plain(&ms))) return rc;
+    if (ms < best.plain_ms) {
+      best = t;
+      best.plain_ms = ms;
+    }
+  }
+  for (int c : chunks) {
+    dmm_tuning t = best;
+    t.td_edges_per_block = c;
     if ((rc = dmm_set_tuning(ctx, &t))) return rc;
     double ms;
     if ((rc = time_plain(&ms))) return rc;
@@ -97,9 +117,45 @@ extern "C" int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, 
       }
     }
   }
+  const int levels[] = {2, 4, 8, 16, 32};
+  for (int lv : levels) {
+    dmm_tuning t = best;
+    t.batch.graph_levels = lv;
+    if ((rc = dmm_set_tuning(ctx, &t))) return rc;
+    double ms;
+    if ((rc = time_batch(&ms))) return rc;
+    if (ms < best.batch_ms) {
+      const double pm = best.plain_ms;
+      best = t;
+      best.plain_ms = pm;
+      best.batch_ms = ms;
+    }
+  }
+  // Compensated PageRank tile window (tiled only on a degree-ordered whole
+  // graph): a few iterations per window.
+  best.pr_ms = 0;
+  if (ctx->relabeled && ctx->nloc == ctx->n) {
+    best.pr_ms = 1e30;
+    for (int w : {16384, 32768}) {
+      dmm_tuning t = best;
+      t.pr_window = w;
+      if ((rc = dmm_set_tuning(ctx, &t))) return rc;
+      dmm_pr_stats st;
+      if ((rc = dmm_pr_pull(ctx, 2, 0.0, DMM_PR_FP32_KAHAN, nullptr, &st))) return rc;  // builds the tiles
+      if ((rc = dmm_pr_pull(ctx, 4, 0.0, DMM_PR_FP32_KAHAN, nullptr, &st))) return rc;
+      if (st.iter_ms < best.pr_ms) {
+        const double pm = best.plain_ms, bm = best.batch_ms;
+        best = t;
+        best.plain_ms = pm;
+        best.batch_ms = bm;
+        best.pr_ms = st.iter_ms;
+      }
+    }
+  }
   if ((rc = dmm_set_tuning(ctx, &best))) return rc;
   ctx->tuning.plain_ms = best.plain_ms;
   ctx->tuning.batch_ms = best.batch_ms;
+  ctx->tuning.pr_ms = best.pr_ms;
   if (out) *out = best;
   return DMM_OK;
 }

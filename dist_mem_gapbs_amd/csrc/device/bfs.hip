@@ -50,8 +50,8 @@ constexpr int kPostThreads = 1024;  // the one-workgroup post kernel
 constexpr int kBuProbe = kHeadNbrs;  // in-neighbours probed per round (bottom-up)
 constexpr int kBuMore = 4;           // ... per round past the head array
 static_assert(kHeadNbrs == 4, "the head array is read as int4");
-constexpr int kItems = 4;                 // TD edges per thread per chunk
-constexpr int kEPB = kThreads * kItems;   // TD edges per chunk
+constexpr int kItems = 4;                 // TD edges per thread per chunk, at most
+constexpr int kEPB = kThreads * kItems;   // TD edges per chunk, at most (sizes the LDS staging)
 
 struct StepArgs {
   const int64_t* out_off;
@@ -74,6 +74,7 @@ struct StepArgs {
   int64_t beta_width;                     // Partition::max_width
   int bu_aligned;                         // local block starts/ends on words
   int undirected;                         // in-CSR == out-CSR
+  int td_items;                           // TD edges per thread per chunk (1, 2 or kItems)
 };
 
 struct BfsParams {
@@ -132,7 +133,8 @@ union StepLds {
 
 // ------------------------------------------------------------------------
 // Top-down step: src/bfs.cc:81-100.  Edge-balanced over the window: chunk k
-// covers window edges [k*kEPB, (k+1)*kEPB); the frontier entries it spans
+// covers window edges [k*C, (k+1)*C), C = kThreads * td_items <= kEPB (a
+// tuning value: 256, 512 or 1024 edges); the frontier entries it spans
 // are staged in LDS and each edge finds its source by binary search.  A
 // claim (successful CAS of parent[v] from its negative value to u) sets v's
 // bit in the spare bitmap.  The claimed vertices of out-degree >= 1 are also
@@ -161,8 +163,11 @@ __device__ void td_phase(const StepArgs& a, const StepIo& io, TdLds& L) {
   const int tid = threadIdx.x;
   const long long qin = io.qin;
   int64_t blk_cnt = 0, blk_degs = 0, my_zero = 0;  // blk_* valid in thread 0
-  for (int64_t e0 = int64_t(blockIdx.x) * kEPB; e0 < T; e0 += int64_t(gridDim.x) * kEPB) {
-    const int64_t e1 = min(T, e0 + kEPB);
+  // Chunk length; edges of a chunk are e0 + tid + k * kThreads for k below
+  // td_items, so the unrolled k loop's bound e < e1 drops the other k.
+  const int64_t chunk = int64_t(kThreads) * a.td_items;
+  for (int64_t e0 = int64_t(blockIdx.x) * chunk; e0 < T; e0 += int64_t(gridDim.x) * chunk) {
+    const int64_t e1 = min(T, e0 + chunk);
     // i0 = last i with qoff[i] <= e0 (wave 0); i1 = last i with qoff[i] <=
     // e1 - 1 (wave 1).
     // A window that fits the LDS staging is staged whole.
@@ -178,7 +183,8 @@ __device__ void td_phase(const StepArgs& a, const StepIo& io, TdLds& L) {
     __syncthreads();
     const int64_t i0 = L.ie[0];
     // Queued vertices have out-degree >= 1, so a chunk spans at most
-    // kEPB + 1 entries; a violation is reported instead of overrunning LDS.
+    // chunk + 1 <= kEPB + 1 entries; a violation is reported instead of
+    // overrunning LDS.
     int nv = static_cast<int>(L.ie[1] - i0 + 1);
     if (nv > kEPB + 2) {
       if (tid == 0) *io.error = 1;
@@ -1053,6 +1059,7 @@ StepArgs make_args(dmm_ctx* ctx, bool with_depth = true) {
   a.beta_width = ctx->beta_width;
   a.undirected = ctx->directed ? 0 : 1;
   a.bu_aligned = ((ctx->vbegin & 63) == 0 && ((ctx->vend & 63) == 0 || ctx->vend == ctx->n)) ? 1 : 0;
+  a.td_items = std::max(1, std::min(kItems, ctx->tuning.td_edges_per_block / kThreads));
   return a;
 }
 

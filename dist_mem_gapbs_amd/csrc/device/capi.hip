@@ -1079,6 +1079,10 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
     return set_err(ctx, DMM_ERR_INVALID, "tuning values must be positive");
   if (t->pr_window != 0 && t->pr_window != 16384 && t->pr_window != 32768)
     return set_err(ctx, DMM_ERR_INVALID, "pr_window must be 16384 or 32768");
+  if (t->td_edges_per_block != 0 && t->td_edges_per_block != 256 && t->td_edges_per_block != 512 &&
+      t->td_edges_per_block != 1024)
+    return set_err(ctx, DMM_ERR_INVALID, "td_edges_per_block must be 256, 512 or 1024");
+  if (t->batch.graph_levels > 64) return set_err(ctx, DMM_ERR_INVALID, "graph_levels above 64");
   if (t->pr_window != 0 && t->pr_window != ctx->tuning.pr_window) {
     ctx->tuning.pr_window = t->pr_window;
     drop_tiles(ctx);  // rebuilt with the new window at the next compensated call
@@ -1088,6 +1092,7 @@ int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t) {
   ctx->tuning.beta = t->beta;
   ctx->tuning.batch = t->batch;
   if (t->grid_per_cu > 0) ctx->tuning.grid_per_cu = std::min(32, t->grid_per_cu);
+  if (t->td_edges_per_block > 0) ctx->tuning.td_edges_per_block = t->td_edges_per_block;
   // Graphs captured with the old shape are rebuilt on next use.
   drop_graphs(ctx);
   return DMM_OK;
@@ -1104,6 +1109,7 @@ int dmm_get_tuning(const dmm_ctx* ctx, dmm_tuning* t) {
   t->plain_ms = ctx->tuning.plain_ms;
   t->batch_ms = ctx->tuning.batch_ms;
   t->pr_window = ctx->tuning.pr_window;
+  t->pr_ms = ctx->tuning.pr_ms;
   return DMM_OK;
 }
 

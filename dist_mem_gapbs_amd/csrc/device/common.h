@@ -102,7 +102,8 @@ struct Tuning {
   int td_edges_per_block = 1024;
   int grid_per_cu = 8;                 // workgroups per CU for grid-stride steps
   dmm_batch_params batch = {20, 0, 16, 0};  // grid_per_cu 0: the one above
-  int pr_window = 32768;               // sources per PageRank tile window
+  int pr_window = 32768;
+  double pr_ms = 0;  // measured compensated PageRank iteration time (autotuner)               // sources per PageRank tile window
   double plain_ms = 0, batch_ms = 0;
 };
 

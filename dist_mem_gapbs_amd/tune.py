@@ -3,10 +3,14 @@ benchmark uses and writes dist_mem_gapbs_amd/tuning.json.
 
 For each graph (key "<family>-<scale>-<degree>", family kron or uniform) the
 library searches, on the GPU:
-  plain DOBFS   workgroups per CU at the reference thresholds (alpha 15,
-                beta 18, src/bfs.cc:142), then alpha x beta;
-  batched BFS   the top-down divisor x the batch kernels' workgroups per CU.
-bench.py installs the launch shapes and the batch settings from this file;
+  plain DOBFS   workgroups per CU and the top-down chunk at the reference
+                thresholds (alpha 15, beta 18, src/bfs.cc:142), then
+                alpha x beta;
+  batched BFS   the top-down divisor x the batch kernels' workgroups per CU,
+                then the levels per captured graph;
+  PageRank      the compensated mode's tile window.
+bench.py installs the launch and tile shapes and the batch settings from
+this file;
 the plain DOBFS keeps alpha 15 / beta 18 as its workload specifies, and the
 tuned alpha / beta are recorded for reference.
 
@@ -32,8 +36,9 @@ def tune_one(dmm, key, budget):
         t = ctx.autotune(srcs, budget)
     g.free()
     return {"alpha": t.alpha, "beta": t.beta, "grid_per_cu": t.grid_per_cu,
-            "td_div": t.batch.td_div, "batch_grid_per_cu": t.batch.grid_per_cu,
-            "graph_levels": t.batch.graph_levels, "plain_ms": t.plain_ms, "batch_ms": t.batch_ms,
+            "td_edges_per_block": t.td_edges_per_block, "td_div": t.batch.td_div,
+            "batch_grid_per_cu": t.batch.grid_per_cu, "graph_levels": t.batch.graph_levels,
+            "pr_window": t.pr_window, "plain_ms": t.plain_ms, "batch_ms": t.batch_ms, "pr_ms": t.pr_ms,
             "search_s": time.time() - t0}
 
 

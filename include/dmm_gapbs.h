@@ -431,17 +431,24 @@ typedef struct dmm_tuning {
   int32_t alpha;
   int32_t beta;
   int32_t grid_per_cu;     /* workgroups per CU of the grid-stride steps   */
-  int32_t td_edges_per_block; /* top-down edges per workgroup chunk (fixed) */
+  int32_t td_edges_per_block; /* top-down edges per workgroup chunk: 256,
+                              512 or 1024 (0 = keep)                       */
   dmm_batch_params batch;
   double plain_ms;         /* measured mean DOBFS time with these values   */
   double batch_ms;         /* measured batch time with these values        */
   int32_t pr_window;       /* compensated PageRank tile window: sources per
                               LDS copy (16384 or 32768; 0 = keep)         */
   int32_t pad;
+  double pr_ms;            /* measured compensated PageRank iteration time */
 } dmm_tuning;
 
-/* Searches alpha/beta and launch shapes on this ctx's graph using the given
- * sample sources; writes the best into *out and installs it on the ctx. */
+/* Searches the direction thresholds, launch and tile shapes on this ctx's
+ * graph using the given sample sources: plain DOBFS workgroups per CU and
+ * top-down chunk at the reference's alpha = 15 / beta = 18, then alpha x
+ * beta; batched BFS top-down divisor x workgroups per CU, then levels per
+ * captured graph; on a degree-ordered whole graph the compensated
+ * PageRank's tile window.  Writes the best into *out and installs it on the
+ * ctx.  No reference counterpart (src/bfs.cc:142 fixes alpha and beta). */
 int dmm_autotune(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc,
                  int32_t budget, dmm_tuning* out);
 int dmm_set_tuning(dmm_ctx* ctx, const dmm_tuning* t);
