@@ -35,6 +35,7 @@ DMM_LAYOUT_DEGREE = 1
 DMM_PR_FP32_SERIAL = 0
 DMM_PR_FP32_KAHAN = 1
 DMM_PR_FP64 = 2
+DMM_BATCH_DEPTH_HBM = 1
 
 
 class BfsStats(C.Structure):
@@ -63,7 +64,7 @@ class BatchParams(C.Structure):
         ("td_div", C.c_int32),
         ("grid_per_cu", C.c_int32),
         ("graph_levels", C.c_int32),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
 
@@ -97,6 +98,9 @@ class BlockLevelStats(C.Structure):
         ("kernel_ms", C.c_double),
         ("edges_examined", C.c_int64),
         ("alg_bytes", C.c_int64),
+        ("kind", C.c_int32),
+        ("pad", C.c_int32),
+        ("frontier", C.c_int64),
     ]
 
 

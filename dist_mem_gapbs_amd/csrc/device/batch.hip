@@ -30,6 +30,8 @@
 // captured in a hipGraph.
 #include "batch.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <vector>
 
@@ -620,6 +622,16 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   return a;
 }
 
+// Direction and frontier list of a PE-block level (see k_front_scan).
+struct BlockCtrl {
+  unsigned long long count;  // frontier vertices listed
+  unsigned long long heavy;  // ... of them with a push list longer than kPushSplit
+  unsigned long long work;   // push-list entries of the frontier
+  int kind;                  // DMM_STEP_TD or DMM_STEP_BU
+  int pad;
+};
+constexpr int64_t kPushSplit = 4096;  // longer push lists are shared by the whole grid
+
 // One bottom-up level over a PE block against the all-gathered frontier
 // words of every vertex (front, global IDs): the sweep of bu_phase with the
 // block's own seen words (lanes reached at the current level or before).
@@ -630,7 +642,7 @@ constexpr int64_t kBlockSplit = 512;
 struct BlockSeg {
   int64_t row, b, e;
 };
-__global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restrict__ in_off,
+__global__ __launch_bounds__(kThreads) void k_block_sweep(const BlockCtrl* ctrl, const int64_t* __restrict__ in_off,
                                                          const int32_t* __restrict__ in_nbr, int64_t nloc,
                                                          const uint64_t* __restrict__ front,
                                                          const uint64_t* __restrict__ fnz,
@@ -638,6 +650,10 @@ __global__ __launch_bounds__(kThreads) void k_block_sweep(const int64_t* __restr
                                                          const uint64_t* __restrict__ seen,
                                                          uint64_t* __restrict__ next, int64_t* part) {
   __shared__ int64_t red[kThreads / 64];
+  if (ctrl->kind != DMM_STEP_BU) {
+    if (threadIdx.x == 0) part[blockIdx.x] = 0;
+    return;
+  }
   const int sub = threadIdx.x & (kGroup - 1);
   const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
@@ -674,14 +690,18 @@ __global__ __launch_bounds__(kThreads) void k_nz_build(const uint64_t* __restric
 
 // Segments of the long rows: the found lanes are ORed into next[row]
 // (cleared by k_block_sweep); partials at part[blockIdx.x].
-__global__ __launch_bounds__(kThreads) void k_block_segs(const BlockSeg* __restrict__ segs, int64_t nsegs,
-                                                        const int32_t* __restrict__ in_nbr,
+__global__ __launch_bounds__(kThreads) void k_block_segs(const BlockCtrl* ctrl, const BlockSeg* __restrict__ segs,
+                                                        int64_t nsegs, const int32_t* __restrict__ in_nbr,
                                                         const uint64_t* __restrict__ front,
                                                         const uint64_t* __restrict__ fnz,
                                                         const uint64_t* __restrict__ fnz2,
                                                         const uint64_t* __restrict__ seen,
                                                         uint64_t* __restrict__ next, int64_t* part) {
   __shared__ int64_t red[kThreads / 64];
+  if (ctrl->kind != DMM_STEP_BU) {
+    if (threadIdx.x == 0) part[blockIdx.x] = 0;
+    return;
+  }
   const int sub = threadIdx.x & (kGroup - 1);
   const int64_t grp = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / kGroup;
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
@@ -718,6 +738,136 @@ __global__ __launch_bounds__(kThreads) void k_level_out(const uint64_t* __restri
   for (int64_t v = gtid; v < n; v += gsize) out[v] = lw[perm ? perm[v] : v];
 }
 
+// ---- top-down block level -------------------------------------------------
+//
+// A PE holds the in-lists of its own rows only, so it expands a sparse
+// all-gathered frontier owner-computes through the transpose of its block:
+// for every global vertex u, the block rows whose in-list holds u (the push
+// index, built once per upload by a radix sort of (u, row) keys).  A level
+// lists the frontier vertices (and the push work they carry), then one
+// kernel decides, as the whole-graph batched BFS does, between the
+// top-down expansion (work below the block's in-edges / td_div) and the
+// bottom-up sweep; the kernels of the other direction return at once.
+
+
+__global__ __launch_bounds__(kThreads) void k_push_keys(const int64_t* __restrict__ in_off,
+                                                       const int32_t* __restrict__ in_nbr, int64_t nloc,
+                                                       uint64_t* __restrict__ keys) {
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  for (int64_t r = wave; r < nloc; r += nwaves)
+    for (int64_t e = in_off[r] + lane; e < in_off[r + 1]; e += 64)
+      keys[e] = (static_cast<uint64_t>(static_cast<uint32_t>(in_nbr[e])) << 32) | static_cast<uint64_t>(r);
+}
+
+// poff[u] = first sorted key of source >= u, for u in [0, n]; targets are
+// the keys' low halves.
+__global__ __launch_bounds__(kThreads) void k_push_index(const uint64_t* __restrict__ keys, int64_t m, int64_t n,
+                                                        int64_t* __restrict__ poff, int32_t* __restrict__ ptgt) {
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i <= m; i += gsize) {
+    const int64_t r0 = i == 0 ? -1 : int64_t(keys[i - 1] >> 32);
+    const int64_t r1 = i == m ? n : int64_t(keys[i] >> 32);
+    for (int64_t r = r0 + 1; r <= r1; r++) poff[r] = i;
+    if (i < m) ptgt[i] = static_cast<int32_t>(static_cast<uint32_t>(keys[i]));
+  }
+}
+
+__global__ void k_block_ctrl_reset(BlockCtrl* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    c->count = 0;
+    c->heavy = 0;
+    c->work = 0;
+    c->kind = DMM_STEP_BU;
+    c->pad = 0;
+  }
+}
+
+// One wavefront per 64 frontier words: the non-zero bitmap (for the
+// bottom-up sweep), the frontier list (light and heavy push lists apart;
+// the heavy ones fill `list` from its end) and the push work.
+__global__ __launch_bounds__(kThreads) void k_front_scan(const uint64_t* __restrict__ front, int64_t n,
+                                                        const int64_t* __restrict__ poff, uint64_t* __restrict__ nz,
+                                                        int32_t* __restrict__ list, BlockCtrl* ctrl) {
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  const int64_t nw = (n + 63) >> 6;
+  unsigned long long work = 0;
+  for (int64_t c = wave; c < nw; c += nwaves) {
+    const int64_t v = c * 64 + lane;
+    const bool on = v < n && front[v] != 0;
+    const uint64_t m = __ballot(on);
+    if (lane == 0) nz[c] = m;
+    if (!m) continue;
+    const int64_t d = on ? poff[v + 1] - poff[v] : 0;
+    const bool heavy = d > kPushSplit;
+    const uint64_t mh = __ballot(heavy);
+    const uint64_t ml = m & ~mh;
+    unsigned long long bl = 0, bh = 0;
+    if (lane == 0) {
+      if (ml) bl = atomicAdd(&ctrl->count, static_cast<unsigned long long>(__popcll(ml)));
+      if (mh) bh = atomicAdd(&ctrl->heavy, static_cast<unsigned long long>(__popcll(mh)));
+    }
+    bl = __shfl(bl, 0);
+    bh = __shfl(bh, 0);
+    if (on && !heavy) list[bl + lane_rank(ml)] = static_cast<int32_t>(v);
+    if (heavy) list[n - 1 - static_cast<int64_t>(bh + lane_rank(mh))] = static_cast<int32_t>(v);
+    work += static_cast<unsigned long long>(d);
+  }
+  for (int o = 32; o > 0; o >>= 1) work += __shfl_xor(work, o);
+  if (lane == 0 && work) atomicAdd(&ctrl->work, work);
+}
+
+__global__ void k_block_decide(BlockCtrl* ctrl, int64_t m_block, int td_div) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    ctrl->kind = static_cast<int64_t>(ctrl->work) * td_div < m_block ? DMM_STEP_TD : DMM_STEP_BU;
+}
+
+// Top-down: clears next, then ORs every frontier word into the push-list
+// rows that have not seen those lanes.  Light lists: one wavefront per
+// frontier vertex; heavy lists: the whole grid strides over each.
+__global__ __launch_bounds__(kThreads) void k_block_clear(const BlockCtrl* ctrl, int64_t nloc,
+                                                         uint64_t* __restrict__ next) {
+  if (ctrl->kind != DMM_STEP_TD) return;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t r = int64_t(blockIdx.x) * kThreads + threadIdx.x; r < nloc; r += gsize) next[r] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_block_push(const BlockCtrl* ctrl, const int32_t* __restrict__ list,
+                                                        int64_t n, const int64_t* __restrict__ poff,
+                                                        const int32_t* __restrict__ ptgt,
+                                                        const uint64_t* __restrict__ front,
+                                                        const uint64_t* __restrict__ seen,
+                                                        uint64_t* __restrict__ next) {
+  if (ctrl->kind != DMM_STEP_TD) return;
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
+  const int64_t nl = static_cast<int64_t>(ctrl->count), nh = static_cast<int64_t>(ctrl->heavy);
+  for (int64_t i = wave; i < nl; i += nwaves) {
+    const int32_t u = list[i];
+    const uint64_t fw = front[u];
+    for (int64_t e = poff[u] + lane; e < poff[u + 1]; e += 64) {
+      const int32_t r = ptgt[e];
+      const uint64_t bits = fw & ~seen[r];
+      if (bits) atomicOr(reinterpret_cast<unsigned long long*>(&next[r]), static_cast<unsigned long long>(bits));
+    }
+  }
+  const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t gsize = int64_t(gridDim.x) * kThreads;
+  for (int64_t h = 0; h < nh; h++) {
+    const int32_t u = list[n - 1 - h];
+    const uint64_t fw = front[u];
+    for (int64_t e = poff[u] + gtid; e < poff[u + 1]; e += gsize) {
+      const int32_t r = ptgt[e];
+      const uint64_t bits = fw & ~seen[r];
+      if (bits) atomicOr(reinterpret_cast<unsigned long long*>(&next[r]), static_cast<unsigned long long>(bits));
+    }
+  }
+}
+
 int ensure_levels(dmm_ctx* ctx, int64_t cap) {
   if (ctx->d_levels && ctx->level_cap >= cap) return DMM_OK;
   if (ctx->d_levels) dev_free(ctx->d_levels);
@@ -747,6 +897,16 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
   if (p.graph_levels <= 0) p.graph_levels = ctx->tuning.batch.graph_levels;
   int rc = ensure_levels(ctx, std::max<int64_t>(ctx->level_cap, 32));
   if (rc) return rc;
+  const bool want_depth = depth || (p.flags & DMM_BATCH_DEPTH_HBM);
+  ctx->bdepth_nsrc = 0;
+  if (want_depth && (!ctx->d_bdepth || ctx->bdepth_cap < int64_t(nsrc) * ctx->nloc)) {
+    if (ctx->d_bdepth) dev_free(ctx->d_bdepth);
+    ctx->d_bdepth = nullptr;
+    ctx->bdepth_cap = 0;
+    rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bdepth), int64_t(nsrc) * ctx->nloc, "batch depth");
+    if (rc) return rc;
+    ctx->bdepth_cap = int64_t(nsrc) * ctx->nloc;
+  }
   // Sources to the device (pinned staging).
   std::copy(sources, sources + nsrc, static_cast<int32_t*>(ctx->h_bsrc));
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_bsrc, ctx->h_bsrc, sizeof(int32_t) * 64, hipMemcpyHostToDevice,
@@ -811,6 +971,17 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
       if (rc) return rc;
       continue;
     }
+    if (want_depth) {
+      hipLaunchKernelGGL(k_bdepth, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, ctx->d_bdepth, nsrc,
+                         ctx->relabeled ? static_cast<const int32_t*>(ctx->d_perm) : nullptr);
+      DMM_HIP_CHECK(ctx, hipGetLastError());
+      DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
+      if (depth)
+        DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_bdepth, int64_t(nsrc) * ctx->nloc, hipMemcpyDeviceToHost,
+                                          ctx->stream));
+      DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      ctx->bdepth_nsrc = nsrc;
+    }
     float ms = 0;
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
     if (stats) {
@@ -829,21 +1000,6 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
           h->sweeps * (40 * ctx->nloc + 8) + 12 * static_cast<int64_t>(h->sweep_scanned_total);
       stats->sweep_ms = 0;
     }
-    if (depth) {
-      if (!ctx->d_bdepth || ctx->bdepth_cap < int64_t(nsrc) * ctx->nloc) {
-        if (ctx->d_bdepth) dev_free(ctx->d_bdepth);
-        ctx->d_bdepth = nullptr;
-        rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bdepth), int64_t(nsrc) * ctx->nloc, "batch depth");
-        if (rc) return rc;
-        ctx->bdepth_cap = int64_t(nsrc) * ctx->nloc;
-      }
-      hipLaunchKernelGGL(k_bdepth, dim3(blocks), dim3(kThreads), 0, ctx->stream, a, ctx->d_bdepth, nsrc,
-                         ctx->relabeled ? static_cast<const int32_t*>(ctx->d_perm) : nullptr);
-      DMM_HIP_CHECK(ctx, hipGetLastError());
-      DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_bdepth, int64_t(nsrc) * ctx->nloc, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-      DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    }
     ctx->batch_valid = true;
     ctx->batch_last_level = h->level;
     if (stats) {
@@ -858,6 +1014,15 @@ int batch_run(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc, uint8_t* depth
     return DMM_OK;
   }
   return set_err(ctx, DMM_ERR_CAPACITY, "batched BFS level storage could not be grown");
+}
+
+int batch_read_depth(dmm_ctx* ctx, uint8_t* depth) {
+  if (!ctx->batch_valid || ctx->bdepth_nsrc <= 0)
+    return set_err(ctx, DMM_ERR_STATE, "the last batched BFS wrote no depths");
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(depth, ctx->d_bdepth, int64_t(ctx->bdepth_nsrc) * ctx->nloc,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DMM_OK;
 }
 
 int batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words) {
@@ -920,16 +1085,58 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
     ctx->n_bsegs = static_cast<int64_t>(segs.size());
     ctx->bsegs_ready = true;
   }
+  if (!ctx->push_ready) {
+    const int64_t m = ctx->m_in;
+    DevBuf keys_a, keys_b, temp;
+    if ((rc = keys_a.alloc(ctx, sizeof(uint64_t) * std::max<int64_t>(1, m), "push index keys")) ||
+        (rc = keys_b.alloc(ctx, sizeof(uint64_t) * std::max<int64_t>(1, m), "push index keys")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_poff), sizeof(int64_t) * (ctx->n + 1), "push index")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_ptgt), sizeof(int32_t) * std::max<int64_t>(1, m),
+                        "push index")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_flist), sizeof(int32_t) * ctx->n, "frontier list")) ||
+        (rc = dev_alloc(ctx, &ctx->d_blkctrl, sizeof(BlockCtrl), "block level control")))
+      return rc;
+    uint64_t* ka = static_cast<uint64_t*>(keys_a.p);
+    uint64_t* kb = static_cast<uint64_t*>(keys_b.p);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_push_keys, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_in_off, ctx->d_in_nbr,
+                         ctx->nloc, ka);
+      int bits = 1;
+      while ((int64_t(1) << bits) < ctx->n) bits++;
+      size_t sort_bytes = 0;
+      DMM_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, ka, kb, m, 0, 32 + bits,
+                                                           ctx->stream));
+      if ((rc = temp.alloc(ctx, std::max<size_t>(1, sort_bytes), "push index sort"))) return rc;
+      DMM_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortKeys(temp.p, sort_bytes, ka, kb, m, 0, 32 + bits,
+                                                           ctx->stream));
+    }
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(65536, (m + kThreads) / kThreads));
+    hipLaunchKernelGGL(k_push_index, dim3(static_cast<unsigned>(grid)), dim3(kThreads), 0, ctx->stream, kb, m, ctx->n,
+                       ctx->d_poff, ctx->d_ptgt);
+    DMM_HIP_CHECK(ctx, hipGetLastError());
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // before the keys are freed
+    ctx->push_ready = true;
+  }
+  BlockCtrl* bc = static_cast<BlockCtrl*>(ctx->d_blkctrl);
   // The timed region starts after the host copies have landed.
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
   const int64_t nzw = (ctx->n + 63) / 64;
-  hipLaunchKernelGGL(k_nz_build, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gfront, ctx->n, ctx->d_gnz);
+  hipLaunchKernelGGL(k_block_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, bc);
+  hipLaunchKernelGGL(k_front_scan, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gfront, ctx->n, ctx->d_poff,
+                     ctx->d_gnz, ctx->d_flist, bc);
+  hipLaunchKernelGGL(k_block_decide, dim3(1), dim3(64), 0, ctx->stream, bc, std::max<int64_t>(1, ctx->m_in),
+                     std::max(1, ctx->tuning.batch.td_div));
+  // Top-down (returns at once on a bottom-up level).
+  hipLaunchKernelGGL(k_block_clear, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->nloc, ctx->d_bnext);
+  hipLaunchKernelGGL(k_block_push, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->d_flist, ctx->n,
+                     ctx->d_poff, ctx->d_ptgt, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext);
+  // Bottom-up (returns at once on a top-down level).
   hipLaunchKernelGGL(k_nz_build, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gnz, nzw, ctx->d_gnz + nzw);
-  hipLaunchKernelGGL(k_block_sweep, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_in_off, ctx->d_in_nbr,
+  hipLaunchKernelGGL(k_block_sweep, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->d_in_off, ctx->d_in_nbr,
                      ctx->nloc, ctx->d_gfront, ctx->d_gnz, ctx->d_gnz + nzw, ctx->d_bseen, ctx->d_bnext,
                      ctx->d_part);
-  hipLaunchKernelGGL(k_block_segs, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+  hipLaunchKernelGGL(k_block_segs, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc,
                      static_cast<const BlockSeg*>(ctx->d_bsegs), ctx->n_bsegs, ctx->d_in_nbr, ctx->d_gfront,
                      ctx->d_gnz, ctx->d_gnz + nzw, ctx->d_bseen, ctx->d_bnext, ctx->d_part + blocks);
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_b, ctx->stream));
@@ -937,17 +1144,31 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
                      ctx->d_scratch64);
   DMM_HIP_CHECK(ctx, hipGetLastError());
   int64_t scanned = 0;
+  BlockCtrl hc;
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(next, ctx->d_bnext, sizeof(uint64_t) * ctx->nloc, hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipMemcpyAsync(&scanned, ctx->d_scratch64, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  DMM_HIP_CHECK(ctx, hipMemcpyAsync(&hc, bc, sizeof(BlockCtrl), hipMemcpyDeviceToHost, ctx->stream));
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (stats) {
     float ms = 0;
     DMM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
     stats->kernel_ms = ms;
-    stats->edges_examined = scanned;
-    // seen read, offsets, next write (24 B per block vertex) + 12 B per
-    // examined in-edge (4 B ID + 8 B frontier word).
-    stats->alg_bytes = 24 * ctx->nloc + 8 + 12 * scanned;
+    stats->kind = hc.kind;
+    stats->pad = 0;
+    stats->frontier = static_cast<int64_t>(hc.count + hc.heavy);
+    // Both: the frontier words (8 B per vertex) read by the scan.
+    // Top-down: next cleared (8 B per block vertex); per frontier vertex its
+    // list entry, word and push offsets (28 B); per push entry the target,
+    // its seen word and the OR into next (20 B).  Bottom-up: seen read,
+    // offsets, next write (24 B per block vertex) + 12 B per examined
+    // in-edge (4 B ID + 8 B frontier word).
+    if (hc.kind == DMM_STEP_TD) {
+      stats->edges_examined = static_cast<int64_t>(hc.work);
+      stats->alg_bytes = 8 * ctx->n + 8 * ctx->nloc + 28 * stats->frontier + 20 * stats->edges_examined;
+    } else {
+      stats->edges_examined = scanned;
+      stats->alg_bytes = 8 * ctx->n + 24 * ctx->nloc + 8 + 12 * scanned;
+    }
   }
   return DMM_OK;
 }

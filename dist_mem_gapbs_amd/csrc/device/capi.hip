@@ -115,7 +115,7 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_bdepth, ctx->d_items, ctx->d_supers, ctx->d_partial, ctx->d_perm, ctx->d_inv,
                   ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
-                  ctx->d_bsegs, ctx->d_head,
+                  ctx->d_bsegs, ctx->d_head, ctx->d_poff, ctx->d_ptgt, ctx->d_flist, ctx->d_blkctrl,
                   ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
                   ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp};
   for (void* p : bufs) dmm::dev_free(p);
@@ -153,12 +153,18 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bsegs = nullptr;
   ctx->n_bsegs = 0;
   ctx->bsegs_ready = false;
+  ctx->d_poff = nullptr;
+  ctx->d_ptgt = nullptr;
+  ctx->d_flist = nullptr;
+  ctx->d_blkctrl = nullptr;
+  ctx->push_ready = false;
   ctx->relabeled = false;
   ctx->h_perm.clear();
   ctx->h_perm.shrink_to_fit();
   ctx->depth_valid = false;
   ctx->level_cap = 0;
   ctx->bdepth_cap = 0;
+  ctx->bdepth_nsrc = 0;
   ctx->has_graph = false;
   ctx->bfs_valid = false;
   ctx->dev_bytes = ctx->fixed_bytes;
@@ -1003,6 +1009,13 @@ int dmm_batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words) {
   return dmm::batch_read_level(ctx, level, words);
 }
 
+int dmm_batch_read_depth(dmm_ctx* ctx, uint8_t* depth) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  if (!depth) return set_err(ctx, DMM_ERR_INVALID, "depth is NULL");
+  return dmm::batch_read_depth(ctx, depth);
+}
+
 int dmm_batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen, uint64_t* next,
                           dmm_block_level_stats* stats) {
   int rc = need_graph(ctx);
@@ -1053,6 +1066,8 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib, float* sco
   if (!global_contrib || !scores_inout) return set_err(ctx, DMM_ERR_INVALID, "NULL buffer");
   if (mode != DMM_PR_FP32_SERIAL && mode != DMM_PR_FP32_KAHAN)
     return set_err(ctx, DMM_ERR_INVALID, "block iteration supports the FP32 modes");
+  if (ctx->relabeled)
+    return set_err(ctx, DMM_ERR_STATE, "block iteration needs DMM_LAYOUT_NATURAL (contributions in the caller's IDs)");
   if ((rc = ensure_pr_state(ctx, mode))) return rc;
   return dmm::pr_block_iteration(ctx, global_contrib, scores_inout, mode, err, ms);
 }

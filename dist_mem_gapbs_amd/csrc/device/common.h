@@ -253,6 +253,7 @@ struct dmm_ctx {
   void* h_bsrc = nullptr;
   uint8_t* d_bdepth = nullptr;
   int64_t bdepth_cap = 0;
+  int32_t bdepth_nsrc = 0;          // sources of the depths held at d_bdepth (0: none)
   hipGraphExec_t batch_graph = nullptr;
   hipGraphExec_t batch_cont = nullptr;
   bool batch_valid = false;         // a finished run's level words are held
@@ -265,6 +266,11 @@ struct dmm_ctx {
   void* d_bsegs = nullptr;          // segments of the block's long rows
   int64_t n_bsegs = 0;
   bool bsegs_ready = false;
+  int64_t* d_poff = nullptr;        // push index of the block: n + 1 offsets
+  int32_t* d_ptgt = nullptr;        // ... block rows per global source
+  int32_t* d_flist = nullptr;       // frontier list (n entries)
+  void* d_blkctrl = nullptr;        // block level control
+  bool push_ready = false;
   int64_t batch_graph_key = -1;
 
   dmm::Tuning tuning;

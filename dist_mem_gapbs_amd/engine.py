@@ -209,24 +209,34 @@ class Context:
         self._check(self._lib.dmm_state_set_level(self._h, level))
 
     # ---- batched BFS ---------------------------------------------------------------
-    def bfs_batch64(self, sources, want_depth=True, td_div=None, graph_levels=None):
-        """Returns (depth[nsrc, nloc] uint8 with 255 = unreached, stats)."""
+    def bfs_batch64(self, sources, want_depth=True, td_div=None, graph_levels=None, depth_in_hbm=False,
+                    with_stats=True):
+        """Returns (depth[nsrc, nloc] uint8 with 255 = unreached, stats).
+
+        depth_in_hbm with want_depth False leaves the depths in HBM
+        (batch_read_depth reads them); with_stats False returns stats None."""
         src = L.i32(sources)
         ns = len(src)
         depth = np.empty((ns, self.nloc), np.uint8) if want_depth else None
         params = None
-        if td_div is not None or graph_levels is not None:
+        if td_div is not None or graph_levels is not None or depth_in_hbm:
             t = self.get_tuning()
             params = L.BatchParams(td_div if td_div is not None else t.batch.td_div,
                                    t.batch.grid_per_cu,
                                    graph_levels if graph_levels is not None else t.batch.graph_levels,
-                                   0)
-        st = L.BatchStats()
+                                   L.DMM_BATCH_DEPTH_HBM if depth_in_hbm else 0)
+        st = L.BatchStats() if with_stats else None
         self._check(self._lib.dmm_bfs_batch64(self._h, L.ptr(src, C.c_int32), ns,
                                               L.ptr(depth, C.c_uint8),
                                               C.byref(params) if params is not None else None,
-                                              C.byref(st)))
+                                              C.byref(st) if st is not None else None))
         return depth, st
+
+    def batch_read_depth(self, nsrc):
+        """Depths (uint8[nsrc, nloc]) of the last bfs_batch64 run that wrote them."""
+        d = np.empty((nsrc, self.nloc), np.uint8)
+        self._check(self._lib.dmm_batch_read_depth(self._h, L.ptr(d, C.c_uint8)))
+        return d
 
     def batch_read_level(self, level):
         """Level words (uint64[nloc], caller's IDs) of the last bfs_batch64 run."""

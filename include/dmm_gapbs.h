@@ -310,14 +310,20 @@ typedef struct dmm_batch_params {
   int32_t grid_per_cu;   /* workgroups per CU of the batched kernels
                             (<= 0: the ctx's grid_per_cu)                 */
   int32_t graph_levels;  /* levels per captured hipGraph chunk             */
-  int32_t reserved;
+  int32_t flags;         /* DMM_BATCH_* bits                               */
 } dmm_batch_params;
+
+/* dmm_batch_params.flags: write the caller-ID depths to HBM even when the
+ * depth argument is NULL (dmm_batch_read_depth copies them out later). */
+#define DMM_BATCH_DEPTH_HBM 1
 
 typedef struct dmm_batch_stats {
   int32_t num_levels;
   int32_t level_kind[DMM_MAX_STEPS];     /* DMM_STEP_TD / DMM_STEP_BU     */
   int64_t level_active[DMM_MAX_STEPS];   /* vertices with new lanes       */
-  double total_ms;                       /* device time of the batch      */
+  double total_ms;                       /* device time of the batch, through
+                                            the depth kernel when depths
+                                            are written                   */
   double sweep_ms;                       /* ... of the bottom-up sweeps   */
   int64_t sweep_alg_bytes;               /* algorithmic bytes of sweeps   */
   int32_t sweeps;                        /* bottom-up sweeps run          */
@@ -340,17 +346,35 @@ int dmm_bfs_batch64(dmm_ctx* ctx, const int32_t* sources, int32_t nsrc,
  * PE-share runs (the all-gathered frontier of dmm_batch_block_level). */
 int dmm_batch_read_level(dmm_ctx* ctx, int32_t level, uint64_t* words);
 
+/* Depths of the last dmm_bfs_batch64 run that wrote them (a depth buffer or
+ * DMM_BATCH_DEPTH_HBM): nsrc x (vend - vbegin) uint8, source-major, in the
+ * caller's IDs, 255 = unreached.  No reference counterpart: the read-out
+ * half of a run that keeps its result in HBM. */
+int dmm_batch_read_depth(dmm_ctx* ctx, uint8_t* depth);
+
 typedef struct dmm_block_level_stats {
-  double kernel_ms;        /* device time of the level: frontier bitmap
-                              build + sweep                               */
-  int64_t edges_examined;  /* in-edges read before every lane was found   */
-  int64_t alg_bytes;       /* 24 B per block vertex + 12 B per examined edge */
+  double kernel_ms;        /* device time of the level: frontier scan,
+                              direction choice and the expansion or sweep */
+  int64_t edges_examined;  /* top-down: push entries of the frontier;
+                              bottom-up: in-edges read before every lane
+                              was found                                   */
+  int64_t alg_bytes;       /* algorithmic bytes of the level              */
+  int32_t kind;            /* DMM_STEP_TD or DMM_STEP_BU                   */
+  int32_t pad;
+  int64_t frontier;        /* vertices with a frontier lane               */
 } dmm_block_level_stats;
 
-/* One bottom-up level of the batched BFS over the uploaded PE block: the
- * pull sweep of dmm_bfs_batch64 with the frontier all-gathered from every
- * PE, as one PE of the vertex-block layout (src/partition.h:69-161) would
- * run it (the per-PE BU step of src/bfs.cc:57-78, 64 sources at once).
+/* One level of the batched BFS over the uploaded PE block, with the
+ * frontier all-gathered from every PE, as one PE of the vertex-block layout
+ * (src/partition.h:69-161) would run it, 64 sources at once.  The direction
+ * is chosen per level as dmm_bfs_batch64 chooses it (top-down while the
+ * frontier's push work is below the block's in-edges / td_div):
+ *   top-down   owner-computes expansion (the per-PE TD step of
+ *              src/bfs.cc:81-100): each frontier vertex's lanes go to the
+ *              block rows whose in-list holds it, through the transpose of
+ *              the block built at the first call after an upload;
+ *   bottom-up  the pull sweep of the block's in-lists (the per-PE BU step
+ *              of src/bfs.cc:57-78).
  *   front  uint64[num_nodes]  lanes whose frontier holds v at the current
  *                             level, for every vertex (global IDs)
  *   seen   uint64[rows]       lanes that reached each block vertex at the

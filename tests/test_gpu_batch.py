@@ -31,6 +31,23 @@ def test_batch64_depths(ctx, dmm, orc, graph, scale, degree, uniform):
     assert st.reached_edges == int(sum(deg[expected(orc, og, [s])[0] != 255].sum() for s in srcs))
 
 
+def test_batch64_depths_kept_in_hbm(ctx, dmm, orc, graph):
+    """DMM_BATCH_DEPTH_HBM: the run writes the caller-ID depths to HBM with
+    no host buffer; dmm_batch_read_depth returns them.  A run that writes no
+    depths leaves nothing to read."""
+    g = graph(14, 16, False)
+    og = orc.CSR.of(g)
+    ctx.upload(g)
+    srcs = dmm.pick_sources(g, 64)[:40]
+    none, st = ctx.bfs_batch64(srcs, want_depth=False, depth_in_hbm=True)
+    assert none is None and st.total_ms > 0
+    assert np.array_equal(ctx.batch_read_depth(len(srcs)), expected(orc, og, srcs))
+    _, st = ctx.bfs_batch64(srcs, want_depth=False, with_stats=False)
+    assert st is None
+    with pytest.raises(L.DmmError):
+        ctx.batch_read_depth(len(srcs))
+
+
 @pytest.mark.parametrize("td_div", [1, 4, 20, 10 ** 6])
 def test_batch64_direction_settings_same_depths(ctx, dmm, orc, graph, td_div):
     g = graph(13, 16, False)
@@ -122,33 +139,81 @@ def level_words(depth, level):
 
 def test_block_level_equals_full_graph(ctx, dmm, orc, graph):
     """One PE's block (src/partition.h, np = 8, pe = 3): fed the all-gathered
-    level words of a whole-graph batched run, each bottom-up block level
-    equals the whole-graph run's next level restricted to the block."""
+    level words of the oracle's depths (serial BFS per source), each block
+    level equals the next level's words restricted to the block.  The
+    whole-graph batched run's level words equal the same oracle words.  The
+    sparse first levels take the top-down expansion, the dense middle ones
+    the bottom-up sweep."""
     hg = graph(14, 16, False)
     g = orc.CSR.of(hg)
     n = g.n
     srcs = dmm.pick_sources(hg, 64)
-    ctx.upload(hg)
-    depth, _ = ctx.bfs_batch64(srcs)
-    levels = []
+    odepth = expected(orc, g, srcs)
+    words = []
     while True:
-        try:
-            levels.append(ctx.batch_read_level(len(levels)))
-        except L.DmmError as ex:
-            assert ex.code == L.DMM_ERR_INVALID
+        w = level_words(odepth, len(words))
+        words.append(w)
+        if not w.any():
             break
-    assert len(levels) >= 3
-    for lv, w in enumerate(levels):
-        assert np.array_equal(w, level_words(depth, lv)), lv
-    assert not levels[-1].any()
+    assert len(words) >= 4
+    ctx.upload(hg)
+    ctx.bfs_batch64(srcs, want_depth=False)
+    for lv in range(len(words) - 1):
+        assert np.array_equal(ctx.batch_read_level(lv), words[lv]), lv
     p = dmm.partition(n, 8, 3)
     b, e = p["start"], p["end"]
     ctx.upload(num_nodes=n, out_offsets=g.oo[b:e + 1] - g.oo[b], out_neighs=g.on[g.oo[b]:g.oo[e]],
                vbegin=b, vend=e, num_edges_directed=g.ned)
     seen = np.zeros(e - b, np.uint64)
-    for lv in range(len(levels) - 1):
-        seen |= levels[lv][b:e]
-        nxt, st = ctx.batch_block_level(levels[lv], seen)
-        assert np.array_equal(nxt, levels[lv + 1][b:e]), lv
+    kinds = []
+    for lv in range(len(words) - 1):
+        seen |= words[lv][b:e]
+        nxt, st = ctx.batch_block_level(words[lv], seen)
+        assert np.array_equal(nxt, words[lv + 1][b:e]), lv
         assert st.kernel_ms > 0
+        assert st.frontier == np.count_nonzero(words[lv])
         assert 0 <= st.edges_examined <= g.oo[e] - g.oo[b]
+        kinds.append(st.kind)
+    assert kinds[0] == L.DMM_STEP_TD and L.DMM_STEP_BU in kinds
+
+
+def _or_over_in_neighbours(g, b, e, front):
+    """exp[r - b] = OR of front over the in-neighbours of row r."""
+    off = g.oo[b:e + 1] - g.oo[b]
+    vals = front[g.on[g.oo[b]:g.oo[e]]]
+    exp = np.zeros(e - b, np.uint64)
+    nonempty = np.flatnonzero(np.diff(off) > 0)
+    if len(nonempty):
+        exp[nonempty] = np.bitwise_or.reduceat(vals, off[nonempty])
+    return exp
+
+
+def test_block_level_edge_cases(dmm, orc):
+    """Empty frontier, a frontier on isolated vertices, a hub whose push list
+    is shared by the whole grid, a dense frontier (bottom-up), and blocks at
+    the start, in the middle and past the last edge."""
+    rng = np.random.default_rng(9)
+    n = 40000
+    pairs = [[0, i] for i in range(1, 12000)] + rng.integers(0, 30000, size=(60000, 2)).tolist()
+    hg = csr_from_pairs(dmm, pairs, n)  # vertices 30000 and above are isolated
+    g = orc.CSR.of(hg)
+    ctx = dmm.Context(0)
+    ctx.set_layout(L.DMM_LAYOUT_NATURAL)
+    for (b, e) in ((0, n), (5000, 17000), (35000, n)):
+        ctx.upload(num_nodes=n, out_offsets=g.oo[b:e + 1] - g.oo[b], out_neighs=g.on[g.oo[b]:g.oo[e]],
+                   vbegin=b, vend=e, num_edges_directed=g.ned)
+        kinds = set()
+        for front_ids in ([], [0], [0, 7, 123], list(range(30001, 30100)), list(rng.integers(0, n, 20000))):
+            front = np.zeros(n, np.uint64)
+            for k, v in enumerate(front_ids):
+                front[v] |= np.uint64(1) << np.uint64(k % 64)
+            seen = front[b:e].copy()
+            seen[::7] |= np.uint64(0xF0F0)
+            nxt, st = ctx.batch_block_level(front, seen)
+            exp = _or_over_in_neighbours(g, b, e, front) & ~seen
+            assert np.array_equal(nxt, exp), (b, e, len(front_ids))
+            assert st.frontier == np.count_nonzero(front)
+            kinds.add(st.kind)
+        if b == 0:
+            assert kinds == {L.DMM_STEP_TD, L.DMM_STEP_BU}
+    ctx.close()
