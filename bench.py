@@ -10,23 +10,36 @@ convention: edges of the reached component / 2 per source, summed, over the
 wall-clock time of the K steps).
 
 Also reported on the same line:
-  roofline      the bottom-up step kernel (k_step<2>) -- algorithmic bytes of
-                its launches over their device time (per-step HIP events in
-                trace mode), against 8 TB/s HBM3E
+  roofline      the timed path: one search is one hipGraph launch of the
+                device-resident loop (a chain of k_slot kernels); its
+                algorithmic bytes (InitParent + per-step TD/BU/conversion
+                bytes) over its device time (HIP events on the library's
+                stream), against 8 TB/s HBM3E; bu_step_kernel gives the same
+                figures for the bottom-up step kernel k_step<2> alone (per-step
+                events in trace mode)
   cpu_baseline  the reference's own DOBFS (gapbs-verify/bfs.cc built into
-                oracle/_ref, OpenMP) on a bounded sample of the same sources
-  kron24        Kronecker scale 24: plain DOBFS GTEPS, batched 64-source BFS
-                aggregate GTEPS and its ratio to plain, PageRankPull
-                edges/s/iter and HBM fraction (FP32 compensated, FP32
-                reference-order, FP64), and the FP32 error versus FP64;
-                the same plain/batched pair on a uniform random graph of
-                scale 24.
+                oracle/_ref, OpenMP) on a bounded sample of the same sources;
+                the CPU model and core counts; the reference's distributed
+                DOBFS semantics (src/bfs.cc:57-188, the oracle's SPMD
+                restatement) at np = 1 and np = P threads; the reference's
+                PageRankPull (gapbs-verify/pr.cc) on a bounded number of
+                iterations at Kronecker scale 24
+  kron24        Kronecker scale 24: plain DOBFS and batched 64-source BFS on
+                the same 64 sources, both leaving caller-ID results in HBM
+                (GTEPS over the wall clock and over device time, and their
+                ratios); PageRankPull edges/s/iter and HBM fraction (FP32
+                compensated, FP32 reference-order, FP64) with each mode's
+                max / L1 / relative error against the FP64 oracle
+                (gapbs-verify/pr.cc in double, 20 iterations) and against the
+                GPU's FP64 mode; the same plain/batched pair on a uniform
+                random graph of scale 24; the tuning values that ran.
 Synthetic graphs from the reference generator (no datasets).
 
 Other modes: --only {bfs,batch,pr} runs one Kronecker --big-scale leg (for
 profiling); --only bc times Brandes' forward phase from --steps sources
 beside the oracle's serial forward phase (one core), and --only build the GPU
-CSR build beside the host builder, each checking the results agree;
+graph generation and CSR build beside the host generator and builder, each
+checking the results agree;
 --pe-share runs one PE's share of Kronecker --pe-scale (PE 3 of
 np = 8), timing the block kernels against whole-graph inputs and checking
 every block result against the whole-graph result; --calibrate-copy adds one
@@ -98,13 +111,39 @@ def load_traffic(kernel, workload):
         return None
 
 
+def cpu_info():
+    """CPU model, logical CPUs, and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def omp_threads():
+    """Threads an OpenMP region of the reference kernels runs with."""
+    v = os.environ.get("OMP_NUM_THREADS")
+    if v and v.isdigit() and int(v) > 0:
+        return int(v)
+    return cpu_info()["usable"]
+
+
 def cpu_baseline(og, sources, m_comp, seconds):
     """Times the CPU DOBFS on repeated passes over the sources until about
     `seconds` of work is done (bounded sample of the same workload)."""
     from oracle import oracle as orc
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = omp_threads()
     if orc.ref_available():
-        rg = orc.RefGraph(og)
+        rg = orc.RefGraph(og, pr=False)
         run = rg.dobfs
         kind = "reference"
     else:
@@ -128,7 +167,57 @@ def cpu_baseline(og, sources, m_comp, seconds):
             else "the oracle's scalar DOBFS")
     return {"value": edges / dt / 1e9, "unit": "GTEPS", "cores": cores, "kind": kind,
             "sample": f"{trials} DOBFS trials of {what} cycling over the same {len(sources)} sources "
-                      f"on the same Kronecker-22 graph, {dt:.1f} s"}
+                      f"on the same Kronecker-22 graph, {dt:.1f} s",
+            "cpu": cpu_info()}
+
+
+def cpu_spmd_baseline(og, sources, m_comp, seconds, npes):
+    """The reference's distributed DOBFS semantics (src/bfs.cc:57-188) as the
+    oracle restates them: npes PEs as threads of one process, vertex-block
+    layout, the all-gathered frontier exchanged through shared memory.
+    Graph500 GTEPS over a bounded sample of the same sources."""
+    from oracle import oracle as orc
+    trials, edges, dt = 0, 0, 0.0
+    t0 = time.perf_counter()
+    while dt < seconds and trials < len(sources):
+        s = int(sources[trials])
+        orc.dmm_spmd_bfs(og, s, npes)
+        edges += m_comp[s]
+        trials += 1
+        dt = time.perf_counter() - t0
+    return {"value": edges / dt / 1e9, "unit": "GTEPS", "cores": npes, "kind": "port", "npes": npes,
+            "sample": f"the oracle's SPMD DOBFS (np = {npes}, one thread per PE) from the first {trials} "
+                      f"of the same sources on the same Kronecker-22 graph, {dt:.1f} s"}
+
+
+def cpu_pr_baseline(og, seconds):
+    """The reference's PageRankPull (gapbs-verify/pr.cc:34-59, OpenMP) from
+    oracle/_ref, or the oracle's FP32 restatement on one core: a bounded
+    number of iterations (eps = 0), edges per second per iteration."""
+    from oracle import oracle as orc
+    m = int(og.oo[-1])
+    if orc.ref_available():
+        rg = orc.RefGraph(og, bfs=False)
+        run = lambda k: rg.pagerank(k, 0.0)
+        kind, cores = "reference", omp_threads()
+    else:
+        rg = None
+        run = lambda k: orc.pr_f32(og, k, 0.0)
+        kind, cores = "port", 1
+    with quiet_stdout():
+        t0 = time.perf_counter()
+        run(1)
+        t1 = time.perf_counter() - t0
+        iters = int(max(1, min(20, seconds / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        run(iters)
+        dt = time.perf_counter() - t0
+    if rg is not None:
+        rg.close()
+    what = "the reference PageRankPull (OpenMP)" if kind == "reference" else "the oracle's FP32 PageRankPull"
+    return {"value": m * iters / dt, "unit": "edges/s/iter", "cores": cores, "kind": kind,
+            "iter_ms": 1e3 * dt / iters,
+            "sample": f"{iters} iterations (eps 0) of {what} on the same graph, {dt:.1f} s"}
 
 
 def load_tuning():
@@ -140,18 +229,29 @@ def load_tuning():
 
 
 def apply_tuning(ctx, tuned):
-    """Installs the tuned launch shapes and batch settings (tuning.json);
-    alpha / beta stay at the workload's 15 / 18."""
+    """Installs the tuned launch and tile shapes and batch settings
+    (tuning.json); alpha / beta stay at the workload's 15 / 18."""
     if not tuned:
         return
     t = ctx.get_tuning()
     t.grid_per_cu = tuned.get("grid_per_cu", t.grid_per_cu)
+    t.td_edges_per_block = tuned.get("td_edges_per_block", t.td_edges_per_block)
     t.batch.td_div = tuned.get("td_div", t.batch.td_div)
     t.batch.grid_per_cu = tuned.get("batch_grid_per_cu", t.batch.grid_per_cu)
+    t.batch.graph_levels = tuned.get("graph_levels", t.batch.graph_levels)
+    t.pr_window = tuned.get("pr_window", t.pr_window)
     ctx.set_tuning(t)
 
 
-def kron24_leg(dmm, args, tuned_all):
+def tuning_in_use(ctx):
+    """The launch and batch settings the ctx runs with."""
+    t = ctx.get_tuning()
+    return {"alpha": 15, "beta": 18, "grid_per_cu": t.grid_per_cu, "td_edges_per_block": t.td_edges_per_block,
+            "batch_td_div": t.batch.td_div, "batch_grid_per_cu": t.batch.grid_per_cu,
+            "batch_graph_levels": t.batch.graph_levels, "pr_window": t.pr_window}
+
+
+def kron24_leg(dmm, args, tuned_all, cpu_seconds):
     import dist_mem_gapbs_amd._lib as L
     out = {}
     t0 = time.time()
@@ -162,35 +262,45 @@ def kron24_leg(dmm, args, tuned_all):
     with dmm.Context(0) as ctx:
         ctx.upload(g)
         apply_tuning(ctx, tuned_all.get(f"kron-{args.big_scale}-16"))
-        # plain DOBFS on 16 sources (alpha/beta = 15/18), batched 64 lanes
-        plain, batched, wall, bst = plain_and_batched(ctx, srcs)
-        out["plain_gteps"] = plain
-        out["batched_gteps"] = batched
-        out["batched_over_plain"] = batched / plain
-        out["batched_levels"] = bst.num_levels
-        out["batched_sweep_bytes_per_s"] = (bst.sweep_alg_bytes / (bst.total_ms / 1e3)
-                                            if bst.total_ms > 0 else None)
+        out["tuning"] = tuning_in_use(ctx)
+        # plain DOBFS and batched BFS over the same 64 sources
+        out.update(plain_and_batched(ctx, srcs))
         # PageRank (eps = 0 forces 20 iterations for the timing)
         pr = {}
-        sc64, _ = ctx.PageRankPull(20, 0.0, L.DMM_PR_FP64)
+        scores = {}
         for name, mode in (("fp32_kahan", L.DMM_PR_FP32_KAHAN), ("fp32_reference_order", L.DMM_PR_FP32_SERIAL),
                            ("fp64", L.DMM_PR_FP64)):
             ctx.PageRankPull(20, 0.0, mode, read=False)
             sc, pst = ctx.PageRankPull(20, 0.0, mode)
             m = g.num_edges_directed
-            ent = {"iter_ms": pst.iter_ms, "edges_per_s_per_iter": m / (pst.iter_ms / 1e3),
-                   "alg_bytes_per_iter": pst.alg_bytes_per_iter,
-                   "hbm_frac": pst.alg_bytes_per_iter / (pst.iter_ms / 1e3) / HBM_PEAK}
-            if mode != L.DMM_PR_FP64:
-                d = np.abs(sc.astype(np.float64) - sc64)
-                ent.update({"max_abs_err_vs_fp64": float(d.max()), "l1_err_vs_fp64": float(d.sum()),
-                            "max_rel_err_vs_fp64": float((d / np.maximum(sc64, 1e-300)).max())})
-            pr[name] = ent
+            pr[name] = {"iter_ms": pst.iter_ms, "edges_per_s_per_iter": m / (pst.iter_ms / 1e3),
+                        "alg_bytes_per_iter": pst.alg_bytes_per_iter,
+                        "hbm_frac": pst.alg_bytes_per_iter / (pst.iter_ms / 1e3) / HBM_PEAK}
+            scores[name] = sc
         # convergence run at the reference tolerance
         _, pst = ctx.PageRankPull(20, 1e-4, L.DMM_PR_FP32_KAHAN, read=False)
         pr["iters_at_eps_1e-4"] = pst.iters
-        out["pagerank"] = pr
-    del deg
+    # Errors against the FP64 oracle (gapbs-verify/pr.cc:34-59 in double),
+    # 20 iterations, eps 0; the GPU's own FP64 mode as a second reference.
+    from oracle import oracle as orc
+    og = orc.CSR.of(g)
+    t0 = time.perf_counter()
+    ref64, _, _ = orc.pr_f64_mt(og, 20, 0.0)
+    log(f"FP64 oracle PageRank (20 iterations): {time.perf_counter() - t0:.1f}s")
+
+    def errs(sc, ref):
+        d = np.abs(sc.astype(np.float64) - ref)
+        return {"max_abs": float(d.max()), "l1": float(d.sum()), "max_rel": float((d / np.maximum(ref, 1e-300)).max())}
+
+    for name, sc in scores.items():
+        pr[name]["err_vs_fp64_oracle"] = errs(sc, ref64)
+        if name != "fp64":
+            pr[name]["err_vs_gpu_fp64"] = errs(sc, scores["fp64"])
+    out["pagerank"] = pr
+    if cpu_seconds > 0:
+        out["cpu_pagerank"] = cpu_pr_baseline(og, cpu_seconds)
+        log(f"cpu PageRank baseline: {out['cpu_pagerank']}")
+    del og, scores, ref64, deg
     g.free()
     # Uniform random graph of the same scale (batched-BFS configuration).
     g = dmm.build_graph(args.big_scale, 16, uniform=True)
@@ -198,34 +308,47 @@ def kron24_leg(dmm, args, tuned_all):
     with dmm.Context(0) as ctx:
         ctx.upload(g)
         apply_tuning(ctx, tuned_all.get(f"uniform-{args.big_scale}-16"))
-        plain, batched, _, _ = plain_and_batched(ctx, srcs)
-    out[f"uniform{args.big_scale}"] = {"plain_gteps": plain, "batched_gteps": batched,
-                                      "batched_over_plain": batched / plain}
+        out[f"uniform{args.big_scale}"] = plain_and_batched(ctx, srcs)
+        out[f"uniform{args.big_scale}"]["tuning"] = tuning_in_use(ctx)
     g.free()
     return out
 
 
 def plain_and_batched(ctx, srcs):
-    """Plain DOBFS over 16 sources and batched 64-source BFS (best of 3),
-    aggregate GTEPS over the wall clock."""
-    mc = {}
-    for s in srcs[:16]:
+    """Plain DOBFS and the batched 64-lane BFS on the same 64 sources, each
+    leaving its result in HBM in the caller's vertex IDs (plain: parents,
+    batched: uint8 depths), no PCIe copy in either timed region.  Aggregate
+    Graph500 GTEPS over the wall clock (best of 3 passes each) and over the
+    device time (HIP events around each search / each batch)."""
+    mc, dev_ms = {}, 0.0
+    for s in srcs:
         _, _, st = ctx.DOBFS(int(s), read=False)
         mc[int(s)] = st.reached_edges // 2
-    t0 = time.perf_counter()
-    for s in srcs[:16]:
-        ctx.DOBFS(int(s), read=False, with_stats=False)
-    dt = time.perf_counter() - t0
-    plain = sum(mc[int(s)] for s in srcs[:16]) / dt / 1e9
-    ctx.bfs_batch64(srcs, want_depth=False)
-    best = None
+        dev_ms += st.total_ms
+    edges = sum(mc[int(s)] for s in srcs)
+    plain_wall = None
     for _ in range(3):
         t0 = time.perf_counter()
-        _, bst = ctx.bfs_batch64(srcs, want_depth=False)
-        wall = time.perf_counter() - t0
-        best = (wall, bst) if best is None or wall < best[0] else best
-    wall, bst = best
-    return plain, bst.reached_edges / 2 / wall / 1e9, wall, bst
+        for s in srcs:
+            ctx.DOBFS(int(s), read=False, with_stats=False)
+        dt = time.perf_counter() - t0
+        plain_wall = dt if plain_wall is None else min(plain_wall, dt)
+    _, bst = ctx.bfs_batch64(srcs, want_depth=False, depth_in_hbm=True)
+    assert bst.reached_edges // 2 == edges
+    batch_wall = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.bfs_batch64(srcs, want_depth=False, depth_in_hbm=True, with_stats=False)
+        dt = time.perf_counter() - t0
+        batch_wall = dt if batch_wall is None else min(batch_wall, dt)
+    plain, batched = edges / plain_wall / 1e9, edges / batch_wall / 1e9
+    plain_dev, batched_dev = edges / (dev_ms / 1e3) / 1e9, edges / (bst.total_ms / 1e3) / 1e9
+    return {"sources": len(srcs), "plain_gteps": plain, "batched_gteps": batched,
+            "batched_over_plain": batched / plain,
+            "plain_device_gteps": plain_dev, "batched_device_gteps": batched_dev,
+            "batched_over_plain_device": batched_dev / plain_dev,
+            "batched_levels": bst.num_levels,
+            "batched_sweep_bytes_per_s": (bst.sweep_alg_bytes / (bst.total_ms / 1e3) if bst.total_ms > 0 else None)}
 
 
 def pe_share_leg(dmm, args):
@@ -280,7 +403,9 @@ def pe_share_leg(dmm, args):
             nxt, st = blk.batch_block_level(levels[lv], seen)
             same = bool(np.array_equal(nxt, levels[lv + 1][b:e]))
             ok = ok and same
-            lv_out.append({"level": lv, "kernel_ms": st.kernel_ms, "edges_examined": st.edges_examined,
+            lv_out.append({"level": lv, "kind": "td" if st.kind == L.DMM_STEP_TD else "bu",
+                           "frontier": st.frontier, "kernel_ms": st.kernel_ms,
+                           "edges_examined": st.edges_examined,
                            "alg_bytes": st.alg_bytes,
                            "gbps": st.alg_bytes / (st.kernel_ms / 1e3) / 1e9 if st.kernel_ms > 0 else None,
                            "equals_whole_graph": same})
@@ -312,23 +437,41 @@ def pe_share_leg(dmm, args):
 
 
 def build_leg(dmm, args):
-    """CSR build of the Kronecker --big-scale edge list: the host Builder
-    against the GPU build (dmm_build_graph_device, both PCIe transfers
-    included), with the two results compared array by array."""
+    """Synthetic graph construction at Kronecker --big-scale: the host
+    generator and Builder against the GPU (dmm_make_graph_device: generate
+    and build with the edge list kept in HBM; dmm_build_graph_device: the
+    GPU build of a host edge list, both PCIe transfers included), with the
+    GPU graph compared array by array with the host graph."""
+    out = {"leg": "build", "scale": args.big_scale}
+    t0 = time.perf_counter()
     pairs = dmm.generate_edgelist(args.big_scale, 16, False)
-    out = {"leg": "build", "scale": args.big_scale, "pairs": int(len(pairs))}
+    out["host_generate_s"] = time.perf_counter() - t0
+    out["pairs"] = int(len(pairs))
+    log(f"host generator: {out['host_generate_s']:.1f}s")
     t0 = time.perf_counter()
     host = dmm.from_edgelist(pairs)
     out["host_build_s"] = time.perf_counter() - t0
     out["host_threads"] = os.cpu_count()
+    log(f"host builder: {out['host_build_s']:.1f}s")
     with dmm.Context(0) as ctx:
-        ctx.build_graph(pairs[:1 << 16])  # first-call set-up outside the timing
+        ctx.make_graph(10, 16, False)  # first-call set-up outside the timing
+        ctx.build_graph(pairs[:1 << 16])
         t0 = time.perf_counter()
         dev = ctx.build_graph(pairs)
         out["device_build_s"] = time.perf_counter() - t0
-    out["identical"] = bool(np.array_equal(host.out_offsets, dev.out_offsets)
-                            and np.array_equal(host.out_neighs, dev.out_neighs))
+        log(f"GPU build of the host edge list: {out['device_build_s']:.1f}s")
+        out["device_build_identical"] = bool(np.array_equal(host.out_offsets, dev.out_offsets)
+                                             and np.array_equal(host.out_neighs, dev.out_neighs))
+        dev.free()
+        del pairs
+        t0 = time.perf_counter()
+        made = ctx.make_graph(args.big_scale, 16, False)
+        out["device_generate_and_build_s"] = time.perf_counter() - t0
+        log(f"GPU generate + build: {out['device_generate_and_build_s']:.1f}s")
+    out["identical"] = bool(np.array_equal(host.out_offsets, made.out_offsets)
+                            and np.array_equal(host.out_neighs, made.out_neighs))
     out["num_edges_directed"] = int(host.num_edges_directed)
+    out["host_generate_and_build_s"] = out["host_generate_s"] + out["host_build_s"]
     print(json.dumps(out), flush=True)
 
 
@@ -470,44 +613,57 @@ def main():
         for i in range(args.warmup):
             ctx.DOBFS(int(srcs[i % 64]), read=False, with_stats=False)
         order = [int(srcs[i % 64]) for i in range(args.steps)]
-        try:
-            import torch
-            sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-        except Exception:
-            sync = lambda: None
-        sync()
+        # dmm_bfs returns after the library's stream has synchronized, so
+        # the timed region starts and ends with the GPU idle.
+        ctx.synchronize()
         t0 = time.perf_counter()
         for s in order:
             ctx.DOBFS(s, read=False, with_stats=False)
-        sync()
+        ctx.synchronize()
         wall = time.perf_counter() - t0
         edges = sum(m_comp[s] for s in order)
         gteps = edges / wall / 1e9
-        # device-only time of the same trials
-        dev_ms = []
+        # Device time and algorithmic bytes of the same trials: the timed
+        # path is the device-resident loop, one hipGraph launch per search.
+        dev_ms, alg = [], []
         for s in order[:64]:
             _, _, st = ctx.DOBFS(s, read=False)
             dev_ms.append(st.total_ms)
+            alg.append(st.alg_bytes)
         dev_gteps = sum(m_comp[s] for s in order[:64]) / (sum(dev_ms) / 1e3) / 1e9
-        # roofline of the bottom-up step kernel
+        search_bps = sum(alg) / (sum(dev_ms) / 1e3)
+        # roofline of the bottom-up step kernel alone (trace mode)
         b, ms, launches = bu_roofline(ctx, srcs[:16], n)
         achieved = b / (ms / 1e3)
-        traffic = load_traffic("k_step<2>", f"kron-{args.scale}-16")
-        roof = {"bound": "hbm", "kernel": "k_step<2> (bottom-up step)", "achieved": achieved / 1e9,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": traffic, "launches": launches, "avg_launch_us": ms * 1e3 / max(1, launches),
-                "alg_bytes_per_launch": b / max(1, launches)}
+        roof = {"bound": "hbm",
+                "kernel": "DOBFS device loop (k_slot chain in one hipGraph launch per search)",
+                "achieved": search_bps / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": search_bps / HBM_PEAK,
+                "traffic": load_traffic("dobfs_search", f"kron-{args.scale}-16"),
+                "launches": len(dev_ms), "avg_launch_us": 1e3 * sum(dev_ms) / len(dev_ms),
+                "alg_bytes_per_launch": sum(alg) / len(alg),
+                "bu_step_kernel": {"kernel": "k_step<2> (bottom-up step, trace mode)",
+                                   "achieved": achieved / 1e9, "frac": achieved / HBM_PEAK,
+                                   "traffic": load_traffic("k_step<2>", f"kron-{args.scale}-16"),
+                                   "launches": launches, "avg_launch_us": ms * 1e3 / max(1, launches),
+                                   "alg_bytes_per_launch": b / max(1, launches)}}
+        tuning = tuning_in_use(ctx)
     og = orc.CSR.of(g)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(og, srcs, m_comp, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+        p = max(2, min(16, cpu["cpu"]["usable"]))
+        cpu["spmd"] = [cpu_spmd_baseline(og, srcs, m_comp, args.cpu_seconds / 2, npes) for npes in (1, p)]
+        log(f"cpu SPMD baselines: {cpu['spmd']}")
     del og
     g.free()
 
     extra = None
     if not args.no_big:
-        extra = kron24_leg(dmm, args, tuned_all)
+        extra = kron24_leg(dmm, args, tuned_all, 0 if args.no_cpu else args.cpu_seconds)
+        if cpu is not None and "cpu_pagerank" in extra:
+            cpu[f"pagerank_kron{args.big_scale}"] = extra.pop("cpu_pagerank")
 
     line = {
         "metric": "BFS GTEPS & PR edges/s/iter, batched/FP32 vs plain GPU, % HBM peak",
@@ -518,7 +674,7 @@ def main():
                                f"64 sources, alpha 15 beta 18, one source per launch",
                    "num_nodes": n, "num_edges_directed": ned,
                    "parallelism": "1 GPU = 1 PE",
-                   "tuning": tuned_all.get(f"kron-{args.scale}-16")},
+                   "tuning": tuning},
         "device_gteps": dev_gteps,
         "roofline": roof,
         "cpu_baseline": cpu,

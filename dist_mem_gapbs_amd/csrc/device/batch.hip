@@ -622,15 +622,18 @@ BArgs make_args(dmm_ctx* ctx, int nsrc, int td_div) {
   return a;
 }
 
-// Direction and frontier list of a PE-block level (see k_front_scan).
+// Direction and frontier list of a PE-block level (see k_front_count).
 struct BlockCtrl {
   unsigned long long count;  // frontier vertices listed
   unsigned long long heavy;  // ... of them with a push list longer than kPushSplit
   unsigned long long work;   // push-list entries of the frontier
+  unsigned long long frontier;  // vertices with a frontier lane
   int kind;                  // DMM_STEP_TD or DMM_STEP_BU
   int pad;
 };
 constexpr int64_t kPushSplit = 4096;  // longer push lists are shared by the whole grid
+constexpr int kScanUnroll = 8;        // independent loads per lane in the frontier scan
+constexpr int kPushUnroll = 4;        // ... in the top-down expansion
 
 // One bottom-up level over a PE block against the all-gathered frontier
 // words of every vertex (front, global IDs): the sweep of bu_phase with the
@@ -708,7 +711,11 @@ __global__ __launch_bounds__(kThreads) void k_block_segs(const BlockCtrl* ctrl, 
   int64_t scanned = 0;
   for (int64_t i = grp; i < nsegs; i += ngrp) {
     const BlockSeg sg = segs[i];
-    const uint64_t need = ~seen[sg.row];
+    // Lanes another segment of the row has already found need no search:
+    // the result is the OR over all segments either way.
+    const uint64_t done = __hip_atomic_load(reinterpret_cast<unsigned long long*>(&next[sg.row]), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t need = ~seen[sg.row] & ~done;
     if (!need) continue;
     uint64_t nxt = 0;
     const int64_t stop = sweep_row<kBlockUnroll>(in_nbr, sg.b, sg.e, sub, need, front, fnz, fnz2, &nxt);
@@ -774,55 +781,180 @@ __global__ __launch_bounds__(kThreads) void k_push_index(const uint64_t* __restr
   }
 }
 
-__global__ void k_block_ctrl_reset(BlockCtrl* c) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    c->count = 0;
-    c->heavy = 0;
-    c->work = 0;
-    c->kind = DMM_STEP_BU;
-    c->pad = 0;
-  }
+// The frontier scan runs in two passes over the same contiguous word range
+// per workgroup, so no same-address atomic is taken per wavefront: the
+// count pass writes the non-zero bitmap (for the bottom-up sweep) and the
+// workgroup's counts of frontier vertices with a light (1 .. kPushSplit)
+// or heavy push list and their push work; one workgroup scans the counts
+// and decides the direction; on a top-down level the list pass writes the
+// vertices at their workgroup's offsets (light ones from the front of
+// `list`, heavy ones from its end).  Vertices with an empty push list are
+// not listed.
+__device__ __forceinline__ void word_range(int64_t nw, int64_t* c0, int64_t* c1) {
+  const int64_t per = (nw + gridDim.x - 1) / gridDim.x;
+  *c0 = std::min<int64_t>(nw, int64_t(blockIdx.x) * per);
+  *c1 = std::min<int64_t>(nw, *c0 + per);
 }
 
-// One wavefront per 64 frontier words: the non-zero bitmap (for the
-// bottom-up sweep), the frontier list (light and heavy push lists apart;
-// the heavy ones fill `list` from its end) and the push work.
-__global__ __launch_bounds__(kThreads) void k_front_scan(const uint64_t* __restrict__ front, int64_t n,
-                                                        const int64_t* __restrict__ poff, uint64_t* __restrict__ nz,
-                                                        int32_t* __restrict__ list, BlockCtrl* ctrl) {
+__global__ __launch_bounds__(kThreads) void k_front_count(const uint64_t* __restrict__ front, int64_t n,
+                                                         const int64_t* __restrict__ poff, uint64_t* __restrict__ nz,
+                                                         const int64_t* __restrict__ in_off,
+                                                         const uint64_t* __restrict__ seen, int64_t nloc,
+                                                         int64_t* __restrict__ part) {
+  __shared__ int64_t red[kThreads / 64];
   const int lane = lane_id();
-  const int64_t wave = (int64_t(blockIdx.x) * kThreads + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * kThreads) >> 6;
-  const int64_t nw = (n + 63) >> 6;
-  unsigned long long work = 0;
-  for (int64_t c = wave; c < nw; c += nwaves) {
-    const int64_t v = c * 64 + lane;
-    const bool on = v < n && front[v] != 0;
-    const uint64_t m = __ballot(on);
-    if (lane == 0) nz[c] = m;
-    if (!m) continue;
-    const int64_t d = on ? poff[v + 1] - poff[v] : 0;
-    const bool heavy = d > kPushSplit;
-    const uint64_t mh = __ballot(heavy);
-    const uint64_t ml = m & ~mh;
-    unsigned long long bl = 0, bh = 0;
-    if (lane == 0) {
-      if (ml) bl = atomicAdd(&ctrl->count, static_cast<unsigned long long>(__popcll(ml)));
-      if (mh) bh = atomicAdd(&ctrl->heavy, static_cast<unsigned long long>(__popcll(mh)));
+  const int wave = threadIdx.x >> 6;
+  // The bottom-up sweep's bound: in-edges of the rows with unseen lanes.
+  // Loads are batched (kScanUnroll independent ones per lane) so enough
+  // bytes are in flight to stream.
+  int64_t bu = 0;
+  const int64_t rper = (nloc + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = std::min<int64_t>(nloc, int64_t(blockIdx.x) * rper), r1 = std::min<int64_t>(nloc, r0 + rper);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kThreads * kScanUnroll) {
+    uint64_t sw[kScanUnroll];
+#pragma unroll
+    for (int u = 0; u < kScanUnroll; u++) {
+      const int64_t q = r + int64_t(u) * kThreads;
+      sw[u] = q < r1 ? ~seen[q] : 0;
     }
-    bl = __shfl(bl, 0);
-    bh = __shfl(bh, 0);
-    if (on && !heavy) list[bl + lane_rank(ml)] = static_cast<int32_t>(v);
-    if (heavy) list[n - 1 - static_cast<int64_t>(bh + lane_rank(mh))] = static_cast<int32_t>(v);
-    work += static_cast<unsigned long long>(d);
+#pragma unroll
+    for (int u = 0; u < kScanUnroll; u++) {
+      const int64_t q = r + int64_t(u) * kThreads;
+      if (sw[u]) bu += in_off[q + 1] - in_off[q];
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) work += __shfl_xor(work, o);
-  if (lane == 0 && work) atomicAdd(&ctrl->work, work);
+  int64_t c0, c1;
+  word_range((n + 63) >> 6, &c0, &c1);
+  int64_t light = 0, heavy = 0, work = 0, all = 0;
+  for (int64_t c = c0 + int64_t(wave) * kScanUnroll; c < c1; c += int64_t(kThreads / 64) * kScanUnroll) {
+    uint64_t f[kScanUnroll];
+#pragma unroll
+    for (int u = 0; u < kScanUnroll; u++) {
+      const int64_t v = (c + u) * 64 + lane;
+      f[u] = (c + u < c1 && v < n) ? front[v] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kScanUnroll; u++) {
+      if (c + u >= c1) break;
+      const int64_t v = (c + u) * 64 + lane;
+      const bool on = f[u] != 0;
+      const uint64_t m = __ballot(on);
+      if (lane == 0) nz[c + u] = m;
+      if (!m) continue;
+      all += on ? 1 : 0;
+      const int64_t d = on ? poff[v + 1] - poff[v] : 0;
+      light += (d > 0 && d <= kPushSplit) ? 1 : 0;
+      heavy += d > kPushSplit ? 1 : 0;
+      work += d;
+    }
+  }
+  light = block_sum<kThreads>(light, red);
+  heavy = block_sum<kThreads>(heavy, red);
+  work = block_sum<kThreads>(work, red);
+  all = block_sum<kThreads>(all, red);
+  bu = block_sum<kThreads>(bu, red);
+  if (threadIdx.x == 0) {
+    part[5 * blockIdx.x] = light;
+    part[5 * blockIdx.x + 1] = heavy;
+    part[5 * blockIdx.x + 2] = work;
+    part[5 * blockIdx.x + 3] = all;
+    part[5 * blockIdx.x + 4] = bu;
+  }
 }
 
-__global__ void k_block_decide(BlockCtrl* ctrl, int64_t m_block, int td_div) {
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    ctrl->kind = static_cast<int64_t>(ctrl->work) * td_div < m_block ? DMM_STEP_TD : DMM_STEP_BU;
+// One workgroup: exclusive scans of the light and heavy counts into
+// offs[2 b], offs[2 b + 1]; totals and the direction into ctrl.
+__global__ __launch_bounds__(kThreads) void k_front_decide(const int64_t* __restrict__ part, int nblocks,
+                                                          int64_t* __restrict__ offs, BlockCtrl* ctrl,
+                                                          int64_t m_block, int td_div) {
+  __shared__ int64_t wl[kThreads / 64], wh[kThreads / 64], red[kThreads / 64];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int per = (nblocks + kThreads - 1) / kThreads;
+  const int b0 = min(nblocks, static_cast<int>(threadIdx.x) * per), b1 = min(nblocks, b0 + per);
+  int64_t l = 0, h = 0, w = 0, f = 0, u = 0;
+  for (int b = b0; b < b1; b++) {
+    l += part[5 * b];
+    h += part[5 * b + 1];
+    w += part[5 * b + 2];
+    f += part[5 * b + 3];
+    u += part[5 * b + 4];
+  }
+  const int64_t il = wave_incl_scan(l), ih = wave_incl_scan(h);
+  if (lane == 63) {
+    wl[wave] = il;
+    wh[wave] = ih;
+  }
+  __syncthreads();
+  int64_t bl = 0, bh = 0;
+  for (int k = 0; k < wave; k++) {
+    bl += wl[k];
+    bh += wh[k];
+  }
+  int64_t el = bl + il - l, eh = bh + ih - h;
+  for (int b = b0; b < b1; b++) {
+    offs[2 * b] = el;
+    offs[2 * b + 1] = eh;
+    el += part[5 * b];
+    eh += part[5 * b + 1];
+  }
+  const int64_t tw = block_sum<kThreads>(w, red);
+  const int64_t tf = block_sum<kThreads>(f, red);
+  const int64_t tu = block_sum<kThreads>(u, red);
+  if (threadIdx.x == kThreads - 1) {
+    ctrl->count = static_cast<unsigned long long>(bl + il);
+    ctrl->heavy = static_cast<unsigned long long>(bh + ih);
+  }
+  if (threadIdx.x == 0) {
+    ctrl->work = static_cast<unsigned long long>(tw);
+    ctrl->frontier = static_cast<unsigned long long>(tf);
+    // Top-down while its work is below 1 / td_div of the block's in-edges
+    // and of the bottom-up sweep's bound.
+    ctrl->kind = tw * td_div < std::min(m_block, tu) ? DMM_STEP_TD : DMM_STEP_BU;
+    ctrl->pad = 0;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_front_list(const BlockCtrl* ctrl, const uint64_t* __restrict__ front,
+                                                        int64_t n, const int64_t* __restrict__ poff,
+                                                        const uint64_t* __restrict__ nz,
+                                                        const int64_t* __restrict__ offs, int32_t* __restrict__ list) {
+  if (ctrl->kind != DMM_STEP_TD) return;
+  __shared__ unsigned int cl, ch;
+  if (threadIdx.x == 0) {
+    cl = 0;
+    ch = 0;
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  int64_t c0, c1;
+  word_range((n + 63) >> 6, &c0, &c1);
+  const int64_t ol = offs[2 * blockIdx.x], oh = offs[2 * blockIdx.x + 1];
+  // 64 non-zero-bitmap words per wavefront step (one per lane); only the
+  // words with frontier vertices are visited.
+  for (int64_t cb = c0 + int64_t(wave) * 64; cb < c1; cb += int64_t(kThreads / 64) * 64) {
+    const uint64_t mine = cb + lane < c1 ? nz[cb + lane] : 0;
+    uint64_t todo = __ballot(mine != 0);
+    while (todo) {
+      const int j = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      todo &= todo - 1;
+      const int64_t v = (cb + j) * 64 + lane;
+      const bool on = v < n && front[v] != 0;
+      const int64_t d = on ? poff[v + 1] - poff[v] : 0;
+      const uint64_t ml = __ballot(d > 0 && d <= kPushSplit);
+      const uint64_t mh = __ballot(d > kPushSplit);
+      unsigned int bl = 0, bh = 0;
+      if (lane == 0) {
+        if (ml) bl = atomicAdd(&cl, static_cast<unsigned int>(__popcll(ml)));
+        if (mh) bh = atomicAdd(&ch, static_cast<unsigned int>(__popcll(mh)));
+      }
+      bl = __shfl(bl, 0);
+      bh = __shfl(bh, 0);
+      if (d > 0 && d <= kPushSplit) list[ol + bl + lane_rank(ml)] = static_cast<int32_t>(v);
+      if (d > kPushSplit) list[n - 1 - (oh + bh + lane_rank(mh))] = static_cast<int32_t>(v);
+    }
+  }
 }
 
 // Top-down: clears next, then ORs every frontier word into the push-list
@@ -849,10 +981,21 @@ __global__ __launch_bounds__(kThreads) void k_block_push(const BlockCtrl* ctrl, 
   for (int64_t i = wave; i < nl; i += nwaves) {
     const int32_t u = list[i];
     const uint64_t fw = front[u];
-    for (int64_t e = poff[u] + lane; e < poff[u + 1]; e += 64) {
-      const int32_t r = ptgt[e];
-      const uint64_t bits = fw & ~seen[r];
-      if (bits) atomicOr(reinterpret_cast<unsigned long long*>(&next[r]), static_cast<unsigned long long>(bits));
+    const int64_t pe = poff[u + 1];
+    // kPushUnroll independent target / seen loads per lane per step.
+    for (int64_t e = poff[u] + lane; e < pe; e += 64 * kPushUnroll) {
+      int32_t r[kPushUnroll];
+      uint64_t sw[kPushUnroll];
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) r[k] = e + 64 * k < pe ? ptgt[e + 64 * k] : -1;
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) sw[k] = r[k] >= 0 ? seen[r[k]] : ~0ull;
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) {
+        const uint64_t bits = fw & ~sw[k];
+        if (bits)
+          atomicOr(reinterpret_cast<unsigned long long*>(&next[r[k]]), static_cast<unsigned long long>(bits));
+      }
     }
   }
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
@@ -1094,7 +1237,8 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_ptgt), sizeof(int32_t) * std::max<int64_t>(1, m),
                         "push index")) ||
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_flist), sizeof(int32_t) * ctx->n, "frontier list")) ||
-        (rc = dev_alloc(ctx, &ctx->d_blkctrl, sizeof(BlockCtrl), "block level control")))
+        (rc = dev_alloc(ctx, &ctx->d_blkctrl, sizeof(BlockCtrl) + sizeof(int64_t) * 7 * (ctx->part_cap / 2),
+                        "block level control")))
       return rc;
     uint64_t* ka = static_cast<uint64_t*>(keys_a.p);
     uint64_t* kb = static_cast<uint64_t*>(keys_b.p);
@@ -1122,11 +1266,14 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
   DMM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   DMM_HIP_CHECK(ctx, hipEventRecord(ctx->ev_a, ctx->stream));
   const int64_t nzw = (ctx->n + 63) / 64;
-  hipLaunchKernelGGL(k_block_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, bc);
-  hipLaunchKernelGGL(k_front_scan, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gfront, ctx->n, ctx->d_poff,
-                     ctx->d_gnz, ctx->d_flist, bc);
-  hipLaunchKernelGGL(k_block_decide, dim3(1), dim3(64), 0, ctx->stream, bc, std::max<int64_t>(1, ctx->m_in),
-                     std::max(1, ctx->tuning.batch.td_div));
+  int64_t* fpart = reinterpret_cast<int64_t*>(static_cast<char*>(ctx->d_blkctrl) + sizeof(BlockCtrl));
+  int64_t* foffs = fpart + 5 * (ctx->part_cap / 2);
+  hipLaunchKernelGGL(k_front_count, dim3(blocks), dim3(kThreads), 0, ctx->stream, ctx->d_gfront, ctx->n, ctx->d_poff,
+                     ctx->d_gnz, ctx->d_in_off, ctx->d_bseen, ctx->nloc, fpart);
+  hipLaunchKernelGGL(k_front_decide, dim3(1), dim3(kThreads), 0, ctx->stream, fpart, blocks, foffs, bc,
+                     std::max<int64_t>(1, ctx->m_in), std::max(1, ctx->tuning.batch.td_div));
+  hipLaunchKernelGGL(k_front_list, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->d_gfront, ctx->n,
+                     ctx->d_poff, ctx->d_gnz, foffs, ctx->d_flist);
   // Top-down (returns at once on a bottom-up level).
   hipLaunchKernelGGL(k_block_clear, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->nloc, ctx->d_bnext);
   hipLaunchKernelGGL(k_block_push, dim3(blocks), dim3(kThreads), 0, ctx->stream, bc, ctx->d_flist, ctx->n,
@@ -1155,7 +1302,7 @@ int batch_block_level(dmm_ctx* ctx, const uint64_t* front, const uint64_t* seen,
     stats->kernel_ms = ms;
     stats->kind = hc.kind;
     stats->pad = 0;
-    stats->frontier = static_cast<int64_t>(hc.count + hc.heavy);
+    stats->frontier = static_cast<int64_t>(hc.frontier);
     // Both: the frontier words (8 B per vertex) read by the scan.
     // Top-down: next cleared (8 B per block vertex); per frontier vertex its
     // list entry, word and push offsets (28 B); per push entry the target,

@@ -367,8 +367,9 @@ typedef struct dmm_block_level_stats {
 /* One level of the batched BFS over the uploaded PE block, with the
  * frontier all-gathered from every PE, as one PE of the vertex-block layout
  * (src/partition.h:69-161) would run it, 64 sources at once.  The direction
- * is chosen per level as dmm_bfs_batch64 chooses it (top-down while the
- * frontier's push work is below the block's in-edges / td_div):
+ * is chosen per level on the device: top-down while the frontier's push
+ * work, times td_div, is below both the block's in-edges and the in-edges
+ * of the rows with unseen lanes (the bottom-up sweep's bound):
  *   top-down   owner-computes expansion (the per-PE TD step of
  *              src/bfs.cc:81-100): each frontier vertex's lanes go to the
  *              block rows whose in-list holds it, through the transpose of

@@ -312,11 +312,11 @@ def ref_make_graph(scale, degree, uniform):
 class RefGraph:
     """The reference's CSRGraph built from CSR arrays (both kernels)."""
 
-    def __init__(self, g):
+    def __init__(self, g, bfs=True, pr=True):
         b, p = _ref_libs()
         self.g = g
-        self.hb = b.ref_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32))
-        self.hp = p.ref_pr_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32))
+        self.hb = b.ref_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32)) if bfs else None
+        self.hp = p.ref_pr_graph_create(g.n, _p(g.oo, C.c_int64), _p(g.on, C.c_int32)) if pr else None
 
     def dobfs(self, source, alpha=15, beta=18):
         b, _ = _ref_libs()
