@@ -155,6 +155,7 @@ SIGNATURES = {
     "dmm_ctx_synchronize": (_I32, [_P]),
     "dmm_build_graph_device": (_I32, [_P, _I64, _PI32, _I64, _I32, C.POINTER(_P)]),
     "dmm_bc_forward": (_I32, [_P, _I32, _PI32, _PU64, _PU64, _PI32]),
+    "dmm_bc_read": (_I32, [_P, _PI32, _PU64, _PU64]),
     "dmm_graph_upload": (_I32, [_P, _I64, _I64, _I64, _PI64, _PI32, _PI64, _PI32, _I64]),
     "dmm_ctx_device_bytes": (_I32, [_P, _PI64]),
     "dmm_ctx_set_layout": (_I32, [_P, _I32]),

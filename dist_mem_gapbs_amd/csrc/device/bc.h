@@ -13,6 +13,8 @@ namespace dmm {
 // out-CSR's edge positions, and the number of levels.  Outputs may be NULL.
 int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts, uint64_t* succ,
                int32_t* num_levels);
+// Copies the last result out of HBM (any output may be NULL).
+int bc_read(dmm_ctx* ctx, int32_t* depth, uint64_t* path_counts, uint64_t* succ);
 
 }  // namespace dmm
 

@@ -904,16 +904,30 @@ __global__ __launch_bounds__(kThreads) void k_init(StepArgs a, const BfsParams* 
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
   const int64_t init_end = prm->init_end;
   const int64_t end = a.depth ? a.nloc : init_end;
-  for (int64_t v = gtid; v < end; v += gsize) {
-    const int64_t d = a.out_off[v + 1] - a.out_off[v];
-    int32_t p = d != 0 ? static_cast<int32_t>(-d) : -1;
-    int32_t dep = -1;
-    if (a.vbegin + v == source) {
-      p = source;
-      dep = 0;
+  // kInitUnroll rows per thread per step, their offset loads issued together.
+  constexpr int kInitUnroll = 4;
+  for (int64_t v0 = gtid; v0 < end; v0 += gsize * kInitUnroll) {
+    int64_t o0[kInitUnroll], o1[kInitUnroll];
+#pragma unroll
+    for (int k = 0; k < kInitUnroll; k++) {
+      const int64_t v = v0 + k * gsize;
+      o0[k] = v < end ? a.out_off[v] : 0;
+      o1[k] = v < end ? a.out_off[v + 1] : 0;
     }
-    if (v < init_end) a.parent[v] = p;
-    if (a.depth) a.depth[v] = dep;
+#pragma unroll
+    for (int k = 0; k < kInitUnroll; k++) {
+      const int64_t v = v0 + k * gsize;
+      if (v >= end) break;
+      const int64_t d = o1[k] - o0[k];
+      int32_t p = d != 0 ? static_cast<int32_t>(-d) : -1;
+      int32_t dep = -1;
+      if (a.vbegin + v == source) {
+        p = source;
+        dep = 0;
+      }
+      if (v < init_end) a.parent[v] = p;
+      if (a.depth) a.depth[v] = dep;
+    }
   }
   for (int64_t w = gtid; w < a.words; w += gsize) {
     uint64_t x = 0;
@@ -964,8 +978,14 @@ __global__ __launch_bounds__(kThreads) void k_finish(StepArgs a, BfsHead* head, 
   if (head && head->done == 0) return;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t v = gtid; v < a.nloc; v += gsize) {
-    if (a.parent[v] < -1) a.parent[v] = -1;
+  constexpr int kUnroll = 4;
+  for (int64_t v0 = gtid; v0 < a.nloc; v0 += gsize * kUnroll) {
+    int32_t p[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) p[k] = v0 + k * gsize < a.nloc ? a.parent[v0 + k * gsize] : 0;
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++)
+      if (p[k] < -1) a.parent[v0 + k * gsize] = -1;
   }
   if (head && gtid == 0) finish_flags(head, flags);
 }

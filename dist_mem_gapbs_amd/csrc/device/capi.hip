@@ -117,7 +117,7 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_bsegs, ctx->d_head, ctx->d_poff, ctx->d_ptgt, ctx->d_flist, ctx->d_blkctrl,
                   ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
-                  ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp};
+                  ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp, ctx->d_bc_bm};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -133,6 +133,8 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bc_pc = ctx->d_bc_succ = ctx->d_bc_tail = nullptr;
   ctx->d_bc_qdeg = ctx->d_bc_qoff = nullptr;
   ctx->d_bc_temp = nullptr;
+  ctx->d_bc_bm = nullptr;
+  ctx->bc_valid = false;
   ctx->bc_temp_bytes = 0;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
@@ -453,6 +455,12 @@ int dmm_bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_
   if (source < 0 || source >= ctx->n)
     return set_err(ctx, DMM_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)ctx->n);
   return dmm::bc_forward(ctx, source, depth, path_counts, succ, num_levels);
+}
+
+int dmm_bc_read(dmm_ctx* ctx, int32_t* depth, uint64_t* path_counts, uint64_t* succ) {
+  int rc = need_graph(ctx);
+  if (rc) return rc;
+  return dmm::bc_read(ctx, depth, path_counts, succ);
 }
 
 int dmm_build_graph_device(dmm_ctx* ctx, int64_t num_pairs, const int32_t* pairs, int64_t num_nodes,

@@ -168,6 +168,8 @@ struct dmm_ctx {
   int64_t* d_bc_qdeg = nullptr;  // frontier degrees, then their exclusive prefix
   int64_t* d_bc_qoff = nullptr;
   void* d_bc_temp = nullptr;     // scan scratch for n + 1 entries
+  uint64_t* d_bc_bm = nullptr;   // two level bitmaps (depth L-1, depth L)
+  bool bc_valid = false;            // a Brandes forward result is held
   size_t bc_temp_bytes = 0;
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed

@@ -61,16 +61,21 @@ __global__ __launch_bounds__(kThreads) void k_map_parent(const int32_t* __restri
   const int64_t scan_end = *scan_end_p;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t v = gtid; v < n; v += gsize) {
-    const int32_t u = perm[v];
-    int32_t po = -1, dep = -1;
-    if (u < scan_end) {
-      const int32_t p = parent[u];
-      if (p >= 0) po = inv[p];
-      if (depth) dep = depth[u];
+  // kUnroll independent perm -> parent -> inv chains per thread per step.
+  constexpr int kUnroll = 4;
+  for (int64_t v0 = gtid; v0 < n; v0 += gsize * kUnroll) {
+    int32_t u[kUnroll], p[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) u[k] = v0 + k * gsize < n ? perm[v0 + k * gsize] : INT32_MAX;
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) p[k] = u[k] < scan_end ? parent[u[k]] : -1;
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+      const int64_t v = v0 + k * gsize;
+      if (v >= n) break;
+      parent_out[v] = p[k] >= 0 ? inv[p[k]] : -1;
+      if (depth) depth_out[v] = u[k] < scan_end ? depth[u[k]] : -1;
     }
-    parent_out[v] = po;
-    if (depth) depth_out[v] = dep;
   }
   if (gate && gtid == 0) finish_flags(gate, flags);
 }

@@ -68,16 +68,29 @@ class Context:
                                                      1 if symmetrize else 0, C.byref(h)))
         return HostGraph(h)
 
-    def bc_forward(self, source):
+    def bc_forward(self, source, read=True):
         """Brandes' forward phase (src/bc.cc:55-96) from source on the uploaded
-        whole graph (natural layout): (depth, path_counts, succ words, levels)."""
-        depth = np.empty(self.nloc, np.int32)
-        pc = np.empty(self.nloc, np.uint64)
-        succ = np.empty(max(1, (self.m_out + 63) // 64), np.uint64)
+        whole graph (natural layout): (depth, path_counts, succ words, levels);
+        with read=False the arrays stay in HBM (bc_read) and come back None."""
         levels = C.c_int32()
+        if not read:
+            self._check(self._lib.dmm_bc_forward(self._h, int(source), None, None, None, C.byref(levels)))
+            return None, None, None, levels.value
+        depth, pc, succ = self.bc_read()
         self._check(self._lib.dmm_bc_forward(self._h, int(source), L.ptr(depth, C.c_int32), L.ptr(pc, C.c_uint64),
                                              L.ptr(succ, C.c_uint64), C.byref(levels)))
-        return depth, pc, succ[:(self.m_out + 63) // 64], levels.value
+        return depth, pc, succ, levels.value
+
+    def bc_read(self, fetch=False):
+        """Buffers for a Brandes forward result; with fetch=True filled with
+        the last result kept in HBM."""
+        depth = np.empty(self.nloc, np.int32)
+        pc = np.empty(self.nloc, np.uint64)
+        succ = np.empty((self.m_out + 63) // 64, np.uint64)
+        if fetch:
+            self._check(self._lib.dmm_bc_read(self._h, L.ptr(depth, C.c_int32), L.ptr(pc, C.c_uint64),
+                                              L.ptr(succ, C.c_uint64)))
+        return depth, pc, succ
 
     def upload(self, g=None, *, num_nodes=None, out_offsets=None, out_neighs=None,
                in_offsets=None, in_neighs=None, vbegin=0, vend=None, num_edges_directed=-1):

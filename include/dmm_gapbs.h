@@ -183,6 +183,12 @@ int dmm_make_graph_device(dmm_ctx* ctx, int32_t scale, int32_t degree, int32_t u
 int dmm_bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_counts,
                    uint64_t* succ, int32_t* num_levels);
 
+/* Copies the last dmm_bc_forward result out of HBM (any output may be
+ * NULL): a search run with NULL outputs keeps its result on the device
+ * until the next search or upload.  No reference counterpart: the
+ * read-out half of a result kept in HBM. */
+int dmm_bc_read(dmm_ctx* ctx, int32_t* depth, uint64_t* path_counts, uint64_t* succ);
+
 /* Uploads one PE's share of a CSR graph (replaces the symmetric-heap CSR of
  * src/builder.h:236-288 and the remote Neighborhood fetch of
  * src/graph.h:138-162).

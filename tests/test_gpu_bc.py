@@ -61,6 +61,28 @@ def test_bc_forward_grid_and_path(natural, dmm, orc):
     _check(natural, orc, orc.CSR.of(g), 350)
 
 
+def test_bc_forward_kept_in_hbm(natural, dmm, orc, graph):
+    """With no host buffers the result stays in HBM; dmm_bc_read returns
+    it.  A complete graph's second level has no unvisited vertex left."""
+    g = graph(13, 16, False)
+    og = orc.CSR.of(g)
+    natural.upload(g)
+    for s in dmm.pick_sources(g, 3):
+        print(f"[bc kept in HBM, source {s}]", flush=True)
+        none, _, _, levels = natural.bc_forward(int(s), read=False)
+        assert none is None
+        d, pc, succ = natural.bc_read(fetch=True)
+        rd, rpc, rsucc, rlevels = orc.bc_forward(og, int(s))
+        assert np.array_equal(d, rd) and np.array_equal(pc, rpc) and np.array_equal(succ, rsucc)
+        assert levels == rlevels
+    k = 200
+    g = dmm.from_edgelist([[i, j] for i in range(k) for j in range(i + 1, k)], k)
+    natural.upload(g)
+    with pytest.raises(dmm.DmmError):
+        natural.bc_read(fetch=True)  # the upload dropped the last result
+    _check(natural, orc, orc.CSR.of(g), 5)
+
+
 def test_bc_forward_needs_natural_layout(ctx, dmm, graph):
     ctx.set_layout(L.DMM_LAYOUT_DEGREE)
     ctx.upload(graph(10, 8, False))
