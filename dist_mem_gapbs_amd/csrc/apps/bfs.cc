@@ -1,18 +1,29 @@
 // bfs: direction-optimizing BFS app with the reference's command line
 // (src/bfs.cc:238-278).  DOBFS keeps the GAPBS signature
 // (gapbs-verify/bfs.cc:122-123); its body is one call into the GPU engine.
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <vector>
 
 #include "harness.h"
 
+// Per-step device times in the step lines need the host-driven search
+// (one launch pair and a synchronisation per step); by default the search
+// runs as the device-resident loop, which records each step's kind and
+// count but not its time (printed as 0).  DMM_STEP_TIMES=1 selects the
+// host-driven search.
+static bool step_times() {
+  const char* v = std::getenv("DMM_STEP_TIMES");
+  return v && v[0] == '1';
+}
+
 // src/bfs.cc:142 without the OpenSHMEM work arrays.
 std::vector<NodeID> DOBFS(const Graph& g, NodeID source, int alpha = 15, int beta = 18) {
   PrintStep("Source", static_cast<int64_t>(source));
   std::vector<NodeID> parent(g.num_nodes());
   dmm_bfs_stats st;
-  if (dmm_bfs(g.ctx(), source, alpha, beta, parent.data(), nullptr, 1, &st) != DMM_OK) {
+  if (dmm_bfs(g.ctx(), source, alpha, beta, parent.data(), nullptr, step_times() ? 1 : 0, &st) != DMM_OK) {
     std::cerr << "dmm_bfs: " << dmm_last_error(g.ctx()) << std::endl;
     std::exit(1);
   }

@@ -15,7 +15,8 @@
 // The reference writes `Edge(udist(rng), udist(rng))`; GCC evaluates the
 // two calls right to left, so the FIRST draw is the pair's v and the second
 // its u.  The same holds for the order of the pair below (checked against a
-// GCC build of the reference by tests/test_ref_parity.py).
+// GCC build of the reference by tests/test_generator.py:
+// test_pair_order_fixture and test_graph_equals_reference_builder).
 #ifndef DMM_HOST_GENERATOR_H_
 #define DMM_HOST_GENERATOR_H_
 
