@@ -601,6 +601,7 @@ __device__ __forceinline__ Comp merge_list(const float2* reg, const uint16_t* __
 }
 
 constexpr int kSegBatch = kRowThreads;  // segments staged per round
+constexpr int kXcds = 8;                 // MI355X: 8 XCDs, each with its own L2
 constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per thread
 
 // Row pass of the tiled mode: persistent workgroups take row blocks in
@@ -623,7 +624,17 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int k = static_cast<int>(threadIdx.x);
-  for (int64_t b = blockIdx.x; b < t.n_blocks; b += gridDim.x) {
+  // XCD-aware order: workgroups are dealt round-robin to the kXcds XCDs, so
+  // workgroup g runs on XCD g % kXcds.  Blocks go out in groups of W
+  // consecutive blocks (W = workgroups per XCD), group j to XCD j % kXcds,
+  // so in each round one XCD's workgroups take W neighbouring blocks, whose
+  // segments share lines of the partial array, and the XCDs alternate
+  // groups along the whole block range.
+  const int64_t W = gridDim.x % kXcds == 0 ? gridDim.x / kXcds : 0;
+  const int64_t xcd = blockIdx.x % kXcds, lw = blockIdx.x / kXcds;
+  for (int64_t i = 0;; i++) {
+    const int64_t b = W ? (xcd + kXcds * i) * W + lw : blockIdx.x + i * gridDim.x;
+    if (b >= t.n_blocks) break;
     const int32_t r0 = t.brow[b], r1 = t.brow[b + 1];
     const int64_t s0 = t.bseg[b], s1 = t.bseg[b + 1];
     const int32_t size = t.bsize[b];
