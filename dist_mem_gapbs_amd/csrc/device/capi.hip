@@ -116,8 +116,8 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_parent_out, ctx->d_depth_out, ctx->d_score_out, ctx->d_score64_out,
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_bsegs, ctx->d_head, ctx->d_poff, ctx->d_ptgt, ctx->d_flist, ctx->d_blkctrl,
-                  ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_tail,
-                  ctx->d_bc_qdeg, ctx->d_bc_qoff, ctx->d_bc_temp, ctx->d_bc_bm};
+                  ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_part,
+                  ctx->d_bc_qoff, ctx->d_bc_bm};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -130,12 +130,10 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bm[0] = ctx->d_bm[1] = ctx->d_bm[2] = nullptr;
   ctx->d_head = nullptr;
   ctx->d_bc_depth = ctx->d_bc_queue = nullptr;
-  ctx->d_bc_pc = ctx->d_bc_succ = ctx->d_bc_tail = nullptr;
-  ctx->d_bc_qdeg = ctx->d_bc_qoff = nullptr;
-  ctx->d_bc_temp = nullptr;
+  ctx->d_bc_pc = ctx->d_bc_succ = nullptr;
+  ctx->d_bc_part = ctx->d_bc_qoff = nullptr;
   ctx->d_bc_bm = nullptr;
   ctx->bc_valid = false;
-  ctx->bc_temp_bytes = 0;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;
   ctx->d_heavy = nullptr;
@@ -409,7 +407,9 @@ int dmm_ctx_create(int32_t device, dmm_ctx** out) {
       hipHostMalloc((void**)&ctx->h_prctrl, sizeof(dmm::PrCtrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&ctx->h_prparams, dmm::kPrParamsBytes, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&ctx->h_bctrl, dmm::batch_ctrl_bytes(), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&ctx->h_bsrc, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc(&ctx->h_bsrc, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_bc_sizes, 64, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&ctx->hd_bc_sizes, ctx->h_bc_sizes, 0) != hipSuccess)
     return bail(set_err(nullptr, DMM_ERR_NOMEM, "pinned host allocation failed"));
   std::memset(ctx->h_bsrc, 0, 64 * sizeof(int32_t));
   ctx->fixed_bytes = ctx->dev_bytes;
@@ -433,7 +433,7 @@ void dmm_ctx_destroy(dmm_ctx* ctx) {
                   ctx->d_err_trace, ctx->d_bctrl, ctx->d_bsrc, ctx->d_part};
   for (void* p : bufs) dmm::dev_free(p);
   void* hbufs[] = {ctx->h_ctrl, ctx->h_params, ctx->h_prctrl, ctx->h_prparams, ctx->h_bctrl, ctx->h_bsrc,
-                   ctx->h_flags};
+                   ctx->h_flags, ctx->h_bc_sizes};
   for (void* p : hbufs)
     if (p) (void)hipHostFree(p);
   for (auto e : ctx->step_events)

@@ -163,14 +163,13 @@ struct dmm_ctx {
   int32_t* d_bc_depth = nullptr;
   uint64_t* d_bc_pc = nullptr;
   uint64_t* d_bc_succ = nullptr;
-  int32_t* d_bc_queue = nullptr;
-  uint64_t* d_bc_tail = nullptr;
-  int64_t* d_bc_qdeg = nullptr;  // frontier degrees, then their exclusive prefix
-  int64_t* d_bc_qoff = nullptr;
-  void* d_bc_temp = nullptr;     // scan scratch for n + 1 entries
-  uint64_t* d_bc_bm = nullptr;   // two level bitmaps (depth L-1, depth L)
+  int32_t* d_bc_queue = nullptr;  // the frontier's vertices of out-degree >= 1, ascending
+  int64_t* d_bc_part = nullptr;   // per-workgroup counts of the frontier compaction
+  int64_t* d_bc_qoff = nullptr;   // exclusive out-degree prefix of d_bc_queue
+  uint64_t* d_bc_bm = nullptr;    // two level bitmaps (depth L-1, depth L)
+  uint64_t* h_bc_sizes = nullptr;   // pinned, mapped: the next frontier's sizes
+  uint64_t* hd_bc_sizes = nullptr;  // device address of h_bc_sizes
   bool bc_valid = false;            // a Brandes forward result is held
-  size_t bc_temp_bytes = 0;
   int64_t beta_width = 0;        // Partition::max_width of this PE
   void* d_params = nullptr;      // BfsParams + num_edges_directed
   void* h_params = nullptr;      // pinned staging for d_params, mapped: the device loop's InitParent reads it

@@ -105,3 +105,18 @@ def test_bc_forward_needs_an_uploaded_graph(dmm):
         with pytest.raises(dmm.DmmError) as e:
             c._check(c._lib.dmm_bc_forward(c._h, 0, None, None, None, None))
         assert e.value.code == L.DMM_ERR_STATE
+
+
+def test_bc_forward_sinks_and_isolated_source(natural, dmm, orc):
+    """Directed graph where most vertices have out-degree 0: frontier sinks
+    are found and counted but expand nothing; a source of out-degree 0 is a
+    one-level search."""
+    rng = np.random.default_rng(5)
+    n = 4000
+    pairs = np.stack([rng.integers(0, n // 4, size=6000), rng.integers(0, n, size=6000)], 1)
+    g = dmm.from_edgelist(pairs, n, symmetrize=False)
+    og = orc.CSR.of(g)
+    assert og.oo[n - 1] == og.oo[n]
+    natural.upload(g)
+    for s in (0, 1, 5, n - 1):
+        _check(natural, orc, og, s)
