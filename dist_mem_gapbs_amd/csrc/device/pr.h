@@ -31,8 +31,8 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 // 16384 or 32768 contributions, 64 or 128 KB of LDS).  Every in-edge sits
 // in a 64-lane chunk of its source's window, kLaneEdges edges per lane, and
 // is summed from an LDS copy of that window; each piece (a row's edges
-// inside one chunk) leaves one (sum, error) partial, and the row kernel
-// merges a row's partials in window order.
+// inside one chunk) leaves one partial, its FP64 sum rounded to FP32, and
+// the row kernel merges a row's partials in window order.
 constexpr int kLaneEdges = 16;
 constexpr int kChunkEdges = 64 * kLaneEdges;
 constexpr int kChunkPieces = 192;   // pieces one chunk may hold
@@ -40,10 +40,10 @@ constexpr int kChunkPieces = 192;   // pieces one chunk may hold
 // kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
 // A block's partials (its runs plus chunk cuts) must number at most
 // kRowSlots: the row kernel gathers them into LDS.
-constexpr int kRowSlots = 4096;
+constexpr int kRowSlots = 8192;
 constexpr int kRowThreads = 512;    // threads of the row kernel
 constexpr int kRowGroupsPerCU = 4;  // row kernel workgroups resident per CU
-constexpr int kBlockRuns = 3072;
+constexpr int kBlockRuns = 6144;
 constexpr int kSoloRuns = 256;
 constexpr int64_t kSoloEdges = 4096;
 

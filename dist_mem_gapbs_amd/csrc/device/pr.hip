@@ -407,7 +407,7 @@ struct TileArgs {
   const uint32_t* meta;    // per lane: end mask | filled slots << 16
   const uint16_t* src;
   int64_t n_chunks;
-  float2* part;            // (sum, error) per piece
+  float* part;             // per piece: its FP64 sum rounded to FP32
   const int32_t* brow;     // row blocks: first row (+ end)
   const int64_t* bseg;     //             first segment (+ end)
   const int32_t* bsize;    //             partials (the region's size)
@@ -441,11 +441,6 @@ __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, in
   return l;
 }
 
-__device__ __forceinline__ float2 split_pair(double s) {
-  const float hi = static_cast<float>(s);
-  return make_float2(hi, static_cast<float>(s - static_cast<double>(hi)));
-}
-
 // One chunk of the tile pass, run by one wavefront.  Lane i holds slots
 // [i * kLaneEdges, (i + 1) * kLaneEdges) of the chunk; a piece may span
 // lanes but never chunks.  First pass: each lane sums its edges after its
@@ -455,12 +450,13 @@ __device__ __forceinline__ float2 split_pair(double s) {
 // the lane sums its edges again from the carry, restarting at every piece
 // end, where a piece is complete.  Values accumulate in FP64 (a piece holds
 // at most kChunkEdges FP32 values, relative error <= kChunkEdges * 2^-53)
-// and a piece enters its row's compensated FP32 sum as the unevaluated pair
-// (hi, lo) = (float(s), float(s - hi)).
+// and a piece enters its row's compensated FP32 sum rounded to FP32 (half
+// a unit in its last place).  Every term of a row's sum is non-negative, so
+// these roundings add at most one unit in the last place of the row's sum.
 //
 // The chunk's pieces (at most kChunkPieces), numbered in lane order, are
-// staged (as FP64 sums) in the wavefront's LDS area and written, split
-// into pairs, to the chunk's contiguous partial range by kChunkPieces / 64
+// staged (as FP64 sums) in the wavefront's LDS area and written, rounded
+// to FP32, to the chunk's contiguous partial range by kChunkPieces / 64
 // store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
@@ -526,7 +522,7 @@ __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win,
 #pragma unroll
     for (int r = 0; r < kChunkPieces / 64; r++) {
       const int e = min(lane + 64 * r, np - 1);
-      t.part[c0 + e] = split_pair(stage[e]);
+      t.part[c0 + e] = static_cast<float>(stage[e]);
     }
     __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
   }
@@ -585,7 +581,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
 // Compensated merge of the region's partials reg[ix[0..cnt)] in order; the
 // offsets are loaded kMergeAhead at a time, all before their LDS reads.
 constexpr int kMergeAhead = 8;
-__device__ __forceinline__ Comp merge_list(const float2* reg, const uint16_t* __restrict__ ix, int cnt) {
+__device__ __forceinline__ Comp merge_list(const float* reg, const uint16_t* __restrict__ ix, int cnt) {
   Comp acc{0.0f, 0.0f};
   for (int k = 0; k < cnt; k += kMergeAhead) {
     uint16_t q[kMergeAhead];
@@ -593,8 +589,8 @@ __device__ __forceinline__ Comp merge_list(const float2* reg, const uint16_t* __
     for (int j = 0; j < kMergeAhead; j++) q[j] = k + j < cnt ? ix[k + j] : 0;
 #pragma unroll
     for (int j = 0; j < kMergeAhead; j++) {
-      const float2 v = reg[q[j]];
-      if (k + j < cnt) acc.merge(v.x, v.y);
+      const float v = reg[q[j]];
+      if (k + j < cnt) acc.add(v);
     }
   }
   return acc;
@@ -614,7 +610,7 @@ constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per threa
 // Rows without in-edges (the tail) only finish.
 __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) void k_rows(PrArgs a, TileArgs t, float* __restrict__ cout,
                                                      float* __restrict__ score) {
-  __shared__ float2 reg[kRowSlots];
+  __shared__ float reg[kRowSlots];
   __shared__ int32_t soff[kSegBatch + 1];
   __shared__ int32_t sstart[kSegBatch];
   __shared__ float2 wred[kRowThreads / 64];
@@ -648,7 +644,7 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
       if (k == 0) soff[ns] = sb + ns < s1 ? t.seg_off[sb + ns] : size;
       __syncthreads();
       const int lo_e = soff[0], hi_e = soff[ns];
-      float2 v[kRegionPer];
+      float v[kRegionPer];
 #pragma unroll
       for (int i = 0; i < kRegionPer; i++) {
         const int e = lo_e + k + i * kRowThreads;
@@ -670,7 +666,7 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
     }
     if (r1 - r0 == 1) {
       FusedAcc<float> acc;
-      for (int p = k; p < size; p += kRowThreads) acc.merge(reg[p]);
+      for (int p = k; p < size; p += kRowThreads) acc.add(reg[p]);
       acc.reduce<64>(lane);
       if (lane == 0) wred[wave] = make_float2(acc.c.s, acc.c.c);
       __syncthreads();
@@ -861,7 +857,7 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       t.meta = ctx->d_wt_meta;
       t.src = ctx->d_wt_src;
       t.n_chunks = ctx->n_chunks;
-      t.part = static_cast<float2*>(ctx->d_wt_part);
+      t.part = static_cast<float*>(ctx->d_wt_part);
       t.brow = ctx->d_wt_brow;
       t.bseg = ctx->d_wt_bseg;
       t.bsize = ctx->d_wt_bsize;

@@ -327,7 +327,7 @@ int pr_tiles_build(dmm_ctx* ctx) {
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_seg_off), th.seg_off, "PR segments")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_istart), th.istart, "PR row lists")) ||
       (rc = upload_vec(ctx, reinterpret_cast<void**>(&ctx->d_wt_idx), th.idx, "PR row piece lists")) ||
-      (rc = dev_alloc(ctx, &ctx->d_wt_part, 8 * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) {
+      (rc = dev_alloc(ctx, &ctx->d_wt_part, sizeof(float) * std::max<int64_t>(1, th.n_pieces), "PR tile partials"))) {
     ctx->tiles_bytes = ctx->dev_bytes - held;  // what the failed build holds, freed with the layout
     return rc;
   }
@@ -354,6 +354,13 @@ struct HostComp {
     const float e = (s - (t - bb)) + (s2 - bb);
     s = t;
     c = (c + c2) + e;
+  }
+  void add(float x) {  // the device's Comp::add
+    const float t = s + x;
+    const float bb = t - s;
+    const float e = (s - (t - bb)) + (x - bb);
+    s = t;
+    c += e;
   }
   float value() const { return s + c; }
 };
@@ -428,7 +435,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       th.brow.back() != zb)
     return bad("sizes");
   std::vector<char> written(std::max<int64_t>(1, np), 0);
-  std::vector<float> px(std::max<int64_t>(1, np)), py(std::max<int64_t>(1, np));
+  std::vector<float> px(std::max<int64_t>(1, np));
   int64_t edges = 0;
   // k_tile, chunk by chunk.
   for (int32_t w = 0; w < th.n_windows; w++) {
@@ -475,7 +482,6 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
             if (slot >= th.cbase[c + 1] || written[slot]) return bad("output slot");
             written[slot] = 1;
             px[slot] = static_cast<float>(a);
-            py[slot] = static_cast<float>(a - static_cast<double>(px[slot]));
             slot++;
             a = 0.0;
           }
@@ -501,21 +507,17 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
     if (!written[p]) return bad("slot never written");
   // k_rows: each block's region gathered from its segments, window order.
   int64_t max_block = 0;
-  std::vector<float> rx, ry;
+  std::vector<float> rx;
   for (int64_t b = 0; b < nb; b++) {
     const int32_t r0 = th.brow[b], r1 = th.brow[b + 1];
     if (r1 <= r0) return bad("row block bounds");
     rx.clear();
-    ry.clear();
     for (int64_t s = th.bseg[b]; s < th.bseg[b + 1]; s++) {
       const int64_t len = (s + 1 < th.bseg[b + 1] ? th.seg_off[s + 1] : th.bsize[b]) - th.seg_off[s];
       if (th.seg_off[s] != static_cast<int64_t>(rx.size()) || th.seg_start[s] < 0 || len <= 0 ||
           th.seg_start[s] + len > np)
         return bad("segment bounds");
-      for (int64_t k = 0; k < len; k++) {
-        rx.push_back(px[th.seg_start[s] + k]);
-        ry.push_back(py[th.seg_start[s] + k]);
-      }
+      for (int64_t k = 0; k < len; k++) rx.push_back(px[th.seg_start[s] + k]);
     }
     if (static_cast<int64_t>(rx.size()) != th.bsize[b]) return bad("block size");
     const int64_t rn = static_cast<int64_t>(rx.size());
@@ -524,7 +526,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
     if (r1 - r0 == 1) {
       HostComp lanes[kRowThreads];
       for (int t = 0; t < kRowThreads; t++)
-        for (int64_t p = t; p < rn; p += kRowThreads) lanes[t].merge(rx[p], ry[p]);
+        for (int64_t p = t; p < rn; p += kRowThreads) lanes[t].add(rx[p]);
       HostComp wv[kRowThreads / 64];
       for (int w = 0; w < kRowThreads / 64; w++) wv[w] = butterfly64(lanes + 64 * w);
       HostComp tot = wv[0];
@@ -539,7 +541,7 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
       for (int64_t k = th.istart[r]; k < th.istart[r + 1]; k++) {
         const int64_t q = th.idx[k];
         if (q >= rn) return bad("partial offset beyond its block");
-        acc.merge(rx[q], ry[q]);
+        acc.add(rx[q]);
       }
       incoming[r] = acc.value();
     }
