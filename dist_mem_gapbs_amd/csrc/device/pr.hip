@@ -455,14 +455,14 @@ __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, in
 // these roundings add at most one unit in the last place of the row's sum.
 //
 // The chunk's pieces (at most kChunkPieces), numbered in lane order, are
-// staged (as FP64 sums) in the wavefront's LDS area and written, rounded
-// to FP32, to the chunk's contiguous partial range by kChunkPieces / 64
+// staged, rounded to FP32, in the wavefront's LDS area and written to the
+// chunk's contiguous partial range by kChunkPieces / 64
 // store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
 template <int N>
 __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win, const ChunkLoad* l, int lane,
-                                            double* stage) {
+                                            float* stage) {
   // N chunks side by side: their gathers, first passes and scans are
   // independent chains the scheduler interleaves.
   float x[N][kLaneEdges];
@@ -509,7 +509,7 @@ __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win,
     for (int j = 0; j < kLaneEdges; j++) {
       a += static_cast<double>(x[n][j]);
       const uint32_t e = (ends[n] >> j) & 1u;
-      stage[e ? qq : kChunkPieces + lane] = a;
+      stage[e ? qq : kChunkPieces + lane] = static_cast<float>(a);
       qq += e;
       a = e ? 0.0 : a;
     }
@@ -522,7 +522,7 @@ __device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win,
 #pragma unroll
     for (int r = 0; r < kChunkPieces / 64; r++) {
       const int e = min(lane + 64 * r, np - 1);
-      t.part[c0 + e] = static_cast<float>(stage[e]);
+      t.part[c0 + e] = stage[e];
     }
     __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
   }
@@ -536,13 +536,13 @@ template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
-  __shared__ double stage_all[kTileThreads / 64][kChunkPieces + 64];
+  __shared__ float stage_all[kTileThreads / 64][kChunkPieces + 64];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar chunk indices
-  double* stage = stage_all[wave];
+  float* stage = stage_all[wave];
   constexpr int kWaves = kTileThreads / 64;
   for (int64_t c = c0; c < c1;) {
     const int32_t w = t.cwin[c];
