@@ -20,19 +20,33 @@
 //
 // Bottom-up (pull), for a frontier whose out-edges outnumber the unvisited
 // vertices' by kBcPull: one wavefront per 64-vertex bitmap word sums, for
-// each unvisited v of the word, the path counts of its in-neighbours at
-// depth L-1 (bit set in the depth-(L-1) bitmap, which stays in L2), with no
-// atomics: rows of up to kBcLight in-edges take an 8-lane group each,
-// longer rows the whole wavefront.  A nonzero sum makes v depth L with that
-// count; the wavefront writes the word of the depth-L bitmap.  The
-// successor bits of the level are then set from the frontier's side by the
-// same edge-balanced pass as the top-down level's, testing v against the
-// depth-L bitmap.
+// each unvisited v of the word, the path counts of its in-neighbours on the
+// frontier, with no atomics: rows of up to kBcLight in-edges take an 8-lane
+// group each, longer rows the whole wavefront.  An in-neighbour u is on
+// the frontier iff its bit is set in the frontier's rank words (per 64
+// vertices: the bits of queued vertices and the count queued before them,
+// N/4 bytes, resident in L2), and its path count is read from the compact
+// copy beside the queue at u's rank -- an in-neighbour has an out-edge, so
+// a frontier in-neighbour is always queued.  A nonzero sum makes v depth L
+// with that count; the wavefront writes the word of the depth-L bitmap.
+// The successor bits of the level are then set from the frontier's side by
+// the same edge-balanced pass as the top-down level's, testing v against
+// the depth-L bitmap.
 //
 // After either direction the depth-L bitmap is compacted in ascending
-// order into the next queue and its degree prefix (a count pass and a
-// write pass over contiguous word ranges), and the next frontier's sizes
-// reach the host through mapped memory: one synchronisation per level.
+// order into the next queue, its degree prefix, rank words and compact
+// path counts: a count pass and a write pass over kBcWW words per
+// wavefront, with a one-workgroup scan of the wavefronts' counts between
+// them.  The write pass clears the bitmap behind it.
+//
+// The level loop runs on the device: the scan also updates a control
+// block (the frontier's sizes, the level, the unvisited out-edges R and
+// the direction of the next level), and every kernel reads it at start,
+// so a level is a fixed sequence of launches -- pull, successor pass,
+// top-down pass (the direction not taken returns at once), count, scan,
+// write -- and kBcLevels levels are captured in one graph.  Levels after
+// the last frontier return at once.  The host launches the graph until
+// the control block, mirrored to mapped memory, says the search is done.
 //
 // Successor bits are set with one atomicOr per run of lanes whose edge
 // positions are consecutive inside a bitmap word.  Depths, counts and bits
@@ -56,8 +70,32 @@ constexpr int kBcChunk = kCThreads * kBcItems;
 constexpr int kBcGroup = 8;       // lanes per light row in the pull level
 constexpr int kBcPullUnroll = 4;  // in-edges in flight per lane
 constexpr int kBcLight = 64;      // longest row an 8-lane group takes
-constexpr int kBcParts = 1024;    // most workgroups of the compaction
-constexpr int kBcPartWidth = 4;   // found, found with out-degree >= 1, their out-degree sum, pad
+constexpr int kBcWW = 16;         // bitmap words per wavefront in the compaction
+constexpr int kBcPartWidth = 4;   // per such wavefront: found, queued (out-degree >= 1), their out-degree sum, pad
+constexpr int kBcScanThreads = 1024;
+constexpr int kBcLevels = 8;      // levels per captured graph
+
+// The level loop's state (device memory; see the file comment).
+struct BcCtl {
+  long long F;      // vertices of the frontier
+  long long T;      // its out-edges
+  long long Fq;     // its vertices of out-degree >= 1 (the queue)
+  long long R;      // out-edges of the vertices not yet visited
+  int level;        // the level being expanded (the frontier has depth level - 1)
+  int levels;       // frontiers with at least one vertex so far
+  int pull;         // this level runs bottom-up
+  int done;         // no level is left to run
+  int scans;        // scans run: the write pass after a scan runs once,
+  int writes;       // while writes (set by the next count pass) trails scans
+  int pad[2];
+};
+
+// The frontier's rank word for vertices [64 w, 64 w + 64): the bits of its
+// queued vertices and how many are queued before vertex 64 w.
+struct BcWord {
+  unsigned long long bits;
+  long long pre;
+};
 
 int grid_of(int64_t items) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(65536, (items + kCThreads - 1) / kCThreads)));
@@ -71,26 +109,45 @@ __global__ __launch_bounds__(kCThreads) void k_bc_init(int64_t n, int64_t words,
                                                        unsigned long long* __restrict__ pc,
                                                        unsigned long long* __restrict__ succ,
                                                        int32_t* __restrict__ queue, int64_t* __restrict__ qoff,
-                                                       unsigned long long* __restrict__ bm, int64_t nw,
-                                                       unsigned long long* __restrict__ sizes) {
+                                                       unsigned long long* __restrict__ pcq,
+                                                       BcWord* __restrict__ rank, unsigned long long* __restrict__ bm,
+                                                       int64_t nw, int64_t m, BcCtl* __restrict__ ctl,
+                                                       unsigned long long* __restrict__ host) {
   const int64_t gtid = int64_t(blockIdx.x) * kCThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kCThreads;
+  const int64_t d = off[source + 1] - off[source];
   for (int64_t v = gtid; v < n; v += gsize) {
     depth[v] = v == source ? 0 : -1;
     pc[v] = v == source ? 1ull : 0ull;
   }
   for (int64_t w = gtid; w < words; w += gsize) succ[w] = 0;
-  // Level bitmaps: [0] holds depth 0 (the source), [1] is cleared.
-  for (int64_t w = gtid; w < 2 * nw; w += gsize)
-    bm[w] = w == (source >> 6) ? 1ull << (source & 63) : 0ull;
+  // The depth-1 bitmap starts clear; the frontier (the source) is queued
+  // if it has an out-edge.
+  for (int64_t w = gtid; w < nw; w += gsize) {
+    bm[w] = 0;
+    const bool mine = w == (source >> 6) && d > 0;
+    rank[w] = BcWord{mine ? 1ull << (source & 63) : 0ull, (d > 0 && w > (source >> 6)) ? 1 : 0};
+  }
   if (gtid == 0) {
-    const int64_t d = off[source + 1] - off[source];
     queue[0] = source;
+    pcq[0] = 1;
     qoff[0] = 0;
     qoff[d > 0 ? 1 : 0] = d;
-    sizes[0] = 1;
-    sizes[1] = static_cast<unsigned long long>(d);
-    sizes[2] = d > 0 ? 1 : 0;
+    BcCtl c;
+    c.F = 1;
+    c.T = d;
+    c.Fq = d > 0 ? 1 : 0;
+    c.R = m - d;
+    c.level = 1;
+    c.levels = 1;
+    c.pull = d * kBcPull > c.R ? 1 : 0;
+    c.done = d == 0 ? 1 : 0;
+    c.scans = c.writes = 0;
+    c.pad[0] = c.pad[1] = 0;
+    *ctl = c;
+    host[0] = static_cast<unsigned long long>(c.done);
+    host[1] = 1;
+    host[2] = 1;
   }
 }
 
@@ -126,8 +183,9 @@ template <bool kTopDown>
 __global__ __launch_bounds__(kCThreads) void k_bc_expand(const int64_t* __restrict__ off,
                                                          const int32_t* __restrict__ nbr,
                                                          const int32_t* __restrict__ queue,
-                                                         const int64_t* __restrict__ qoff, int64_t F, int64_t T,
-                                                         int32_t level, int32_t* __restrict__ depth,
+                                                         const int64_t* __restrict__ qoff,
+                                                         const unsigned long long* __restrict__ pcq,
+                                                         const BcCtl* __restrict__ ctl, int32_t* __restrict__ depth,
                                                          unsigned long long* __restrict__ pc,
                                                          unsigned long long* __restrict__ succ,
                                                          unsigned long long* __restrict__ fnew) {
@@ -135,6 +193,9 @@ __global__ __launch_bounds__(kCThreads) void k_bc_expand(const int64_t* __restri
   __shared__ int64_t base[kBcChunk];   // out_off[u] - qoff[entry]: edge e sits at base + e
   __shared__ unsigned long long pcu[kTopDown ? kBcChunk : 1];
   __shared__ int64_t ie[2];
+  if (ctl->done || ctl->pull == (kTopDown ? 1 : 0)) return;
+  const int64_t F = ctl->Fq, T = ctl->T;
+  const int32_t level = ctl->level;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   for (int64_t e0 = int64_t(blockIdx.x) * kBcChunk; e0 < T; e0 += int64_t(gridDim.x) * kBcChunk) {
@@ -151,7 +212,7 @@ __global__ __launch_bounds__(kCThreads) void k_bc_expand(const int64_t* __restri
       const int32_t u = queue[i0 + i];
       rel[i] = static_cast<int32_t>(max(qo - e0, int64_t(0)));
       base[i] = off[u] - qo;
-      if (kTopDown) pcu[i] = pc[u];
+      if (kTopDown) pcu[i] = pcq[i0 + i];
     }
     __syncthreads();
     int64_t pos[kBcItems];
@@ -222,14 +283,35 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int r) {
   return p;
 }
 
+// The path counts of the frontier in-neighbours among u[] (-1: none),
+// added to sum; any is set if one is on the frontier.
+__device__ __forceinline__ void pull_terms(const int32_t* u, const BcWord* __restrict__ rank,
+                                           const unsigned long long* __restrict__ pcq, unsigned long long& sum,
+                                           int& any) {
+  BcWord q[kBcPullUnroll];
+#pragma unroll
+  for (int j = 0; j < kBcPullUnroll; j++) q[j] = u[j] >= 0 ? rank[u[j] >> 6] : BcWord{0ull, 0};
+#pragma unroll
+  for (int j = 0; j < kBcPullUnroll; j++) {
+    const int b = u[j] & 63;
+    if ((q[j].bits >> b) & 1ull) {
+      sum += pcq[q[j].pre + __popcll(q[j].bits & ((1ull << b) - 1))];
+      any = 1;
+    }
+  }
+}
+
 // Bottom-up level (see the file comment): wavefront per word w of the
 // depth-L bitmap fnew, which it writes whole.
 __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict__ in_off,
                                                        const int32_t* __restrict__ in_nbr, int64_t n, int64_t nw,
-                                                       int32_t level, int32_t* __restrict__ depth,
+                                                       const BcCtl* __restrict__ ctl, int32_t* __restrict__ depth,
                                                        unsigned long long* __restrict__ pc,
-                                                       const unsigned long long* __restrict__ fprev,
+                                                       const BcWord* __restrict__ rank,
+                                                       const unsigned long long* __restrict__ pcq,
                                                        unsigned long long* __restrict__ fnew) {
+  if (ctl->done || !ctl->pull) return;
+  const int32_t level = ctl->level;
   const int lane = lane_id();
   const int sub = lane & (kBcGroup - 1);
   const int grp = lane / kBcGroup;
@@ -257,17 +339,9 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
       int any = 0;  // a parent found (the count itself may wrap to 0 mod 2^64)
       for (int64_t e = b + sub; e < end; e += kBcGroup * kBcPullUnroll) {
         int32_t u[kBcPullUnroll];
-        bool f[kBcPullUnroll];
 #pragma unroll
         for (int j = 0; j < kBcPullUnroll; j++) u[j] = e + j * kBcGroup < end ? in_nbr[e + j * kBcGroup] : -1;
-#pragma unroll
-        for (int j = 0; j < kBcPullUnroll; j++) f[j] = u[j] >= 0 && ((fprev[u[j] >> 6] >> (u[j] & 63)) & 1ull);
-#pragma unroll
-        for (int j = 0; j < kBcPullUnroll; j++)
-          if (f[j]) {
-            sum += pc[u[j]];
-            any = 1;
-          }
+        pull_terms(u, rank, pcq, sum, any);
       }
 #pragma unroll
       for (int dd = kBcGroup / 2; dd >= 1; dd >>= 1) {
@@ -289,17 +363,9 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
       int any = 0;
       for (int64_t e = b + lane; e < end; e += 64 * kBcPullUnroll) {
         int32_t u[kBcPullUnroll];
-        bool f[kBcPullUnroll];
 #pragma unroll
         for (int j = 0; j < kBcPullUnroll; j++) u[j] = e + j * 64 < end ? in_nbr[e + j * 64] : -1;
-#pragma unroll
-        for (int j = 0; j < kBcPullUnroll; j++) f[j] = u[j] >= 0 && ((fprev[u[j] >> 6] >> (u[j] & 63)) & 1ull);
-#pragma unroll
-        for (int j = 0; j < kBcPullUnroll; j++)
-          if (f[j]) {
-            sum += pc[u[j]];
-            any = 1;
-          }
+        pull_terms(u, rank, pcq, sum, any);
       }
       sum = wave_sum(sum);
       if (__ballot(any)) {
@@ -316,115 +382,151 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
   }
 }
 
-// Compaction of the depth-L bitmap, first pass: per workgroup (contiguous
-// words [b * wpb, (b + 1) * wpb)) the vertices found, those of out-degree
-// >= 1 and their out-degree sum.  Also clears the depth-(L-1) bitmap,
-// which the next level fills as its depth-(L+1) bitmap.
+// Compaction of the depth-L bitmap, first pass: wavefront g takes words
+// [g * kBcWW, (g + 1) * kBcWW) and counts the vertices found there, those
+// of out-degree >= 1 (queued) and their out-degree sum.
 __global__ __launch_bounds__(kCThreads) void k_bc_count(const int64_t* __restrict__ off,
-                                                        const unsigned long long* __restrict__ fnew,
-                                                        unsigned long long* __restrict__ fclear, int64_t nw,
-                                                        int64_t wpb, int64_t* __restrict__ part) {
-  __shared__ int64_t red[3][kCWaves];
+                                                        const unsigned long long* __restrict__ fnew, int64_t nw,
+                                                        BcCtl* __restrict__ ctl, int64_t* __restrict__ part) {
+  // The previous level's write pass has run (stream order).
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->writes = ctl->scans;
+  if (ctl->done) return;
   const int lane = lane_id();
-  const int wv = threadIdx.x >> 6;
-  int64_t c_all = 0, c_pos = 0, c_deg = 0;  // uniform in the wavefront
-  const int64_t wb = int64_t(blockIdx.x) * wpb, we = min(nw, wb + wpb);
-  for (int64_t w = wb + wv; w < we; w += kCWaves) {
-    const unsigned long long word = fnew[w];
-    if (lane == 0) fclear[w] = 0;
-    if (word) {
-      const int64_t v = w * 64 + lane;
-      const int64_t d = (word >> lane) & 1ull ? off[v + 1] - off[v] : 0;
-      c_all += __popcll(word);
-      c_pos += __popcll(__ballot(d > 0));
-      c_deg += wave_sum(d);
-    }
+  const int64_t g = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
+  const int64_t w0 = g * kBcWW;
+  if (w0 >= nw) return;  // whole wavefronts
+  const unsigned long long mine = lane < kBcWW && w0 + lane < nw ? fnew[w0 + lane] : 0ull;
+  unsigned long long word[kBcWW];
+  int64_t d[kBcWW];
+#pragma unroll
+  for (int j = 0; j < kBcWW; j++) word[j] = __shfl(mine, j);
+#pragma unroll
+  for (int j = 0; j < kBcWW; j++) {
+    const int64_t v = (w0 + j) * 64 + lane;
+    d[j] = (word[j] >> lane) & 1ull ? off[v + 1] - off[v] : 0;
   }
+  int64_t c_all = 0, c_pos = 0, c_deg = 0;
+#pragma unroll
+  for (int j = 0; j < kBcWW; j++) {
+    c_all += __popcll(word[j]);
+    c_pos += __popcll(__ballot(d[j] > 0));
+    c_deg += d[j];
+  }
+  c_deg = wave_sum(c_deg);
   if (lane == 0) {
-    red[0][wv] = c_all;
-    red[1][wv] = c_pos;
-    red[2][wv] = c_deg;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    int64_t t = 0;
-    for (int i = 0; i < kCWaves; i++) t += red[threadIdx.x][i];
-    part[blockIdx.x * kBcPartWidth + threadIdx.x] = t;
+    part[g * kBcPartWidth] = c_all;
+    part[g * kBcPartWidth + 1] = c_pos;
+    part[g * kBcPartWidth + 2] = c_deg;
   }
 }
 
-// Second pass: the queue (ascending vertices of out-degree >= 1) and its
-// exclusive degree prefix, at the workgroup's offsets from the partials
-// of the workgroups before it.  Workgroup 0 writes qoff[F] = T and the
-// sizes for the host.
-__global__ __launch_bounds__(kCThreads) void k_bc_write(const int64_t* __restrict__ off,
-                                                        const unsigned long long* __restrict__ fnew, int64_t nw,
-                                                        int64_t wpb, const int64_t* __restrict__ part, int nparts,
-                                                        int32_t* __restrict__ queue, int64_t* __restrict__ qoff,
-                                                        unsigned long long* __restrict__ sizes) {
-  __shared__ int64_t red[5][kCWaves];
-  __shared__ int64_t wsc[2][kCWaves];
+// One workgroup: exclusive prefixes of the wavefronts' queued counts and
+// degree sums (in place), qoff[F] = T, and the control block's next level
+// (mirrored to the host's flags).
+__global__ __launch_bounds__(kBcScanThreads) void k_bc_scan(int64_t* __restrict__ part, int64_t nparts,
+                                                            int64_t* __restrict__ qoff, BcCtl* __restrict__ ctl,
+                                                            unsigned long long* __restrict__ host) {
+  __shared__ int64_t wt[3][kBcScanThreads / 64];
+  if (ctl->done) return;
   const int lane = lane_id();
-  const int wv = threadIdx.x >> 6;
-  int64_t acc[5] = {0, 0, 0, 0, 0};  // prefix pos, prefix degrees, total found, total pos, total degrees
-  for (int j = threadIdx.x; j < nparts; j += kCThreads) {
-    const int64_t a = part[j * kBcPartWidth], b = part[j * kBcPartWidth + 1], c = part[j * kBcPartWidth + 2];
-    if (j < static_cast<int>(blockIdx.x)) {
-      acc[0] += b;
-      acc[1] += c;
-    }
-    acc[2] += a;
-    acc[3] += b;
-    acc[4] += c;
+  const int wave = threadIdx.x >> 6;
+  const int64_t per = (nparts + kBcScanThreads - 1) / kBcScanThreads;
+  const int64_t j0 = min(nparts, int64_t(threadIdx.x) * per), j1 = min(nparts, j0 + per);
+  int64_t a = 0, p = 0, dsum = 0;
+#pragma unroll 4
+  for (int64_t j = j0; j < j1; j++) {
+    a += part[j * kBcPartWidth];
+    p += part[j * kBcPartWidth + 1];
+    dsum += part[j * kBcPartWidth + 2];
   }
-#pragma unroll
-  for (int k = 0; k < 5; k++) {
-    const int64_t s = wave_sum(acc[k]);
-    if (lane == 0) red[k][wv] = s;
+  const int64_t ia = wave_incl_scan(a), ip = wave_incl_scan(p), id = wave_incl_scan(dsum);
+  if (lane == 63) {
+    wt[0][wave] = ia;
+    wt[1][wave] = ip;
+    wt[2][wave] = id;
   }
   __syncthreads();
+  int64_t ta = 0, tp = 0, td = 0, ep = ip - p, ed = id - dsum;
+  for (int w = 0; w < kBcScanThreads / 64; w++) {
+    if (w < wave) {
+      ep += wt[1][w];
+      ed += wt[2][w];
+    }
+    ta += wt[0][w];
+    tp += wt[1][w];
+    td += wt[2][w];
+  }
+#pragma unroll 4
+  for (int64_t j = j0; j < j1; j++) {
+    const int64_t pj = part[j * kBcPartWidth + 1], dj = part[j * kBcPartWidth + 2];
+    part[j * kBcPartWidth + 1] = ep;
+    part[j * kBcPartWidth + 2] = ed;
+    ep += pj;
+    ed += dj;
+  }
+  if (threadIdx.x == 0) {
+    qoff[tp] = td;
+    BcCtl c = *ctl;
+    c.F = ta;
+    c.T = td;
+    c.Fq = tp;
+    c.R -= td;
+    c.level += 1;
+    c.levels += ta > 0 ? 1 : 0;
+    c.pull = td * kBcPull > c.R ? 1 : 0;
+    c.done = (ta == 0 || td == 0) ? 1 : 0;
+    c.scans += 1;
+    *ctl = c;
+    host[0] = static_cast<unsigned long long>(c.done);
+    host[1] = static_cast<unsigned long long>(c.levels);
+    host[2] = static_cast<unsigned long long>(ta);
+  }
+}
+
+// Second pass, same words per wavefront: the queue (ascending vertices of
+// out-degree >= 1), its degree prefix, the compact path counts and the
+// rank words, from the wavefront's prefixes; then the words are cleared
+// for the next level.
+__global__ __launch_bounds__(kCThreads) void k_bc_write(const int64_t* __restrict__ off,
+                                                        unsigned long long* __restrict__ fnew, int64_t nw,
+                                                        const int64_t* __restrict__ part,
+                                                        const unsigned long long* __restrict__ pc,
+                                                        int32_t* __restrict__ queue, int64_t* __restrict__ qoff,
+                                                        unsigned long long* __restrict__ pcq,
+                                                        BcWord* __restrict__ rank, const BcCtl* __restrict__ ctl) {
+  // Runs once after each scan, also after the one that ended the search.
+  if (ctl->scans == ctl->writes) return;
+  const int lane = lane_id();
+  const int64_t g = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
+  const int64_t w0 = g * kBcWW;
+  if (w0 >= nw) return;  // whole wavefronts
+  const bool own = lane < kBcWW && w0 + lane < nw;
+  const unsigned long long mine = own ? fnew[w0 + lane] : 0ull;
+  if (own) fnew[w0 + lane] = 0;
+  int64_t pos = part[g * kBcPartWidth + 1], deg = part[g * kBcPartWidth + 2];
+  unsigned long long word[kBcWW];
+  int64_t d[kBcWW];
 #pragma unroll
-  for (int k = 0; k < 5; k++) {
-    acc[k] = 0;
-    for (int i = 0; i < kCWaves; i++) acc[k] += red[k][i];
+  for (int j = 0; j < kBcWW; j++) word[j] = __shfl(mine, j);
+#pragma unroll
+  for (int j = 0; j < kBcWW; j++) {
+    const int64_t v = (w0 + j) * 64 + lane;
+    d[j] = (word[j] >> lane) & 1ull ? off[v + 1] - off[v] : 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    qoff[acc[3]] = acc[4];
-    sizes[0] = static_cast<unsigned long long>(acc[2]);
-    sizes[1] = static_cast<unsigned long long>(acc[4]);
-    sizes[2] = static_cast<unsigned long long>(acc[3]);
-  }
-  int64_t run_pos = acc[0], run_deg = acc[1];
-  const int64_t wb = int64_t(blockIdx.x) * wpb, we = min(nw, wb + wpb);
-  for (int64_t w0 = wb; w0 < we; w0 += kCWaves) {
-    const int64_t w = w0 + wv;
-    const unsigned long long word = w < we ? fnew[w] : 0ull;
-    const int64_t v = w * 64 + lane;
-    const int64_t d = (word >> lane) & 1ull ? off[v + 1] - off[v] : 0;
-    const unsigned long long pm = __ballot(d > 0);
-    const int64_t dinc = wave_incl_scan(d);
-    if (lane == 63) {
-      wsc[0][wv] = __popcll(pm);
-      wsc[1][wv] = dinc;
-    }
-    __syncthreads();
-    int64_t bpos = run_pos, bdeg = run_deg, tpos = 0, tdeg = 0;
-    for (int i = 0; i < kCWaves; i++) {
-      if (i < wv) {
-        bpos += wsc[0][i];
-        bdeg += wsc[1][i];
-      }
-      tpos += wsc[0][i];
-      tdeg += wsc[1][i];
-    }
-    if (d > 0) {
-      const int64_t at = bpos + lane_rank(pm);
+#pragma unroll
+  for (int j = 0; j < kBcWW; j++) {
+    const int64_t v = (w0 + j) * 64 + lane;
+    const unsigned long long pm = __ballot(d[j] > 0);
+    const int64_t dinc = wave_incl_scan(d[j]);
+    if (d[j] > 0) {
+      const int64_t at = pos + lane_rank(pm);
       queue[at] = static_cast<int32_t>(v);
-      qoff[at] = bdeg + dinc - d;
+      qoff[at] = deg + dinc - d[j];
+      pcq[at] = pc[v];
     }
-    run_pos += tpos;
-    run_deg += tdeg;
-    __syncthreads();
+    if (lane == 0 && w0 + j < nw) rank[w0 + j] = BcWord{pm, static_cast<long long>(pos)};
+    pos += __popcll(pm);
+    deg += __shfl(dinc, 63);
   }
 }
 
@@ -436,6 +538,7 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
   const int64_t m = ctx->m_out;
   const int64_t words = (m + 63) / 64;
   const int64_t nw = (n + 63) / 64;
+  const int64_t nparts = std::max<int64_t>(1, ceil_div(nw, kBcWW));  // compaction wavefronts
   int rc;
   if (!ctx->d_bc_depth) {
     if ((rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_depth), sizeof(int32_t) * n, "BC depths")) ||
@@ -443,71 +546,71 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_succ), sizeof(uint64_t) * std::max<int64_t>(1, words),
                         "BC successor bitmap")) ||
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_queue), sizeof(int32_t) * n, "BC queue")) ||
-        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_part), sizeof(int64_t) * kBcPartWidth * kBcParts,
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_part), sizeof(int64_t) * kBcPartWidth * nparts,
                         "BC compaction partials")) ||
         (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_qoff), sizeof(int64_t) * (n + 1), "BC prefix")) ||
-        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_bm), sizeof(uint64_t) * 2 * nw,
-                        "BC level bitmaps")))
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_pcq), sizeof(uint64_t) * n,
+                        "BC frontier path counts")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_rank), sizeof(BcWord) * nw, "BC frontier ranks")) ||
+        (rc = dev_alloc(ctx, &ctx->d_bc_ctl, sizeof(BcCtl), "BC level control")) ||
+        (rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_bm), sizeof(uint64_t) * nw, "BC level bitmap")))
       return rc;
   }
   hipStream_t s = ctx->stream;
   int32_t* d_depth = ctx->d_bc_depth;
   unsigned long long* d_pc = reinterpret_cast<unsigned long long*>(ctx->d_bc_pc);
+  unsigned long long* d_pcq = reinterpret_cast<unsigned long long*>(ctx->d_bc_pcq);
+  BcWord* d_rank = static_cast<BcWord*>(ctx->d_bc_rank);
+  BcCtl* d_ctl = static_cast<BcCtl*>(ctx->d_bc_ctl);
   unsigned long long* d_succ = reinterpret_cast<unsigned long long*>(ctx->d_bc_succ);
-  unsigned long long* d_bm = reinterpret_cast<unsigned long long*>(ctx->d_bc_bm);
-  unsigned long long* d_sizes = reinterpret_cast<unsigned long long*>(ctx->hd_bc_sizes);
-  volatile uint64_t* h_sizes = ctx->h_bc_sizes;
+  unsigned long long* fnew = reinterpret_cast<unsigned long long*>(ctx->d_bc_bm);  // depth L, clear between levels
+  unsigned long long* d_host = reinterpret_cast<unsigned long long*>(ctx->hd_bc_sizes);
+  volatile uint64_t* h_host = ctx->h_bc_sizes;
   ctx->bc_valid = false;
-  hipLaunchKernelGGL(k_bc_init, dim3(grid_of(std::max(std::max(n, words), 2 * nw))), dim3(kCThreads), 0, s, n, words,
-                     source, ctx->d_out_off, d_depth, d_pc, d_succ, ctx->d_bc_queue, ctx->d_bc_qoff, d_bm, nw,
-                     d_sizes);
-  DMM_HIP_CHECK(ctx, hipGetLastError());
-  DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  int64_t F = static_cast<int64_t>(h_sizes[0]);
-  int64_t T = static_cast<int64_t>(h_sizes[1]);
-  int64_t Fq = static_cast<int64_t>(h_sizes[2]);
-  const int cap = std::max(1, ctx->cu_count) * 8;
-  const int nparts = static_cast<int>(std::min<int64_t>(kBcParts, std::max<int64_t>(1, ceil_div(nw, 16))));
-  const int64_t wpb = ceil_div(nw, nparts);
-  int32_t level = 1;
-  // Out-edges of the unvisited vertices (the pull level's scan bound).
-  int64_t unvisited_edges = m;
-  while (F > 0) {
-    // The frontier's out-edges leave the unvisited set's count (each
-    // vertex is in one frontier, the level it is visited).
-    unvisited_edges -= T;
-    if (T == 0) {
-      F = 0;  // no out-edges: nothing is found
-    } else {
-      // fprev: depth L-1 (the frontier); fnew: depth L, cleared by the
-      // previous level's count pass (by k_bc_init for level 1).
-      unsigned long long* fprev = d_bm + ((level - 1) & 1) * nw;
-      unsigned long long* fnew = d_bm + (level & 1) * nw;
-      const int egrid = static_cast<int>(std::min<int64_t>(cap, ceil_div(T, kBcChunk)));
-      if (T * kBcPull > unvisited_edges) {
-        const int pgrid = static_cast<int>(std::min<int64_t>(cap, ceil_div(nw, kCWaves)));
-        hipLaunchKernelGGL(k_bc_pull, dim3(pgrid), dim3(kCThreads), 0, s, ctx->d_in_off, ctx->d_in_nbr, n, nw, level,
-                           d_depth, d_pc, fprev, fnew);
-        hipLaunchKernelGGL(k_bc_expand<false>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
-                           ctx->d_bc_queue, ctx->d_bc_qoff, Fq, T, level, d_depth, d_pc, d_succ, fnew);
-      } else {
-        hipLaunchKernelGGL(k_bc_expand<true>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
-                           ctx->d_bc_queue, ctx->d_bc_qoff, Fq, T, level, d_depth, d_pc, d_succ, fnew);
-      }
-      hipLaunchKernelGGL(k_bc_count, dim3(nparts), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, fprev, nw, wpb,
+  if (!ctx->bc_graph) {
+    // kBcLevels levels, each the fixed launch sequence of the file comment.
+    const int cap = std::max(1, ctx->cu_count) * 8;
+    const int pgrid = static_cast<int>(std::min<int64_t>(cap, ceil_div(nw, kCWaves)));
+    const int egrid = static_cast<int>(std::min<int64_t>(cap, ceil_div(m, kBcChunk)));
+    const int cgrid = static_cast<int>(ceil_div(nparts, kCWaves));
+    hipGraph_t g = nullptr;
+    DMM_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kBcLevels; k++) {
+      hipLaunchKernelGGL(k_bc_pull, dim3(pgrid), dim3(kCThreads), 0, s, ctx->d_in_off, ctx->d_in_nbr, n, nw, d_ctl,
+                         d_depth, d_pc, d_rank, d_pcq, fnew);
+      hipLaunchKernelGGL(k_bc_expand<false>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
+                         ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_ctl, d_depth, d_pc, d_succ, fnew);
+      hipLaunchKernelGGL(k_bc_expand<true>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
+                         ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_ctl, d_depth, d_pc, d_succ, fnew);
+      hipLaunchKernelGGL(k_bc_count, dim3(cgrid), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, nw, d_ctl,
                          ctx->d_bc_part);
-      hipLaunchKernelGGL(k_bc_write, dim3(nparts), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, nw, wpb,
-                         ctx->d_bc_part, nparts, ctx->d_bc_queue, ctx->d_bc_qoff, d_sizes);
-      DMM_HIP_CHECK(ctx, hipGetLastError());
-      DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
-      F = static_cast<int64_t>(h_sizes[0]);
-      T = static_cast<int64_t>(h_sizes[1]);
-      Fq = static_cast<int64_t>(h_sizes[2]);
-      if (F > n || Fq > F || T > m) return set_err(ctx, DMM_ERR_STATE, "BC frontier sizes out of range");
+      hipLaunchKernelGGL(k_bc_scan, dim3(1), dim3(kBcScanThreads), 0, s, ctx->d_bc_part, nparts, ctx->d_bc_qoff,
+                         d_ctl, d_host);
+      hipLaunchKernelGGL(k_bc_write, dim3(cgrid), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, nw, ctx->d_bc_part,
+                         d_pc, ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_rank, d_ctl);
     }
-    level++;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "BC capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->bc_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(ctx, DMM_ERR_HIP, "BC instantiate failed: %s", hipGetErrorString(e));
   }
-  if (num_levels) *num_levels = level - 1;
+  hipLaunchKernelGGL(k_bc_init, dim3(grid_of(std::max(n, words))), dim3(kCThreads), 0, s, n, words, source,
+                     ctx->d_out_off, d_depth, d_pc, d_succ, ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_rank, fnew, nw,
+                     m, d_ctl, d_host);
+  DMM_HIP_CHECK(ctx, hipGetLastError());
+  // Every level adds at least one vertex to the visited set, so a search
+  // ends within n levels; the bound only guards the host loop.
+  const int64_t max_launches = n / kBcLevels + 2;
+  for (int64_t i = 0;; i++) {
+    if (i >= max_launches) return set_err(ctx, DMM_ERR_STATE, "BC level loop did not end");
+    DMM_HIP_CHECK(ctx, hipGraphLaunch(ctx->bc_graph, s));
+    DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (h_host[0]) break;
+  }
+  const int64_t levels = static_cast<int64_t>(h_host[1]);
+  if (levels < 1 || levels > n) return set_err(ctx, DMM_ERR_STATE, "BC level count out of range");
+  if (num_levels) *num_levels = static_cast<int32_t>(levels);
   ctx->bc_valid = true;
   return bc_read(ctx, depth, path_counts, succ);
 }

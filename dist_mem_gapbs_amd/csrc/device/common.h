@@ -166,7 +166,10 @@ struct dmm_ctx {
   int32_t* d_bc_queue = nullptr;  // the frontier's vertices of out-degree >= 1, ascending
   int64_t* d_bc_part = nullptr;   // per-workgroup counts of the frontier compaction
   int64_t* d_bc_qoff = nullptr;   // exclusive out-degree prefix of d_bc_queue
-  uint64_t* d_bc_bm = nullptr;    // two level bitmaps (depth L-1, depth L)
+  uint64_t* d_bc_pcq = nullptr;   // path counts of d_bc_queue's vertices
+  void* d_bc_rank = nullptr;      // per 64 vertices: queued bits, queued before
+  void* d_bc_ctl = nullptr;       // the device-side level loop's state
+  uint64_t* d_bc_bm = nullptr;    // the depth-L bitmap of the level being expanded
   uint64_t* h_bc_sizes = nullptr;   // pinned, mapped: the next frontier's sizes
   uint64_t* hd_bc_sizes = nullptr;  // device address of h_bc_sizes
   bool bc_valid = false;            // a Brandes forward result is held
@@ -239,6 +242,7 @@ struct dmm_ctx {
   void* d_prparams = nullptr;
   void* h_prparams = nullptr;
   hipGraphExec_t pr_graph = nullptr;
+  hipGraphExec_t bc_graph = nullptr;  // Brandes: a fixed number of levels
   int pr_graph_key = -1;
   int pr_mode = -1;
 
