@@ -184,6 +184,7 @@ SIGNATURES = {
     "dmm_pr_block_iteration": (_I32, [_P, _PF32, _PF32, _I32, _PD, _PD]),
     "dmm_pr_block_iteration_noalloc": (_I32, [_P, _P, _P, _P, _I32, _P]),
     "dmm_pr_tiles_check": (_I32, [_I64, _I64, _I64, _PI64, _PI32, _I64, _PF32, _PF32, _PI64]),
+    "dmm_pr_blocks_check": (_I32, [_I64, _I64, _PI64, _PI32, _PF32, _PF32, _PI64]),
     "dmm_autotune": (_I32, [_P, _PI32, _I32, _I32, C.POINTER(Tuning)]),
     "dmm_set_tuning": (_I32, [_P, C.POINTER(Tuning)]),
     "dmm_get_tuning": (_I32, [_P, C.POINTER(Tuning)]),

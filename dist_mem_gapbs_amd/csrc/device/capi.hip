@@ -103,11 +103,24 @@ void drop_tiles(dmm_ctx* ctx) {
   ctx->n_windows = 0;
   if (ctx->tiles_ready) drop_graphs(ctx);
   ctx->tiles_ready = false;
+  // The PE-block propagation layout.
+  void* bbufs[] = {ctx->d_bp_src, ctx->d_bp_wstart, ctx->d_bp_wlen, ctx->d_bp_wseg, ctx->d_bp_seg_e,
+                   ctx->d_bp_seg_d, ctx->d_bp_brow, ctx->d_bp_units, ctx->d_bp_splits, ctx->d_bp_val,
+                   ctx->d_bp_part};
+  for (void* p : bbufs) dmm::dev_free(p);
+  ctx->dev_bytes -= ctx->bp_bytes;
+  ctx->bp_bytes = 0;
+  ctx->d_bp_src = ctx->d_bp_wstart = ctx->d_bp_wlen = ctx->d_bp_wseg = ctx->d_bp_seg_e = nullptr;
+  ctx->d_bp_seg_d = ctx->d_bp_brow = ctx->d_bp_units = ctx->d_bp_splits = ctx->d_bp_val = nullptr;
+  ctx->d_bp_part = nullptr;
+  ctx->n_bp_windows = ctx->n_bp_units = ctx->n_bp_splits = ctx->n_bp_buckets = 0;
+  ctx->bp_ready = false;
 }
 
 void free_graph_state(dmm_ctx* ctx) {
   drop_tiles(ctx);
   ctx->tiles_unusable = false;
+  ctx->bp_unusable = false;
   drop_graphs(ctx);
   void* bufs[] = {ctx->d_out_off, ctx->d_out_nbr, ctx->d_parent, ctx->d_depth, ctx->d_queue,
                   ctx->d_qoff,    ctx->d_bm[0],   ctx->d_bm[1],  ctx->d_bm[2],
@@ -263,14 +276,30 @@ int ensure_pr_state(dmm_ctx* ctx, int mode) {
     ctx->d_err_part = nullptr;
     drop_graphs(ctx);
   }
+  // Compensated iteration of a PE block: propagation layout (a block too
+  // large for it keeps the untiled kernels).
+  if (mode == DMM_PR_FP32_KAHAN && ctx->nloc < ctx->n && !ctx->relabeled && !ctx->bp_ready && !ctx->bp_unusable) {
+    rc = dmm::pr_blocks_build(ctx);
+    if (rc == DMM_ERR_CAPACITY) {
+      drop_tiles(ctx);
+      ctx->bp_unusable = true;
+    } else if (rc) {
+      return rc;
+    }
+    if (ctx->d_err_part) dmm::dev_free(ctx->d_err_part);  // resized below
+    ctx->d_err_part = nullptr;
+  }
   if (!ctx->d_err_part) {
     const int64_t parts =
         std::max<int64_t>(std::max<int64_t>(dmm::pr_light_blocks(ctx) + dmm::pr_heavy_blocks(ctx),
                                             dmm::pr_fused_blocks(ctx)),
                           dmm::pr_row_blocks(ctx)) +
         dmm::pr_super_blocks(ctx);
-    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * parts, "error partials"))) return rc;
-    ctx->err_part_cap = parts;
+    const int64_t bparts = dmm::pr_bsum_blocks(ctx) + dmm::pr_bsplit_blocks(ctx);
+    if ((rc = dmm::dev_alloc(ctx, (void**)&ctx->d_err_part, sizeof(double) * std::max(parts, bparts),
+                             "error partials")))
+      return rc;
+    ctx->err_part_cap = std::max(parts, bparts);
   }
   return DMM_OK;
 }

@@ -222,6 +222,12 @@ struct dmm_ctx {
   int64_t n_chunks = 0, n_pieces = 0, n_row_blocks = 0, tile_slots_filled = 0, tiles_window = 0;
   int64_t tiles_bytes = 0;          // device bytes the layout holds
   bool tiles_unusable = false;      // the graph does not fit the layout: untiled kernels
+  // Compensated PE-block iteration: propagation layout (pr_blocks.hip).
+  void *d_bp_src = nullptr, *d_bp_wstart = nullptr, *d_bp_wlen = nullptr, *d_bp_wseg = nullptr;
+  void *d_bp_seg_e = nullptr, *d_bp_seg_d = nullptr, *d_bp_brow = nullptr, *d_bp_units = nullptr;
+  void *d_bp_splits = nullptr, *d_bp_val = nullptr, *d_bp_part = nullptr;
+  int64_t n_bp_windows = 0, n_bp_units = 0, n_bp_splits = 0, n_bp_buckets = 0, bp_bytes = 0;
+  bool bp_ready = false, bp_unusable = false;
   int32_t n_windows = 0;
   // Rows of the light loop and the in-degree-0 tail (see the upload).
   int64_t pr_light_begin = 0, pr_light_end = 0, pr_zero_begin = 0;

@@ -77,11 +77,57 @@ int build_tiles_host(int64_t n, int64_t zb, int64_t n_gather, const int64_t* off
 // upload, at the first compensated call).  Returns DMM_OK or an error.
 int pr_tiles_build(dmm_ctx* ctx);
 
+// Compensated PageRank of a PE block (rows in the caller's order, sources
+// anywhere in the whole graph): propagation layout.  Pass 1 copies one
+// window of kBWin contributions into LDS at a time and writes each in-edge's
+// contribution to the edge's place in the bucket-major array; pass 2 sums
+// each bucket's rows exactly, as 128-bit fixed-point integers in LDS.
+constexpr int kBWin = 32768;             // sources per window (128 KB of LDS)
+constexpr int kBRows = 8192;             // rows per bucket (its accumulators fill 128 KB of LDS)
+constexpr int64_t kBEdges = 1 << 19;     // in-edges per bucket, and per unit of a one-row bucket
+constexpr int kBSegMax = 4000;           // buckets a window's segment table may hold
+constexpr int kBPer = 8;                 // edges per thread in pass 1 (window starts padded to it)
+struct BlockUnit {  // pass 2 work: bucket rows [row0, row0 + nrows), edges [e0, e1)
+  int32_t row0;
+  int32_t nrows;
+  int32_t slot;  // >= 0: a part of a one-row bucket, its sum goes to partial slot `slot`
+  int32_t pad;
+  int64_t e0, e1;
+};
+struct BlockSplit {  // a one-row bucket in parts: partial slots [first, first + count)
+  int32_t row;
+  int32_t first;
+  int32_t count;
+  int32_t pad;
+};
+struct BlockHost {
+  std::vector<uint16_t> src;       // window-major source offsets (window starts padded to kBPer)
+  std::vector<int64_t> wstart;     // window starts (+ end)
+  std::vector<int32_t> wlen;       // window edge counts
+  std::vector<int64_t> wseg;       // first segment of each window (+ end)
+  std::vector<int32_t> seg_e;      // segment start inside its window
+  std::vector<int32_t> seg_d;      // segment destination minus seg_e
+  std::vector<uint16_t> brow_off;  // bucket-major row offsets inside the bucket
+  std::vector<BlockUnit> units;
+  std::vector<BlockSplit> splits;
+  int32_t n_slots = 0, n_windows = 0;
+  int64_t n_buckets = 0;
+};
+// Builds the propagation layout of an in-CSR of n rows with sources in
+// [0, n_gather).  Returns DMM_OK or an error code (message in *err).
+int build_blocks_host(int64_t n, int64_t n_gather, const int64_t* off, const int32_t* nbr, BlockHost* out,
+                      std::string* err);
+// Builds it from the device CSR (host work, once per upload, at the first
+// compensated block iteration).
+int pr_blocks_build(dmm_ctx* ctx);
+
 int pr_light_blocks(dmm_ctx* ctx);
 int pr_heavy_blocks(dmm_ctx* ctx);
 int pr_super_blocks(dmm_ctx* ctx);
 int pr_fused_blocks(dmm_ctx* ctx);  // persistent KAHAN / FP64 kernel
 int pr_row_blocks(dmm_ctx* ctx);    // row kernel of the tiled compensated mode
+int pr_bsum_blocks(dmm_ctx* ctx);   // PE-block compensated mode: bucket sums
+int pr_bsplit_blocks(dmm_ctx* ctx); //   and the rows summed in parts
 // Size of the PrParams block (bytes) the ctx must hold at d_prparams.
 constexpr size_t kPrParamsBytes = 64;
 

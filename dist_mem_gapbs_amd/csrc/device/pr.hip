@@ -729,6 +729,270 @@ __global__ __launch_bounds__(kThreads) void k_f64_super(PrArgs a, double* __rest
   if (threadIdx.x == 0) a.err_part[a.main_blocks + blockIdx.x] = t;
 }
 
+// ---- FP32_KAHAN on a PE block: propagation layout, exact sums ---------------
+//
+// Pass 1 (k_bprop) gathers every in-edge's contribution from an LDS copy of
+// its source's window and writes it to the edge's place in the bucket-major
+// array; pass 2 (k_bsum) sums each bucket's rows.  The sums are exact: a
+// contribution c enters as the 128-bit two's complement integer
+// c * 2^kFxShift (exact for |c| >= 2^-57, truncated below 2^-kFxShift), so
+// the order of the additions does not matter and the LDS accumulators take
+// plain integer atomics; each row's sum is rounded to FP32 once.  Row sums
+// must stay below 2^47 in magnitude (PageRank contributions are in [0, 1]).
+
+constexpr int kBPropThreads = 1024;
+constexpr int kBSumThreads = 1024;
+constexpr int kFxShift = 80;
+
+struct BpArgs {
+  const uint16_t* src;
+  const int64_t* wstart;
+  const int32_t* wlen;
+  const int64_t* wseg;
+  const int32_t* seg_e;
+  const int32_t* seg_d;
+  const uint16_t* brow;
+  const BlockUnit* units;
+  const BlockSplit* splits;
+  float* val;
+  long long* part;  // (lo, hi) per partial slot
+  int32_t nwin;
+  int64_t n_units, n_splits;
+};
+
+// c * 2^kFxShift as a 128-bit two's complement integer (lo, hi).
+__device__ __forceinline__ void fx_of(float c, unsigned long long& lo, long long& hi) {
+  const unsigned bits = __float_as_uint(c);
+  const int e = static_cast<int>((bits >> 23) & 0xFFu);
+  const unsigned long long m = (bits & 0x7FFFFFu) | 0x800000u;
+  lo = 0;
+  hi = 0;
+  if (e == 0) return;  // zero or subnormal: below 2^-126
+  const int sh = e - 150 + kFxShift;  // |c| = m * 2^(e - 150)
+  if (sh <= -24) return;
+  if (sh < 0) {
+    lo = m >> (-sh);
+  } else if (sh < 64) {
+    lo = m << sh;
+    hi = sh > 40 ? static_cast<long long>(m >> (64 - sh)) : 0;
+  } else {
+    hi = static_cast<long long>(m << (sh - 64));
+  }
+  if (bits >> 31) {
+    lo = ~lo + 1;
+    hi = ~hi + (lo == 0 ? 1 : 0);
+  }
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long& lo, long long& hi, unsigned long long xlo, long long xhi) {
+  const unsigned long long s = lo + xlo;
+  hi += xhi + (s < lo ? 1 : 0);
+  lo = s;
+}
+
+// The FP32 rounding (to nearest) of the 128-bit integer (lo, hi) * 2^-kFxShift:
+// the top 64 significant bits, with any lower bit set folded into the last
+// (sticky) bit, convert to FP32 with one rounding; the scaling by a power of
+// two is exact.
+__device__ __forceinline__ float fx_value(unsigned long long lo, long long hi) {
+  const bool neg = hi < 0;
+  if (neg) {
+    lo = ~lo + 1;
+    hi = ~hi + (lo == 0 ? 1 : 0);
+  }
+  const unsigned long long uh = static_cast<unsigned long long>(hi);
+  float f;
+  if (uh == 0) {
+    f = ldexpf(static_cast<float>(lo), -kFxShift);
+  } else {
+    const int lz = __clzll(static_cast<long long>(uh));
+    const unsigned long long top = lz ? (uh << lz) | (lo >> (64 - lz)) : uh;
+    const unsigned long long rest = lz ? lo << lz : lo;
+    f = ldexpf(static_cast<float>(top | (rest != 0 ? 1ull : 0ull)), 64 - lz - kFxShift);
+  }
+  return neg ? -f : f;
+}
+
+// Pass 1: persistent workgroups take windows in turn.  A window's
+// contributions and segment table go to LDS; each thread takes kBPer
+// consecutive edges of the window (one 16-byte load of their source
+// offsets), finds the first one's segment by binary search and walks on.
+__global__ __launch_bounds__(kBPropThreads) void k_bprop(BpArgs b, const float* __restrict__ cin, int64_t n_gather,
+                                                         const PrCtrl* ctrl) {
+  __shared__ float win[kBWin];
+  __shared__ int32_t se[kBSegMax + 1];
+  __shared__ int32_t sd[kBSegMax];
+  if (ctrl->pad) return;
+  for (int32_t w = blockIdx.x; w < b.nwin; w += gridDim.x) {
+    const int64_t base = int64_t(w) * kBWin;
+    const int cnt = static_cast<int>(min<int64_t>(kBWin, n_gather - base));
+    const int64_t s0 = b.wseg[w];
+    const int ns = static_cast<int>(b.wseg[w + 1] - s0);
+    const int len = b.wlen[w];
+    const int64_t e0 = b.wstart[w];
+    __syncthreads();  // the previous window's readers are done
+    if (cnt == kBWin) {
+      const float4* s4 = reinterpret_cast<const float4*>(cin + base);
+      float4* d4 = reinterpret_cast<float4*>(win);
+      constexpr int kPer = kBWin / 4 / kBPropThreads;
+      float4 v[kPer];
+#pragma unroll
+      for (int i = 0; i < kPer; i++) v[i] = s4[threadIdx.x + i * kBPropThreads];
+#pragma unroll
+      for (int i = 0; i < kPer; i++) d4[threadIdx.x + i * kBPropThreads] = v[i];
+    } else {
+      for (int j = threadIdx.x; j < cnt; j += kBPropThreads) win[j] = cin[base + j];
+    }
+    for (int k = threadIdx.x; k < ns; k += kBPropThreads) {
+      se[k] = b.seg_e[s0 + k];
+      sd[k] = b.seg_d[s0 + k];
+    }
+    if (threadIdx.x == 0) se[ns] = len;
+    __syncthreads();
+    for (int i0 = threadIdx.x * kBPer; i0 < len; i0 += kBPropThreads * kBPer) {
+      const uint4 q = *reinterpret_cast<const uint4*>(b.src + e0 + i0);
+      const uint32_t sw[4] = {q.x, q.y, q.z, q.w};
+      int lo = 0, hi = ns;  // the last segment with se <= i0
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (se[mid] <= i0) lo = mid; else hi = mid;
+      }
+      int next = se[lo + 1];
+      int64_t dst = int64_t(sd[lo]) + i0;
+#pragma unroll
+      for (int k = 0; k < kBPer; k++) {
+        const int i = i0 + k;
+        if (i < len) {
+          if (i == next) {  // segments are non-empty: one step
+            lo++;
+            next = se[lo + 1];
+            dst = int64_t(sd[lo]) + i;
+          }
+          b.val[dst] = win[(sw[k >> 1] >> (16 * (k & 1))) & 0xFFFFu];
+          dst++;
+        }
+      }
+    }
+  }
+}
+
+// Pass 2: persistent workgroups take units in turn.  A bucket of several
+// rows accumulates in LDS (128-bit integer per row: the low word by a
+// 64-bit atomic add, its carry and the high word by a second); a one-row
+// unit in registers, reduced over the workgroup.  Rows finish here, except
+// the parts of a one-row bucket beyond kBEdges edges, whose sums go to
+// partial slots for k_bsplit.
+constexpr int kBSumUnroll = 4;
+__global__ __launch_bounds__(kBSumThreads) void k_bsum(PrArgs a, BpArgs b, float* __restrict__ cout,
+                                                       float* __restrict__ score) {
+  __shared__ unsigned long long alo[kBRows];
+  __shared__ unsigned long long ahi[kBRows];
+  __shared__ unsigned long long rlo[kBSumThreads / 64];
+  __shared__ long long rhi[kBSumThreads / 64];
+  __shared__ double red[kBSumThreads / 64];
+  if (a.ctrl->pad) return;
+  double err = 0;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid >> 6;
+  for (int64_t u = blockIdx.x; u < b.n_units; u += gridDim.x) {
+    const BlockUnit U = b.units[u];
+    if (U.nrows == 1) {
+      unsigned long long lo = 0;
+      long long hi = 0;
+      for (int64_t e = U.e0 + tid; e < U.e1; e += int64_t(kBSumThreads) * kBSumUnroll) {
+        float v[kBSumUnroll];
+#pragma unroll
+        for (int j = 0; j < kBSumUnroll; j++) v[j] = e + j * kBSumThreads < U.e1 ? b.val[e + j * kBSumThreads] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < kBSumUnroll; j++) {
+          unsigned long long xl;
+          long long xh;
+          fx_of(v[j], xl, xh);
+          fx_add(lo, hi, xl, xh);
+        }
+      }
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long ol = __shfl_xor(lo, d);
+        const long long oh = __shfl_xor(hi, d);
+        fx_add(lo, hi, ol, oh);
+      }
+      if (lane == 0) {
+        rlo[wave] = lo;
+        rhi[wave] = hi;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        unsigned long long tl = 0;
+        long long th = 0;
+        for (int w = 0; w < kBSumThreads / 64; w++) fx_add(tl, th, rlo[w], rhi[w]);
+        if (U.slot >= 0) {
+          b.part[2 * int64_t(U.slot)] = static_cast<long long>(tl);
+          b.part[2 * int64_t(U.slot) + 1] = th;
+        } else {
+          err += finish_f32(a, U.row0, fx_value(tl, th), score, cout);
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    for (int r = tid; r < U.nrows; r += kBSumThreads) {
+      alo[r] = 0;
+      ahi[r] = 0;
+    }
+    __syncthreads();
+    for (int64_t e = U.e0 + tid; e < U.e1; e += int64_t(kBSumThreads) * kBSumUnroll) {
+      float v[kBSumUnroll];
+      int r[kBSumUnroll];
+#pragma unroll
+      for (int j = 0; j < kBSumUnroll; j++) {
+        const int64_t k = e + j * kBSumThreads;
+        v[j] = k < U.e1 ? b.val[k] : 0.0f;
+        r[j] = k < U.e1 ? b.brow[k] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < kBSumUnroll; j++) {
+        unsigned long long xl;
+        long long xh;
+        fx_of(v[j], xl, xh);
+        if (xl | static_cast<unsigned long long>(xh)) {
+          const unsigned long long old = atomicAdd(&alo[r[j]], xl);
+          const long long c = xh + (old + xl < old ? 1 : 0);
+          if (c) atomicAdd(&ahi[r[j]], static_cast<unsigned long long>(c));
+        }
+      }
+    }
+    __syncthreads();
+    for (int r = tid; r < U.nrows; r += kBSumThreads)
+      err += finish_f32(a, U.row0 + r, fx_value(alo[r], static_cast<long long>(ahi[r])), score, cout);
+    __syncthreads();
+  }
+  const double t = block_err_sum<kBSumThreads>(err, red);
+  if (threadIdx.x == 0) a.err_part[blockIdx.x] = t;
+}
+
+// The one-row buckets in parts: each row's partial sums (exact) in slot
+// order, then the row finishes.
+__global__ __launch_bounds__(kThreads) void k_bsplit(PrArgs a, BpArgs b, float* __restrict__ cout,
+                                                     float* __restrict__ score) {
+  __shared__ double red[kThreads / 64];
+  if (a.ctrl->pad) return;
+  double err = 0;
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < b.n_splits) {
+    const BlockSplit sp = b.splits[i];
+    unsigned long long lo = 0;
+    long long hi = 0;
+    for (int j = 0; j < sp.count; j++)
+      fx_add(lo, hi, static_cast<unsigned long long>(b.part[2 * int64_t(sp.first + j)]),
+             b.part[2 * int64_t(sp.first + j) + 1]);
+    err = finish_f32(a, sp.row, fx_value(lo, hi), score, cout);
+  }
+  const double t = block_err_sum(err, red);
+  if (threadIdx.x == 0) a.err_part[a.main_blocks + blockIdx.x] = t;
+}
+
 // ---- iteration bookkeeping --------------------------------------------------
 
 // Sums the error partials in a fixed order; records the error and the stop
@@ -847,6 +1111,31 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       double* cout = ctx->d_contrib64[(it + 1) & 1];
       hipLaunchKernelGGL(k_fused<double>, dim3(fb), dim3(kFusedThreads), 0, s, a, cin, cout, ctx->d_score64);
       hipLaunchKernelGGL(k_f64_super, dim3(sb), dim3(kThreads), 0, s, a, cout, ctx->d_score64);
+    } else if (ctx->bp_ready && gcontrib) {
+      const float* cin = static_cast<const float*>(gcontrib);
+      float* cout = contrib_out ? contrib_out : ctx->d_contrib[(it + 1) & 1];
+      BpArgs b;
+      b.src = static_cast<const uint16_t*>(ctx->d_bp_src);
+      b.wstart = static_cast<const int64_t*>(ctx->d_bp_wstart);
+      b.wlen = static_cast<const int32_t*>(ctx->d_bp_wlen);
+      b.wseg = static_cast<const int64_t*>(ctx->d_bp_wseg);
+      b.seg_e = static_cast<const int32_t*>(ctx->d_bp_seg_e);
+      b.seg_d = static_cast<const int32_t*>(ctx->d_bp_seg_d);
+      b.brow = static_cast<const uint16_t*>(ctx->d_bp_brow);
+      b.units = static_cast<const BlockUnit*>(ctx->d_bp_units);
+      b.splits = static_cast<const BlockSplit*>(ctx->d_bp_splits);
+      b.val = static_cast<float*>(ctx->d_bp_val);
+      b.part = static_cast<long long*>(ctx->d_bp_part);
+      b.nwin = static_cast<int32_t>(ctx->n_bp_windows);
+      b.n_units = ctx->n_bp_units;
+      b.n_splits = ctx->n_bp_splits;
+      hipLaunchKernelGGL(k_bprop, dim3(std::max(1, ctx->cu_count)), dim3(kBPropThreads), 0, s, b, cin, a.n_gather,
+                         a.ctrl);
+      a.main_blocks = pr_bsum_blocks(ctx);
+      a.super_blocks = pr_bsplit_blocks(ctx);
+      hipLaunchKernelGGL(k_bsum, dim3(a.main_blocks), dim3(kBSumThreads), 0, s, a, b, cout, score_f);
+      if (a.super_blocks > 0)
+        hipLaunchKernelGGL(k_bsplit, dim3(a.super_blocks), dim3(kThreads), 0, s, a, b, cout, score_f);
     } else if (ctx->tiles_ready && !gcontrib) {
       const float* cin = ctx->d_contrib[it & 1];
       float* cout = ctx->d_contrib[(it + 1) & 1];
@@ -904,6 +1193,12 @@ int pr_fused_blocks(dmm_ctx* ctx) {
 }
 
 int pr_row_blocks(dmm_ctx* ctx) { return kRowGroupsPerCU * std::max(1, ctx->cu_count); }
+
+int pr_bsum_blocks(dmm_ctx* ctx) {
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(std::max(1, ctx->cu_count), ctx->n_bp_units)));
+}
+
+int pr_bsplit_blocks(dmm_ctx* ctx) { return static_cast<int>(ceil_div(ctx->n_bp_splits, kThreads)); }
 
 int pr_super_blocks(dmm_ctx* ctx) {
   return static_cast<int>(std::max<int64_t>(1, ceil_div(ctx->n_super, kThreads)));
@@ -1093,7 +1388,8 @@ int pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_gcontrib, float* d_s
   if (!base) return set_err(ctx, DMM_ERR_STATE, "the context has no scratch address");
   PrArgs a = make_args(ctx);
   const int parts = mode == DMM_PR_FP32_SERIAL ? a.light_blocks + a.heavy_blocks + a.super_blocks
-                                               : pr_fused_blocks(ctx) + a.super_blocks;
+                    : ctx->bp_ready ? pr_bsum_blocks(ctx) + pr_bsplit_blocks(ctx)
+                                    : pr_fused_blocks(ctx) + a.super_blocks;
   if (parts > kScratchErrParts)
     return set_err(ctx, DMM_ERR_CAPACITY, "%d error partials exceed the %d the scratch array holds", parts,
                    kScratchErrParts);

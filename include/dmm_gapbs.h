@@ -421,7 +421,12 @@ int dmm_pr_pull(dmm_ctx* ctx, int32_t max_iters, double epsilon,
  * global contribution vector (N floats, host) -- the in-HBM replacement of
  * the per-edge shmem_getmem of src/pr.cc:58.  scores_inout holds the block's
  * scores (vend - vbegin floats) and is updated in place; *err receives the
- * block's L1 change.  mode: DMM_PR_FP32_SERIAL or DMM_PR_FP32_KAHAN. */
+ * block's L1 change.  mode: DMM_PR_FP32_SERIAL or DMM_PR_FP32_KAHAN.  On a
+ * PE block (vend - vbegin < N) the first DMM_PR_FP32_KAHAN call builds the
+ * block's propagation layout (host work, kept until the next upload); its
+ * row sums are exact (contributions as integer multiples of 2^-80) and
+ * rounded once to FP32.  A block beyond that layout's limits keeps the
+ * untiled compensated kernels. */
 int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
                            float* scores_inout, int32_t mode, double* err,
                            double* ms);
@@ -436,7 +441,10 @@ int dmm_pr_block_iteration(dmm_ctx* ctx, const float* global_contrib,
  * life of the process; two calls that overlap on one GPU share it and race.
  * Returns DMM_ERR_CAPACITY for a block whose error partials or split-row
  * segments do not fit that array.  mode: DMM_PR_FP32_SERIAL or
- * DMM_PR_FP32_KAHAN; the upload must use DMM_LAYOUT_NATURAL. */
+ * DMM_PR_FP32_KAHAN; the upload must use DMM_LAYOUT_NATURAL.  It runs the
+ * kernels dmm_pr_block_iteration runs: in DMM_PR_FP32_KAHAN on a PE block,
+ * the propagation layout once a dmm_pr_block_iteration call has built it
+ * (this entry point builds nothing), the untiled kernels before. */
 int dmm_pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_global_contrib,
                                    float* d_scores, float* d_contrib_out, int32_t mode,
                                    double* d_err);
@@ -453,6 +461,21 @@ int dmm_pr_block_iteration_noalloc(dmm_ctx* ctx, const float* d_global_contrib,
 int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const int64_t* in_offsets,
                        const int32_t* in_neighs, int64_t window, const float* contrib,
                        float* incoming, int64_t* stats);
+
+/* Host check of the propagation layout that DMM_PR_FP32_KAHAN uses on a PE
+ * block (no GPU involved; for tests).  Builds the layout of the block's
+ * in-CSR (n rows, sources in [0, n_gather)), checks every index the kernels
+ * use against its buffer (each edge written once, each row finished once,
+ * each unit holding exactly its rows' in-edges), and replays one
+ * iteration's per-row incoming sums (float[n]) from the contributions
+ * (float[n_gather]) with the kernels' arithmetic: the exact sum of the
+ * contributions (each taken as an integer multiple of 2^-80, exact from
+ * 2^-57 up), rounded once to FP32.  stats[6]: windows, row buckets, units,
+ * rows summed in parts, non-empty segments, partial slots.
+ * No reference counterpart: it checks this engine's own layout. */
+int dmm_pr_blocks_check(int64_t n, int64_t n_gather, const int64_t* in_offsets,
+                        const int32_t* in_neighs, const float* contrib, float* incoming,
+                        int64_t* stats);
 
 /* ======================================================================== */
 /* Autotuner                                                                 */

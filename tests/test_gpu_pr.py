@@ -119,6 +119,43 @@ def test_pr_pe_block_iteration(ctx, dmm, orc, graph):
     assert err == pytest.approx(float(np.abs(s2[b:e].astype(np.float64) - s1[b:e]).sum()), rel=1e-9)
 
 
+def test_pr_pe_block_kahan_exact_sums(ctx, dmm, orc, graph):
+    """PE block (np = 8, pe = 3) in the compensated mode: the propagation
+    layout sums every row exactly and rounds once, so the new scores equal
+    base + damp * (FP32 rounding of the exact in-neighbour sum), the row sums
+    dmm_pr_blocks_check replays on the host (tests/test_pr_blocks.py checks
+    those against Python-integer sums)."""
+    import ctypes as C
+    import dist_mem_gapbs_amd._lib as L
+    hg = graph(14, 16, False)
+    g = orc.CSR.of(hg)
+    n = g.n
+    s1, _, _ = orc.pr_f32(g, 1, 0.0)
+    deg = np.diff(g.oo).astype(np.float32)
+    with np.errstate(divide="ignore"):
+        contrib = (s1 / deg).astype(np.float32)
+    p = dmm.partition(n, 8, 3)
+    b, e = p["start"], p["end"]
+    oo = np.ascontiguousarray(g.oo[b:e + 1] - g.oo[b])
+    on = np.ascontiguousarray(g.on[g.oo[b]:g.oo[e]])
+    inc = np.empty(e - b, np.float32)
+    st = np.zeros(6, np.int64)
+    assert L.load().dmm_pr_blocks_check(e - b, n, L.ptr(oo, C.c_int64), L.ptr(on, C.c_int32),
+                                        L.ptr(contrib, C.c_float), L.ptr(inc, C.c_float),
+                                        L.ptr(st, C.c_int64)) == 0
+    base = (np.float32(1.0) - np.float32(0.85)) / np.float32(n)
+    expect = (base + (np.float32(0.85) * inc).astype(np.float32)).astype(np.float32)
+    ctx.set_layout(L.DMM_LAYOUT_NATURAL)
+    try:
+        ctx.upload(num_nodes=n, out_offsets=oo, out_neighs=on, vbegin=b, vend=e, num_edges_directed=g.ned)
+        for _ in range(2):  # the first call builds the layout
+            got, err, _ = ctx.pr_block_iteration(contrib, s1[b:e], MODES["kahan"])
+            assert np.array_equal(got, expect)
+            assert err == pytest.approx(float(np.abs(expect.astype(np.float64) - s1[b:e]).sum()), rel=1e-9)
+    finally:
+        ctx.set_layout(L.DMM_LAYOUT_DEGREE)
+
+
 def test_pr_layouts_bit_identical(ctx, dmm, orc, graph):
     """The degree-ordered layout keeps each row's neighbour order, so the
     reference-order FP32 sums are unchanged."""

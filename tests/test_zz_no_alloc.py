@@ -5,7 +5,8 @@ library's per-GPU scratch array.
 Its results must equal, bit for bit, those of dmm_pr_block_iteration (same
 kernels, same summation order) on the same inputs: the block's new scores,
 its L1 change, and contributions that are the new scores over the
-out-degrees.  Two calls in a row, the second on a smaller block, show that
+out-degrees.  dmm_pr_block_iteration runs first: on a PE block in the
+compensated mode it builds the propagation layout both then use.  Two calls in a row, the second on a smaller block, show that
 what the first call leaves in the scratch array does not leak into the
 second.  The calls must not change the device memory the context holds."""
 import ctypes as C
@@ -43,8 +44,8 @@ def _noalloc(c, contrib, scores, mode):
 
 
 def _compare(c, contrib, scores, deg, mode):
-    got, cout, err = _noalloc(c, contrib, scores, mode)
     ref, ref_err, _ = c.pr_block_iteration(contrib, scores, mode)
+    got, cout, err = _noalloc(c, contrib, scores, mode)
     assert np.array_equal(got, ref)
     assert err == ref_err
     with np.errstate(divide="ignore", invalid="ignore"):
