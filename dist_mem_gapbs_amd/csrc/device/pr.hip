@@ -814,22 +814,30 @@ __device__ __forceinline__ float fx_value(unsigned long long lo, long long hi) {
 }
 
 // Pass 1: persistent workgroups take windows in turn.  A window's
-// contributions and segment table go to LDS; each thread takes kBPer
-// consecutive edges of the window (one 16-byte load of their source
-// offsets), finds the first one's segment by binary search and walks on.
+// contributions and segment table go to LDS.  A wavefront takes kBPer
+// groups of 64 consecutive edges at a time, lane l the l-th edge of each
+// group, so every store instruction writes 64 consecutive destinations
+// inside a segment (segments average tens of edges).  The group's first
+// segment is found once per wavefront by binary search; each lane steps
+// from it to its own edge's segment.  The next round's source offsets are
+// loaded before this round's stores.
 __global__ __launch_bounds__(kBPropThreads) void k_bprop(BpArgs b, const float* __restrict__ cin, int64_t n_gather,
                                                          const PrCtrl* ctrl) {
   __shared__ float win[kBWin];
   __shared__ int32_t se[kBSegMax + 1];
   __shared__ int32_t sd[kBSegMax];
   if (ctrl->pad) return;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  constexpr int kRound = 64 * kBPer;                         // edges per wavefront round
+  constexpr int kStride = (kBPropThreads / 64) * kRound;     // edges per workgroup round
   for (int32_t w = blockIdx.x; w < b.nwin; w += gridDim.x) {
     const int64_t base = int64_t(w) * kBWin;
     const int cnt = static_cast<int>(min<int64_t>(kBWin, n_gather - base));
     const int64_t s0 = b.wseg[w];
     const int ns = static_cast<int>(b.wseg[w + 1] - s0);
     const int len = b.wlen[w];
-    const int64_t e0 = b.wstart[w];
+    const uint16_t* src = b.src + b.wstart[w];
     __syncthreads();  // the previous window's readers are done
     if (cnt == kBWin) {
       const float4* s4 = reinterpret_cast<const float4*>(cin + base);
@@ -849,29 +857,37 @@ __global__ __launch_bounds__(kBPropThreads) void k_bprop(BpArgs b, const float* 
     }
     if (threadIdx.x == 0) se[ns] = len;
     __syncthreads();
-    for (int i0 = threadIdx.x * kBPer; i0 < len; i0 += kBPropThreads * kBPer) {
-      const uint4 q = *reinterpret_cast<const uint4*>(b.src + e0 + i0);
-      const uint32_t sw[4] = {q.x, q.y, q.z, q.w};
-      int lo = 0, hi = ns;  // the last segment with se <= i0
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (se[mid] <= i0) lo = mid; else hi = mid;
-      }
-      int next = se[lo + 1];
-      int64_t dst = int64_t(sd[lo]) + i0;
+    int c0 = wv * kRound;
+    uint32_t q[kBPer];
+#pragma unroll
+    for (int k = 0; k < kBPer; k++) {
+      const int i = c0 + lane + 64 * k;
+      q[k] = i < len ? src[i] : 0u;
+    }
+    for (; c0 < len; c0 += kStride) {
+      uint32_t qn[kBPer];
 #pragma unroll
       for (int k = 0; k < kBPer; k++) {
-        const int i = i0 + k;
-        if (i < len) {
-          if (i == next) {  // segments are non-empty: one step
-            lo++;
-            next = se[lo + 1];
-            dst = int64_t(sd[lo]) + i;
-          }
-          b.val[dst] = win[(sw[k >> 1] >> (16 * (k & 1))) & 0xFFFFu];
-          dst++;
-        }
+        const int i = c0 + kStride + lane + 64 * k;
+        qn[k] = i < len ? src[i] : 0u;
       }
+      int lo = 0, hi = ns;  // the last segment with se <= c0 (the same for every lane)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (se[mid] <= c0) lo = mid; else hi = mid;
+      }
+#pragma unroll
+      for (int k = 0; k < kBPer; k++) {
+        const int i = c0 + lane + 64 * k;
+        int sg = lo;
+        if (i < len) {
+          while (se[sg + 1] <= i) sg++;
+          b.val[int64_t(sd[sg]) + i] = win[q[k]];
+        }
+        lo = __shfl(sg, 63);  // lane 63 holds the group's last edge: the next group starts in its segment or later
+      }
+#pragma unroll
+      for (int k = 0; k < kBPer; k++) q[k] = qn[k];
     }
   }
 }
@@ -882,7 +898,7 @@ __global__ __launch_bounds__(kBPropThreads) void k_bprop(BpArgs b, const float* 
 // unit in registers, reduced over the workgroup.  Rows finish here, except
 // the parts of a one-row bucket beyond kBEdges edges, whose sums go to
 // partial slots for k_bsplit.
-constexpr int kBSumUnroll = 4;
+constexpr int kBSumUnroll = 8;
 __global__ __launch_bounds__(kBSumThreads) void k_bsum(PrArgs a, BpArgs b, float* __restrict__ cout,
                                                        float* __restrict__ score) {
   __shared__ unsigned long long alo[kBRows];
@@ -895,12 +911,13 @@ __global__ __launch_bounds__(kBSumThreads) void k_bsum(PrArgs a, BpArgs b, float
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid >> 6;
+  constexpr int64_t kStep = int64_t(kBSumThreads) * kBSumUnroll;
   for (int64_t u = blockIdx.x; u < b.n_units; u += gridDim.x) {
     const BlockUnit U = b.units[u];
     if (U.nrows == 1) {
       unsigned long long lo = 0;
       long long hi = 0;
-      for (int64_t e = U.e0 + tid; e < U.e1; e += int64_t(kBSumThreads) * kBSumUnroll) {
+      for (int64_t e = U.e0 + tid; e < U.e1; e += kStep) {
         float v[kBSumUnroll];
 #pragma unroll
         for (int j = 0; j < kBSumUnroll; j++) v[j] = e + j * kBSumThreads < U.e1 ? b.val[e + j * kBSumThreads] : 0.0f;
@@ -942,14 +959,24 @@ __global__ __launch_bounds__(kBSumThreads) void k_bsum(PrArgs a, BpArgs b, float
       ahi[r] = 0;
     }
     __syncthreads();
-    for (int64_t e = U.e0 + tid; e < U.e1; e += int64_t(kBSumThreads) * kBSumUnroll) {
-      float v[kBSumUnroll];
-      int r[kBSumUnroll];
+    // Each round's loads are issued before the previous round's atomics.
+    float v[kBSumUnroll];
+    int r[kBSumUnroll];
+    int64_t e = U.e0 + tid;
+#pragma unroll
+    for (int j = 0; j < kBSumUnroll; j++) {
+      const int64_t k = e + j * kBSumThreads;
+      v[j] = k < U.e1 ? b.val[k] : 0.0f;
+      r[j] = k < U.e1 ? b.brow[k] : 0;
+    }
+    for (; e < U.e1; e += kStep) {
+      float vn[kBSumUnroll];
+      int rn[kBSumUnroll];
 #pragma unroll
       for (int j = 0; j < kBSumUnroll; j++) {
-        const int64_t k = e + j * kBSumThreads;
-        v[j] = k < U.e1 ? b.val[k] : 0.0f;
-        r[j] = k < U.e1 ? b.brow[k] : 0;
+        const int64_t k = e + kStep + j * kBSumThreads;
+        vn[j] = k < U.e1 ? b.val[k] : 0.0f;
+        rn[j] = k < U.e1 ? b.brow[k] : 0;
       }
 #pragma unroll
       for (int j = 0; j < kBSumUnroll; j++) {
@@ -962,10 +989,15 @@ __global__ __launch_bounds__(kBSumThreads) void k_bsum(PrArgs a, BpArgs b, float
           if (c) atomicAdd(&ahi[r[j]], static_cast<unsigned long long>(c));
         }
       }
+#pragma unroll
+      for (int j = 0; j < kBSumUnroll; j++) {
+        v[j] = vn[j];
+        r[j] = rn[j];
+      }
     }
     __syncthreads();
-    for (int r = tid; r < U.nrows; r += kBSumThreads)
-      err += finish_f32(a, U.row0 + r, fx_value(alo[r], static_cast<long long>(ahi[r])), score, cout);
+    for (int rr = tid; rr < U.nrows; rr += kBSumThreads)
+      err += finish_f32(a, U.row0 + rr, fx_value(alo[rr], static_cast<long long>(ahi[rr])), score, cout);
     __syncthreads();
   }
   const double t = block_err_sum<kBSumThreads>(err, red);
