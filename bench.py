@@ -421,6 +421,9 @@ def pe_share_leg(dmm, args):
         out["pr_serial_block_equals_whole_graph"] = bool(np.array_equal(got, s2[b:e]))
         ms = []
         sc = s1[b:e].copy()
+        t0 = time.time()
+        sc, err, kms = blk.pr_block_iteration(contrib, sc, L.DMM_PR_FP32_KAHAN)  # builds the block's layout
+        out["pr_kahan_block_first_call_s"] = time.time() - t0
         for _ in range(20):
             sc, err, kms = blk.pr_block_iteration(contrib, sc, L.DMM_PR_FP32_KAHAN)
             ms.append(kms)
