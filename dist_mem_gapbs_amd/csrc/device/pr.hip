@@ -679,9 +679,16 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
       continue;
     }
     for (int32_t r = r0 + k; r < r1; r += kRowThreads) {
+      // The row's old score and out-degree load beside its list bounds,
+      // ahead of the merge.
       const int32_t i0 = t.istart[r], i1 = t.istart[r + 1];
+      const float old = score[r];
+      const int64_t deg = out_deg(a, r);
       const Comp acc = merge_list(reg, t.idx + i0, i1 - i0);
-      err += finish_f32(a, r, acc.value(), score, cout);
+      const float nw = a.prm->base_f + a.prm->damp_f * acc.value();
+      score[r] = nw;
+      cout[r] = nw / static_cast<float>(deg);
+      err += static_cast<double>(fabsf(nw - old));
     }
     __syncthreads();  // LDS readers done before the next block's gather
   }
