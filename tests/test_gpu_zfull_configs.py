@@ -198,6 +198,21 @@ def test_config5_kron27_pe3_of_8(dmm, orc):
             got, err, _ = blk.pr_block_iteration(contrib, s1[b:e], L.DMM_PR_FP32_SERIAL)
             assert np.array_equal(got, s2[b:e])
             assert err == pytest.approx(float(np.abs(s2[b:e].astype(np.float64) - s1[b:e]).sum()), rel=1e-9)
+            log("block iteration (reference order) equals the whole graph's")
+            # Compensated mode on the block: exact row sums, rounded once.  The
+            # FP64 row sums here differ from exact by about 1e-16 relative, so
+            # their FP32 rounding, and the score from it, may differ by one unit
+            # in the last place where the exact sum lies next to a tie.
+            got_k, _, _ = blk.pr_block_iteration(contrib, s1[b:e], L.DMM_PR_FP32_KAHAN)
+            rows = np.repeat(np.arange(e - b, dtype=np.int64), np.diff(np.asarray(oo[b:e + 1])))
+            w = contrib[np.asarray(g.out_neighs[oo[b]:oo[e]])].astype(np.float64)
+            inc = np.bincount(rows, weights=w, minlength=e - b).astype(np.float32)
+            del rows, w
+            base = (np.float32(1.0) - np.float32(0.85)) / np.float32(n)
+            expect = (base + (np.float32(0.85) * inc).astype(np.float32)).astype(np.float32)
+            diff = np.abs(got_k.astype(np.float64) - expect.astype(np.float64))
+            assert (diff <= np.spacing(np.abs(expect)).astype(np.float64)).all()
+            log(f"compensated block iteration: {int((diff > 0).sum())} rows one unit off the FP64-sum rounding")
         pool.shutdown()
     finally:
         g.free()
