@@ -87,8 +87,15 @@ struct BcCtl {
   int done;         // no level is left to run
   int scans;        // scans run: the write pass after a scan runs once,
   int writes;       // while writes (set by the next count pass) trails scans
-  int pad[2];
+  int marks;        // this pulled level sets its successor bits in the pull
+  int pad;
 };
+
+// A pulled level sets its successor bits in the pull (one atomicOr per
+// successor edge, at most R) rather than in the successor pass (one probe
+// per frontier out-edge, T) when T > kBcMarkRatio * R: a random atomicOr
+// costs about five such probes.
+constexpr long long kBcMarkRatio = 8;
 
 // The frontier's rank word for vertices [64 w, 64 w + 64): the bits of its
 // queued vertices and how many are queued before vertex 64 w.
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(kCThreads) void k_bc_init(int64_t n, int64_t words,
                                                        int32_t* __restrict__ queue, int64_t* __restrict__ qoff,
                                                        unsigned long long* __restrict__ pcq,
                                                        BcWord* __restrict__ rank, unsigned long long* __restrict__ bm,
-                                                       int64_t nw, int64_t m, BcCtl* __restrict__ ctl,
+                                                       int64_t nw, int64_t m, int marks_ok, BcCtl* __restrict__ ctl,
                                                        unsigned long long* __restrict__ host) {
   const int64_t gtid = int64_t(blockIdx.x) * kCThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kCThreads;
@@ -143,7 +150,8 @@ __global__ __launch_bounds__(kCThreads) void k_bc_init(int64_t n, int64_t words,
     c.pull = d * kBcPull > c.R ? 1 : 0;
     c.done = d == 0 ? 1 : 0;
     c.scans = c.writes = 0;
-    c.pad[0] = c.pad[1] = 0;
+    c.marks = marks_ok && c.pull && d > kBcMarkRatio * c.R ? 1 : 0;
+    c.pad = 0;
     *ctl = c;
     host[0] = static_cast<unsigned long long>(c.done);
     host[1] = 1;
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(kCThreads) void k_bc_expand(const int64_t* __restri
   __shared__ int64_t base[kBcChunk];   // out_off[u] - qoff[entry]: edge e sits at base + e
   __shared__ unsigned long long pcu[kTopDown ? kBcChunk : 1];
   __shared__ int64_t ie[2];
-  if (ctl->done || ctl->pull == (kTopDown ? 1 : 0)) return;
+  if (ctl->done || ctl->pull == (kTopDown ? 1 : 0) || (!kTopDown && ctl->marks)) return;
   const int64_t F = ctl->Fq, T = ctl->T;
   const int32_t level = ctl->level;
   const int tid = threadIdx.x;
@@ -283,11 +291,15 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int r) {
   return p;
 }
 
-// The path counts of the frontier in-neighbours among u[] (-1: none),
-// added to sum; any is set if one is on the frontier.
-__device__ __forceinline__ void pull_terms(const int32_t* u, const BcWord* __restrict__ rank,
-                                           const unsigned long long* __restrict__ pcq, unsigned long long& sum,
-                                           int& any) {
+// The path counts of the frontier in-neighbours among u[] (-1: none; in-edge
+// positions e + j * stride), added to sum; any is set if one is on the
+// frontier.  With a mirror (the out-CSR position of every in-edge), each
+// such in-edge's successor bit is set here: the in-edges of a vertex found
+// at depth L from depth L-1 are exactly the level's successor edges.
+__device__ __forceinline__ void pull_terms(const int32_t* u, int64_t e, int stride, const BcWord* __restrict__ rank,
+                                           const unsigned long long* __restrict__ pcq,
+                                           const int32_t* __restrict__ mirror, unsigned long long* __restrict__ succ,
+                                           unsigned long long& sum, int& any) {
   BcWord q[kBcPullUnroll];
 #pragma unroll
   for (int j = 0; j < kBcPullUnroll; j++) q[j] = u[j] >= 0 ? rank[u[j] >> 6] : BcWord{0ull, 0};
@@ -297,6 +309,10 @@ __device__ __forceinline__ void pull_terms(const int32_t* u, const BcWord* __res
     if ((q[j].bits >> b) & 1ull) {
       sum += pcq[q[j].pre + __popcll(q[j].bits & ((1ull << b) - 1))];
       any = 1;
+      if (mirror) {
+        const int32_t p = mirror[e + j * stride];
+        atomicOr(&succ[p >> 6], 1ull << (p & 63));
+      }
     }
   }
 }
@@ -309,9 +325,12 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
                                                        unsigned long long* __restrict__ pc,
                                                        const BcWord* __restrict__ rank,
                                                        const unsigned long long* __restrict__ pcq,
+                                                       const int32_t* __restrict__ mirror,
+                                                       unsigned long long* __restrict__ succ,
                                                        unsigned long long* __restrict__ fnew) {
   if (ctl->done || !ctl->pull) return;
   const int32_t level = ctl->level;
+  if (!ctl->marks) mirror = nullptr;  // the successor pass sets this level's bits
   const int lane = lane_id();
   const int sub = lane & (kBcGroup - 1);
   const int grp = lane / kBcGroup;
@@ -341,7 +360,7 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
         int32_t u[kBcPullUnroll];
 #pragma unroll
         for (int j = 0; j < kBcPullUnroll; j++) u[j] = e + j * kBcGroup < end ? in_nbr[e + j * kBcGroup] : -1;
-        pull_terms(u, rank, pcq, sum, any);
+        pull_terms(u, e, kBcGroup, rank, pcq, mirror, succ, sum, any);
       }
 #pragma unroll
       for (int dd = kBcGroup / 2; dd >= 1; dd >>= 1) {
@@ -365,7 +384,7 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
         int32_t u[kBcPullUnroll];
 #pragma unroll
         for (int j = 0; j < kBcPullUnroll; j++) u[j] = e + j * 64 < end ? in_nbr[e + j * 64] : -1;
-        pull_terms(u, rank, pcq, sum, any);
+        pull_terms(u, e, 64, rank, pcq, mirror, succ, sum, any);
       }
       sum = wave_sum(sum);
       if (__ballot(any)) {
@@ -379,6 +398,38 @@ __global__ __launch_bounds__(kCThreads) void k_bc_pull(const int64_t* __restrict
 #pragma unroll
     for (int dd = 32; dd >= 1; dd >>= 1) found |= __shfl_xor(found, dd);
     if (lane == 0) fnew[w] = found;
+  }
+}
+
+// The mirror: for in-edge position e of row v (in-neighbour u), the
+// position of the out-edge u -> v in u's out-row, by binary search.  Out-rows
+// that are not strictly increasing, or an in-edge without its out-edge,
+// clear *ok: the level loop then keeps the successor pass.
+__global__ __launch_bounds__(kCThreads) void k_bc_mirror(const int64_t* __restrict__ in_off,
+                                                         const int32_t* __restrict__ in_nbr,
+                                                         const int64_t* __restrict__ out_off,
+                                                         const int32_t* __restrict__ out_nbr, int64_t n,
+                                                         int32_t* __restrict__ mirror, int* __restrict__ ok) {
+  const int lane = lane_id();
+  const int64_t wave = (int64_t(blockIdx.x) * kCThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * kCThreads) >> 6;
+  for (int64_t v = wave; v < n; v += nwaves) {
+    for (int64_t e = in_off[v] + lane; e < in_off[v + 1]; e += 64) {
+      const int32_t u = in_nbr[e];
+      int64_t lo = out_off[u], hi = out_off[u + 1];
+      const int64_t end = hi;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (out_nbr[mid] < v) lo = mid + 1; else hi = mid;
+      }
+      if (lo >= end || out_nbr[lo] != v) {
+        *ok = 0;
+        lo = 0;
+      }
+      mirror[e] = static_cast<int32_t>(lo);
+    }
+    for (int64_t i = out_off[v] + lane; i + 1 < out_off[v + 1]; i += 64)
+      if (out_nbr[i] >= out_nbr[i + 1]) *ok = 0;
   }
 }
 
@@ -424,8 +475,8 @@ __global__ __launch_bounds__(kCThreads) void k_bc_count(const int64_t* __restric
 // degree sums (in place), qoff[F] = T, and the control block's next level
 // (mirrored to the host's flags).
 __global__ __launch_bounds__(kBcScanThreads) void k_bc_scan(int64_t* __restrict__ part, int64_t nparts,
-                                                            int64_t* __restrict__ qoff, BcCtl* __restrict__ ctl,
-                                                            unsigned long long* __restrict__ host) {
+                                                            int64_t* __restrict__ qoff, int marks_ok,
+                                                            BcCtl* __restrict__ ctl, unsigned long long* __restrict__ host) {
   __shared__ int64_t wt[3][kBcScanThreads / 64];
   if (ctl->done) return;
   const int lane = lane_id();
@@ -475,6 +526,7 @@ __global__ __launch_bounds__(kBcScanThreads) void k_bc_scan(int64_t* __restrict_
     c.levels += ta > 0 ? 1 : 0;
     c.pull = td * kBcPull > c.R ? 1 : 0;
     c.done = (ta == 0 || td == 0) ? 1 : 0;
+    c.marks = marks_ok && c.pull && td > kBcMarkRatio * c.R ? 1 : 0;
     c.scans += 1;
     *ctl = c;
     host[0] = static_cast<unsigned long long>(c.done);
@@ -567,6 +619,30 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
   unsigned long long* d_host = reinterpret_cast<unsigned long long*>(ctx->hd_bc_sizes);
   volatile uint64_t* h_host = ctx->h_bc_sizes;
   ctx->bc_valid = false;
+  // The mirror of the in-CSR, once per upload (graphs below 2^31 in-edges).
+  if (!ctx->bc_mirror_tried) {
+    ctx->bc_mirror_tried = true;
+    ctx->bc_mirror_ok = false;
+    if (ctx->m_in < (int64_t(1) << 31) && ctx->m_in > 0) {
+      if ((rc = dev_alloc(ctx, reinterpret_cast<void**>(&ctx->d_bc_mirror), sizeof(int32_t) * ctx->m_in,
+                          "BC in-edge mirror")))
+        return rc;
+      int* h_ok = reinterpret_cast<int*>(ctx->h_bc_sizes) + 8;  // mapped scratch beside the level flags
+      int* d_ok = reinterpret_cast<int*>(ctx->hd_bc_sizes) + 8;
+      *h_ok = 1;
+      hipLaunchKernelGGL(k_bc_mirror, dim3(grid_of(n * 64)), dim3(kCThreads), 0, s, ctx->d_in_off, ctx->d_in_nbr,
+                         ctx->d_out_off, ctx->d_out_nbr, n, ctx->d_bc_mirror, d_ok);
+      DMM_HIP_CHECK(ctx, hipGetLastError());
+      DMM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+      ctx->bc_mirror_ok = *static_cast<volatile int*>(h_ok) != 0;
+      if (!ctx->bc_mirror_ok) {
+        dev_free(ctx->d_bc_mirror);
+        ctx->dev_bytes -= static_cast<int64_t>(sizeof(int32_t) * ctx->m_in);
+        ctx->d_bc_mirror = nullptr;
+      }
+    }
+  }
+  const int32_t* mirror = ctx->bc_mirror_ok ? ctx->d_bc_mirror : nullptr;
   if (!ctx->bc_graph) {
     // kBcLevels levels, each the fixed launch sequence of the file comment.
     const int cap = std::max(1, ctx->cu_count) * 8;
@@ -577,7 +653,7 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
     DMM_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kBcLevels; k++) {
       hipLaunchKernelGGL(k_bc_pull, dim3(pgrid), dim3(kCThreads), 0, s, ctx->d_in_off, ctx->d_in_nbr, n, nw, d_ctl,
-                         d_depth, d_pc, d_rank, d_pcq, fnew);
+                         d_depth, d_pc, d_rank, d_pcq, mirror, d_succ, fnew);
       hipLaunchKernelGGL(k_bc_expand<false>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
                          ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_ctl, d_depth, d_pc, d_succ, fnew);
       hipLaunchKernelGGL(k_bc_expand<true>, dim3(egrid), dim3(kCThreads), 0, s, ctx->d_out_off, ctx->d_out_nbr,
@@ -585,7 +661,7 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
       hipLaunchKernelGGL(k_bc_count, dim3(cgrid), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, nw, d_ctl,
                          ctx->d_bc_part);
       hipLaunchKernelGGL(k_bc_scan, dim3(1), dim3(kBcScanThreads), 0, s, ctx->d_bc_part, nparts, ctx->d_bc_qoff,
-                         d_ctl, d_host);
+                         mirror ? 1 : 0, d_ctl, d_host);
       hipLaunchKernelGGL(k_bc_write, dim3(cgrid), dim3(kCThreads), 0, s, ctx->d_out_off, fnew, nw, ctx->d_bc_part,
                          d_pc, ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_rank, d_ctl);
     }
@@ -597,7 +673,7 @@ int bc_forward(dmm_ctx* ctx, int32_t source, int32_t* depth, uint64_t* path_coun
   }
   hipLaunchKernelGGL(k_bc_init, dim3(grid_of(std::max(n, words))), dim3(kCThreads), 0, s, n, words, source,
                      ctx->d_out_off, d_depth, d_pc, d_succ, ctx->d_bc_queue, ctx->d_bc_qoff, d_pcq, d_rank, fnew, nw,
-                     m, d_ctl, d_host);
+                     m, mirror ? 1 : 0, d_ctl, d_host);
   DMM_HIP_CHECK(ctx, hipGetLastError());
   // Every level adds at least one vertex to the visited set, so a search
   // ends within n levels; the bound only guards the host loop.

@@ -132,7 +132,7 @@ void free_graph_state(dmm_ctx* ctx) {
                   ctx->d_words_tmp, ctx->d_gfront, ctx->d_bseen, ctx->d_bnext, ctx->d_gnz,
                   ctx->d_bsegs, ctx->d_head, ctx->d_poff, ctx->d_ptgt, ctx->d_flist, ctx->d_blkctrl,
                   ctx->d_bc_depth, ctx->d_bc_pc, ctx->d_bc_succ, ctx->d_bc_queue, ctx->d_bc_part,
-                  ctx->d_bc_qoff, ctx->d_bc_pcq, ctx->d_bc_rank, ctx->d_bc_ctl, ctx->d_bc_bm};
+                  ctx->d_bc_qoff, ctx->d_bc_pcq, ctx->d_bc_rank, ctx->d_bc_ctl, ctx->d_bc_mirror, ctx->d_bc_bm};
   for (void* p : bufs) dmm::dev_free(p);
   if (ctx->directed) {
     dmm::dev_free(ctx->d_in_off);
@@ -149,6 +149,8 @@ void free_graph_state(dmm_ctx* ctx) {
   ctx->d_bc_part = ctx->d_bc_qoff = nullptr;
   ctx->d_bc_pcq = ctx->d_bc_bm = nullptr;
   ctx->d_bc_rank = ctx->d_bc_ctl = nullptr;
+  ctx->d_bc_mirror = nullptr;
+  ctx->bc_mirror_tried = ctx->bc_mirror_ok = false;
   ctx->bc_valid = false;
   ctx->d_score = ctx->d_contrib[0] = ctx->d_contrib[1] = ctx->d_gcontrib = nullptr;
   ctx->d_score64 = ctx->d_contrib64[0] = ctx->d_contrib64[1] = ctx->d_err_part = nullptr;

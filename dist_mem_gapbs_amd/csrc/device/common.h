@@ -169,6 +169,8 @@ struct dmm_ctx {
   uint64_t* d_bc_pcq = nullptr;   // path counts of d_bc_queue's vertices
   void* d_bc_rank = nullptr;      // per 64 vertices: queued bits, queued before
   void* d_bc_ctl = nullptr;       // the device-side level loop's state
+  int32_t* d_bc_mirror = nullptr; // out-CSR position of every in-edge (pulled levels' successor bits)
+  bool bc_mirror_tried = false, bc_mirror_ok = false;
   uint64_t* d_bc_bm = nullptr;    // the depth-L bitmap of the level being expanded
   uint64_t* h_bc_sizes = nullptr;   // pinned, mapped: the next frontier's sizes
   uint64_t* hd_bc_sizes = nullptr;  // device address of h_bc_sizes

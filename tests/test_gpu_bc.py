@@ -120,3 +120,20 @@ def test_bc_forward_sinks_and_isolated_source(natural, dmm, orc):
     natural.upload(g)
     for s in (0, 1, 5, n - 1):
         _check(natural, orc, og, s)
+
+
+def test_bc_forward_unsorted_rows(natural, dmm, orc):
+    """Rows whose neighbours are not in ascending order: the successor bits
+    still index the caller's edge positions (the level loop then finds
+    every successor edge from the frontier's side)."""
+    g = dmm.build_graph(12, 16, uniform=False)
+    og = orc.CSR.of(g)
+    rng = np.random.default_rng(4)
+    on = og.on.copy()
+    for r in range(og.n):
+        seg = on[og.oo[r]:og.oo[r + 1]]
+        rng.shuffle(seg)
+    shuffled = orc.CSR(og.n, og.oo, on)
+    natural.upload(num_nodes=og.n, out_offsets=og.oo, out_neighs=on)
+    for s in dmm.pick_sources(g, 4):
+        _check(natural, orc, shuffled, int(s))
