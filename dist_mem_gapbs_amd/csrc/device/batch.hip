@@ -634,6 +634,7 @@ struct BlockCtrl {
 constexpr int64_t kPushSplit = 4096;  // longer push lists are shared by the whole grid
 constexpr int kScanUnroll = 8;        // independent loads per lane in the frontier scan
 constexpr int kPushUnroll = 4;        // ... in the top-down expansion
+constexpr int kHeavyStage = 1024;     // heavy push lists staged in LDS per round
 
 // One bottom-up level over a PE block against the all-gathered frontier
 // words of every vertex (front, global IDs): the sweep of bu_phase with the
@@ -998,15 +999,57 @@ __global__ __launch_bounds__(kThreads) void k_block_push(const BlockCtrl* ctrl, 
       }
     }
   }
+  // Heavy lists, kHeavyStage at a time: every workgroup stages their
+  // starts, flattened prefix and frontier words in LDS, and the grid takes
+  // the flattened entries kPushUnroll per thread, each finding its list by
+  // binary search in LDS, so all lists proceed at once.
+  __shared__ int64_t hpre[kHeavyStage + 1];
+  __shared__ int64_t hbeg[kHeavyStage];
+  __shared__ uint64_t hfw[kHeavyStage];
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * kThreads;
-  for (int64_t h = 0; h < nh; h++) {
-    const int32_t u = list[n - 1 - h];
-    const uint64_t fw = front[u];
-    for (int64_t e = poff[u] + gtid; e < poff[u + 1]; e += gsize) {
-      const int32_t r = ptgt[e];
-      const uint64_t bits = fw & ~seen[r];
-      if (bits) atomicOr(reinterpret_cast<unsigned long long*>(&next[r]), static_cast<unsigned long long>(bits));
+  for (int64_t h0 = 0; h0 < nh; h0 += kHeavyStage) {
+    const int nb = static_cast<int>(min<int64_t>(kHeavyStage, nh - h0));
+    __syncthreads();  // the previous round's readers are done
+    for (int i = threadIdx.x; i < nb; i += kThreads) {
+      const int32_t u = list[n - 1 - (h0 + i)];
+      hbeg[i] = poff[u];
+      hpre[i + 1] = poff[u + 1] - poff[u];
+      hfw[i] = front[u];
+    }
+    if (threadIdx.x == 0) hpre[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < nb; i++) hpre[i + 1] += hpre[i];
+    __syncthreads();
+    const int64_t total = hpre[nb];
+    for (int64_t x0 = gtid; x0 < total; x0 += gsize * kPushUnroll) {
+      int32_t r[kPushUnroll];
+      uint64_t fw[kPushUnroll];
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) {
+        const int64_t x = x0 + k * gsize;
+        r[k] = -1;
+        fw[k] = 0;
+        if (x < total) {
+          int lo = 0, hi = nb;  // the last list with hpre <= x
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (hpre[mid] <= x) lo = mid; else hi = mid;
+          }
+          r[k] = ptgt[hbeg[lo] + (x - hpre[lo])];
+          fw[k] = hfw[lo];
+        }
+      }
+      uint64_t sw[kPushUnroll];
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) sw[k] = r[k] >= 0 ? seen[r[k]] : ~0ull;
+#pragma unroll
+      for (int k = 0; k < kPushUnroll; k++) {
+        const uint64_t bits = fw[k] & ~sw[k];
+        if (bits)
+          atomicOr(reinterpret_cast<unsigned long long*>(&next[r[k]]), static_cast<unsigned long long>(bits));
+      }
     }
   }
 }
