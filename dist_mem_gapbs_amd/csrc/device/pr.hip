@@ -419,8 +419,6 @@ struct TileArgs {
 };
 
 constexpr int kTileThreads = 1024;
-constexpr int kBatch = 2;  // chunks a wavefront loads together
-constexpr int kPair = 2;   // chunks a wavefront sums side by side
 
 // What one lane reads of a chunk from HBM: its end mask / slot count, its
 // kLaneEdges 16-bit source offsets and the chunk's partial range.
@@ -563,16 +561,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
       for (int j = threadIdx.x; j < cnt; j += kTileThreads) win[j] = cin[base + j];
     }
     __syncthreads();
-    // kBatch chunks per round: all their loads first, then each chunk in
-    // turn, so only the first waits for earlier stores.  Chunks past the
-    // range are the range's last chunk again (the same partials rewritten
-    // with the same values), keeping every round's memory operations fixed.
-    for (int64_t cb = c + wave; cb < wend; cb += kBatch * kWaves) {
-      ChunkLoad l[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) l[k] = load_chunk(t, min(cb + k * kWaves, wend - 1), lane);
-#pragma unroll
-      for (int k = 0; k < kBatch; k += kPair) tile_chunks<kPair>(t, win, l + k, lane, stage);
+    // Each wavefront's next chunk is loaded before the current one is summed
+    // and stored, so its wait does not include those stores (vector memory
+    // operations complete in issue order).  The load after the range's last
+    // chunk repeats that chunk and is not summed.
+    int64_t cb = c + wave;
+    ChunkLoad cur = load_chunk(t, min(cb, wend - 1), lane);
+    for (; cb < wend; cb += kWaves) {
+      ChunkLoad nxt = load_chunk(t, min(cb + kWaves, wend - 1), lane);
+      tile_chunks<1>(t, win, &cur, lane, stage);
+      cur = nxt;
     }
     c = wend;
   }
