@@ -452,78 +452,78 @@ __device__ __forceinline__ ChunkLoad load_chunk(const TileArgs& t, int64_t c, in
 // a unit in its last place).  Every term of a row's sum is non-negative, so
 // these roundings add at most one unit in the last place of the row's sum.
 //
+// Slots past a lane's filled count are not masked: every chunk closes its
+// last piece at its last filled slot, so those slots (which read offset 0)
+// lie after the chunk's last piece end and reach no piece.
+//
 // The chunk's pieces (at most kChunkPieces), numbered in lane order, are
 // staged, rounded to FP32, in the wavefront's LDS area and written to the
 // chunk's contiguous partial range by kChunkPieces / 64
 // store instructions of consecutive lanes.  With that fixed count, a chunk whose loads were issued before
 // these stores is waited for without waiting for the stores (vector memory
 // operations complete in issue order).
-template <int N>
-__device__ __forceinline__ void tile_chunks(const TileArgs& t, const float* win, const ChunkLoad* l, int lane,
-                                            float* stage) {
-  // N chunks side by side: their gathers, first passes and scans are
-  // independent chains the scheduler interleaves.
-  float x[N][kLaneEdges];
-  uint32_t ends[N];
+__device__ __forceinline__ void tile_chunk(const TileArgs& t, const float* win, ChunkLoad l, int lane,
+                                           float* stage) {
+  // Every use of the chunk's loaded registers stays after the previous
+  // chunk's stores (the asm is ordered with memory operations), so the
+  // compiler cannot move one early into a wait that covers those stores.
+  __asm__ volatile(""
+                   : "+v"(l.mt), "+v"(l.q0.x), "+v"(l.q0.y), "+v"(l.q0.z), "+v"(l.q0.w), "+v"(l.q1.x), "+v"(l.q1.y),
+                     "+v"(l.q1.z), "+v"(l.q1.w), "+v"(l.c0), "+v"(l.c1)
+                   :
+                   : "memory");
+  const uint32_t sw[8] = {l.q0.x, l.q0.y, l.q0.z, l.q0.w, l.q1.x, l.q1.y, l.q1.z, l.q1.w};
+  const uint32_t ends = l.mt & 0xFFFFu;
+  double x[kLaneEdges];
 #pragma unroll
-  for (int n = 0; n < N; n++) {
-    const uint32_t sw[8] = {l[n].q0.x, l[n].q0.y, l[n].q0.z, l[n].q0.w, l[n].q1.x, l[n].q1.y, l[n].q1.z, l[n].q1.w};
-    ends[n] = l[n].mt & 0xFFFFu;
+  for (int j = 0; j < kLaneEdges; j++) x[j] = static_cast<double>(win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu]);
+  // The lane's last piece end (-1 if none): its tail is the sum after it.
+  const int last = ends ? 31 - __builtin_clz(ends) : -1;
+  double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < kLaneEdges; j++) x[n][j] = win[(sw[j >> 1] >> (16 * (j & 1))) & 0xFFFFu];
-  }
-  double carry[N];
-  uint32_t q[N];
+  for (int j = 0; j < kLaneEdges; j++) acc += j > last ? x[j] : 0.0;
+  const unsigned long long has_end = __ballot(ends != 0u);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const unsigned long long h = has_end & upto;
+  const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
+  const double carry_out = seg_scan(acc, lane, seg);
+  double a = __shfl_up(carry_out, 1, 64);
+  if (lane == 0) a = 0.0;
+  const uint32_t k_i = __builtin_popcount(ends);
+  // Second pass: the lanes where a piece ends at slot j write it to its
+  // staging entry and restart.
+  float* qq = stage + (wave_scan_u32(k_i) - k_i);  // the lane's first piece
 #pragma unroll
-  for (int n = 0; n < N; n++) {
-    const int cnt = static_cast<int>(l[n].mt >> 16);
-#pragma unroll
-    for (int j = 0; j < kLaneEdges; j++)
-      if (j >= cnt) x[n][j] = 0.0f;
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < kLaneEdges; j++) {
-      acc += static_cast<double>(x[n][j]);
-      if ((ends[n] >> j) & 1u) acc = 0.0;
+  for (int j = 0; j < kLaneEdges; j++) {
+    a += x[j];
+    if ((ends >> j) & 1u) {
+      *qq++ = static_cast<float>(a);
+      a = 0.0;
     }
-    const unsigned long long has_end = __ballot(ends[n] != 0u);
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    const unsigned long long h = has_end & upto;
-    const int seg = h ? 63 - __clzll(static_cast<long long>(h)) : 0;
-    const double carry_out = seg_scan(acc, lane, seg);
-    carry[n] = __shfl_up(carry_out, 1, 64);
-    if (lane == 0) carry[n] = 0.0;
-    const uint32_t k_i = __builtin_popcount(ends[n]);
-    q[n] = wave_scan_u32(k_i) - k_i;  // the lane's first piece number
   }
+  // The staged pieces are read back by other lanes of this wavefront; LDS
+  // operations of one wavefront complete in issue order, so only the
+  // compiler's order needs holding here.
+  __asm__ volatile("" ::: "memory");
+  // Always kChunkPieces / 64 stores, with no branch around them: lanes past
+  // the chunk's last piece write that piece again.  The chunk's first
+  // partial and piece count are wave-uniform: the stores take a scalar base.
+  const int32_t c0 = __builtin_amdgcn_readfirstlane(l.c0);
+  const int np = __builtin_amdgcn_readfirstlane(l.c1) - c0;
+  float* const dst = t.part + c0;
+  float v[kChunkPieces / 64];
 #pragma unroll
-  for (int n = 0; n < N; n++) {
-    // Second pass, branch-free: every slot writes its running sum, to the
-    // piece's staging entry where a piece ends and to the lane's own spare
-    // entry elsewhere.
-    double a = carry[n];
-    uint32_t qq = q[n];
+  for (int r = 0; r < kChunkPieces / 64; r++) v[r] = stage[min(lane + 64 * r, np - 1)];
 #pragma unroll
-    for (int j = 0; j < kLaneEdges; j++) {
-      a += static_cast<double>(x[n][j]);
-      const uint32_t e = (ends[n] >> j) & 1u;
-      stage[e ? qq : kChunkPieces + lane] = static_cast<float>(a);
-      qq += e;
-      a = e ? 0.0 : a;
-    }
-    // The staged pieces are read back by other lanes of this wavefront (LDS
-    // operations of one wavefront complete in order).
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    // Always kChunkPieces / 64 stores, with no branch around them: lanes
-    // past the chunk's last piece write that piece again.
-    const int32_t c0 = l[n].c0, np = l[n].c1 - l[n].c0;
-#pragma unroll
-    for (int r = 0; r < kChunkPieces / 64; r++) {
-      const int e = min(lane + 64 * r, np - 1);
-      t.part[c0 + e] = stage[e];
-    }
-    __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
-  }
+  for (int r = 0; r < kChunkPieces / 64; r++) dst[min(lane + 64 * r, np - 1)] = v[r];
+  __asm__ volatile("" ::: "memory");  // the next chunk's staging follows these reads
+}
+
+// Waits for a chunk's loads (used where no vector memory operation follows
+// them, so that the wait there does not cover later stores).
+__device__ __forceinline__ void chunk_ready(const ChunkLoad& l) {
+  __asm__ volatile("" ::"v"(l.mt), "v"(l.q0.x), "v"(l.q0.y), "v"(l.q0.z), "v"(l.q0.w), "v"(l.q1.x), "v"(l.q1.y),
+                   "v"(l.q1.z), "v"(l.q1.w), "v"(l.c0), "v"(l.c1));
 }
 
 // Tile pass: persistent workgroups (one per CU), each owning a contiguous
@@ -534,7 +534,7 @@ template <int kWin>
 __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* __restrict__ cin, int64_t n_gather,
                                                       const PrCtrl* ctrl) {
   __shared__ float win[kWin];
-  __shared__ float stage_all[kTileThreads / 64][kChunkPieces + 64];
+  __shared__ float stage_all[kTileThreads / 64][kChunkPieces];
   if (ctrl->pad) return;  // converged
   const int64_t per = (t.n_chunks + gridDim.x - 1) / gridDim.x;
   const int64_t c0 = min(t.n_chunks, int64_t(blockIdx.x) * per), c1 = min(t.n_chunks, c0 + per);
@@ -563,14 +563,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs t, const float* 
     __syncthreads();
     // Each wavefront's next chunk is loaded before the current one is summed
     // and stored, so its wait does not include those stores (vector memory
-    // operations complete in issue order).  The load after the range's last
-    // chunk repeats that chunk and is not summed.
+    // operations complete in issue order).  Two load buffers alternate, so no
+    // copy between them waits early.  The load after the range's last chunk
+    // repeats that chunk and is not summed.
     int64_t cb = c + wave;
-    ChunkLoad cur = load_chunk(t, min(cb, wend - 1), lane);
-    for (; cb < wend; cb += kWaves) {
-      ChunkLoad nxt = load_chunk(t, min(cb + kWaves, wend - 1), lane);
-      tile_chunks<1>(t, win, &cur, lane, stage);
-      cur = nxt;
+    if (cb < wend) {
+      ChunkLoad ca = load_chunk(t, cb, lane);
+      chunk_ready(ca);
+      for (;;) {
+        const ChunkLoad cn = load_chunk(t, min(cb + kWaves, wend - 1), lane);
+        tile_chunk(t, win, ca, lane, stage);
+        cb += kWaves;
+        if (cb >= wend) break;
+        ca = load_chunk(t, min(cb + kWaves, wend - 1), lane);
+        tile_chunk(t, win, cn, lane, stage);
+        cb += kWaves;
+        if (cb >= wend) break;
+      }
     }
     c = wend;
   }

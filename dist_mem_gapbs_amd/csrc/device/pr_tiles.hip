@@ -459,7 +459,8 @@ extern "C" int dmm_pr_tiles_check(int64_t n, int64_t zb, int64_t n_gather, const
         double acc = 0.0;
         for (int j = 0; j < kLaneEdges; j++) {
           const uint16_t s = th.src[c * kChunkEdges + l * kLaneEdges + j];
-          if (s >= cnt_w && j < cnt) return bad("source offset");
+          // Unfilled slots are summed unmasked by the tile kernel: offset 0.
+          if (j < cnt ? s >= cnt_w : s != 0) return bad("source offset");
           x[l][j] = j < cnt ? contrib[base + s] : 0.0f;
           acc += static_cast<double>(x[l][j]);
           if ((ends >> j) & 1u) acc = 0.0;
