@@ -606,12 +606,16 @@ __device__ __forceinline__ Comp merge_list(const float* reg, const uint16_t* __r
 constexpr int kSegBatch = kRowThreads;  // segments staged per round
 constexpr int kXcds = 8;                 // MI355X: 8 XCDs, each with its own L2
 constexpr int kRegionPer = kRowSlots / kRowThreads;  // region entries per thread
+constexpr int kGroupShift = 5;           // region entries per search group: 32
+constexpr int kGroups = kRowSlots >> kGroupShift;
+static_assert(kSegBatch <= 32767, "segment numbers are stored as int16");
 
 // Row pass of the tiled mode: persistent workgroups take row blocks in
 // turn.  A block's region (its partials, window order) is gathered into LDS
 // from its segments: every thread loads its region entries (all loads in
 // flight before the LDS stores), finding each entry's segment by binary
-// search over the segments' region offsets.  A one-row block merges its
+// search over the segments' region offsets, between the segments that hold
+// the first entries of its 32-entry group and of the next group.  A one-row block merges its
 // region with its threads striding it in order, then a fixed tree; in a
 // multi-row block every thread merges one row's partials in window order.
 // Rows without in-edges (the tail) only finish.
@@ -620,6 +624,7 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
   __shared__ float reg[kRowSlots];
   __shared__ int32_t soff[kSegBatch + 1];
   __shared__ int32_t sstart[kSegBatch];
+  __shared__ int16_t gseg[kGroups + 1];  // segment of each group's first entry
   __shared__ float2 wred[kRowThreads / 64];
   __shared__ double red[kRowThreads / 64];
   if (a.ctrl->pad) return;
@@ -651,12 +656,24 @@ __global__ __launch_bounds__(kRowThreads, kRowGroupsPerCU * kRowThreads / 256) v
       if (k == 0) soff[ns] = sb + ns < s1 ? t.seg_off[sb + ns] : size;
       __syncthreads();
       const int lo_e = soff[0], hi_e = soff[ns];
+      const int ngroups = (hi_e - lo_e + (1 << kGroupShift) - 1) >> kGroupShift;
+      if (k < ns) {
+        // Segments are not empty; group g's first entry lo_e + 32 g lies in
+        // exactly one of them.
+        const int g0 = (soff[k] - lo_e + (1 << kGroupShift) - 1) >> kGroupShift;
+        const int g1 = (soff[k + 1] - lo_e + (1 << kGroupShift) - 1) >> kGroupShift;
+        for (int g = g0; g < g1; g++) gseg[g] = static_cast<int16_t>(k);
+      }
+      __syncthreads();
       float v[kRegionPer];
 #pragma unroll
       for (int i = 0; i < kRegionPer; i++) {
         const int e = lo_e + k + i * kRowThreads;
         if (e < hi_e) {
-          int lo = 0, hi = ns;  // the last segment with soff <= e
+          // The last segment with soff <= e, between the segments of this
+          // group's first entry and of the next group's.
+          const int g = (e - lo_e) >> kGroupShift;
+          int lo = gseg[g], hi = g + 1 < ngroups ? gseg[g + 1] + 1 : ns;
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (soff[mid] <= e) lo = mid; else hi = mid;
