@@ -295,11 +295,26 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
   const int32_t* __restrict__ in_nbr = a.in_nbr;
   int32_t* __restrict__ parent = a.parent;
   int64_t awake = 0, edges = 0, scanned = 0;
-  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+  // The parent entries of a wave's next chunk load at the start of the
+  // current one, ahead of its head-row load, so the wait for the head row
+  // covers them, and are waited for before the current chunk's stores: the
+  // next chunk then starts without a wait that would also cover those
+  // stores (vector memory operations complete in issue order).  Two entry
+  // registers alternate between the chunks, so no copy between them waits.
+  // Only this wave writes these entries during the phase.
+  auto load_parents = [&](int64_t ch) -> int32_t {
     const int64_t lu = ch * 64 + lane;
+    return ch < nchunks && lu < a.nloc ? parent[lu] : 0;
+  };
+  // One chunk: pcur holds its parent entries; pnext (loaded here) the next
+  // chunk's.
+  auto chunk = [&](int64_t ch, int32_t pcur, int32_t& pnext) {
+    const int64_t lu = ch * 64 + lane;
+    pnext = load_parents(ch + nwaves);
     bool found = false;
-    if (lu < a.nloc) {
-      if (parent[lu] < 0) {
+    int32_t pv = 0;
+    {
+      if (lu < a.nloc && pcur < 0) {
         scanned++;
         // kBuProbe in-neighbours per round: their IDs, then their frontier
         // bits; the first hit in row order is the parent (the sequential
@@ -322,7 +337,6 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
           for (int j = 1; j < kBuProbe; j++) hh[j] = h0[j] >= 0 && ((front[h0[j] >> 6] >> (h0[j] & 63)) & 1ull);
         }
         int hj = kBuProbe;
-        int32_t pv = 0;
 #pragma unroll
         for (int j = kBuProbe - 1; j >= 0; j--)
           if (hh[j]) {
@@ -357,12 +371,16 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
           found = hit < e;
           ex = found ? hit - s + 1 : e - s;
         }
-        if (found) {
-          parent[lu] = pv;
-          if (a.depth) a.depth[lu] = level + 1;
-        }
         edges += ex;
       }
+    }
+    // Every load of this chunk is complete here on the scanning path (the
+    // head-row wait covered the next chunk's entries); on the other path
+    // this is where they are waited for.  The stores follow.
+    __asm__ volatile("" ::"v"(pnext) : "memory");
+    if (found) {
+      parent[lu] = pv;
+      if (a.depth) a.depth[lu] = level + 1;
     }
     const unsigned long long m = __ballot(found);
     if (a.bu_aligned) {
@@ -372,6 +390,16 @@ __device__ void bu_phase(const StepArgs& a, const StepIo& io, SmallLds& L) {
       atomicOr(reinterpret_cast<unsigned long long*>(&next[g >> 6]), 1ull << (g & 63));
     }
     if (lane == 0) awake += __popcll(m);
+  };
+  int64_t ch = wave;
+  int32_t pa = load_parents(ch), pb = 0;
+  __asm__ volatile("" ::"v"(pa));
+  while (ch < nchunks) {
+    chunk(ch, pa, pb);
+    ch += nwaves;
+    if (ch >= nchunks) break;
+    chunk(ch, pb, pa);
+    ch += nwaves;
   }
   edges = wave_sum(edges);
   scanned = wave_sum(scanned);
