@@ -271,26 +271,57 @@ __device__ void bu_phase(const BArgs& a, BCtrl* c, SmallLds& L, int level) {
   const int64_t ngrp = (int64_t(gridDim.x) * kThreads) / kGroup;
   int64_t scanned = 0, cnt = 0, degs = 0;
   // Rows from bend on have no in-edges; their F[L + 1] words stay zero.
-  for (int64_t u = grp; u < a.bend; u += ngrp) {
-    const uint64_t s = a.seen[u] | Fl[u];
-    const uint64_t need = a.lanes & ~s;
-    const int64_t b = in_off[u], e = in_off[u + 1];
-    uint64_t nxt = 0;
-    if (need && e > b) {
-      const int64_t stop = sweep_row<kSweepUnroll>(in_nbr, b, e, sub, need, Fl, nullptr, nullptr, &nxt);
-      scanned += stop - b;
+  // A group loads its next row's words and bounds at the start of the
+  // current row and waits for them before the current row's stores: on
+  // gfx950 vector loads and stores complete in issue order, so a row's
+  // first loads issued after the previous row's stores would wait for them.
+  // Two head buffers alternate between rows (no copy, which would wait).
+  // Only this group writes its rows' seen words during the phase.
+  struct RowHead {
+    uint64_t seen, fl;
+    int64_t b, e;
+  };
+  auto load_head = [&](int64_t u) -> RowHead {
+    RowHead h{0, 0, 0, 0};
+    if (u < a.bend) {
+      h.seen = a.seen[u];
+      h.fl = Fl[u];
+      h.b = in_off[u];
+      h.e = in_off[u + 1];
     }
+    return h;
+  };
+  auto row = [&](int64_t u, const RowHead& h, RowHead& hn) {
+    hn = load_head(u + ngrp);
+    const uint64_t s = h.seen | h.fl;
+    const uint64_t need = a.lanes & ~s;
+    uint64_t nxt = 0;
+    if (need && h.e > h.b) {
+      const int64_t stop = sweep_row<kSweepUnroll>(in_nbr, h.b, h.e, sub, need, Fl, nullptr, nullptr, &nxt);
+      scanned += stop - h.b;
+    }
+    // Out-degree of a found row: its in-row length when in-CSR == out-CSR.
+    int64_t d = 0;
+    if (sub == 0 && nxt) d = a.undirected ? h.e - h.b : a.out_off[u + 1] - a.out_off[u];
+    __asm__ volatile("" ::"v"(hn.seen), "v"(hn.fl), "v"(hn.b), "v"(hn.e) : "memory");
     if (sub == 0) {
       Fn[u] = nxt;
-      if (Fl[u] | nxt) a.seen[u] = s | nxt;
-      if (nxt) {
-        const int64_t d = a.out_off[u + 1] - a.out_off[u];
-        if (d > 0) {
-          cnt++;
-          degs += d;
-        }
+      if (h.fl | nxt) a.seen[u] = s | nxt;
+      if (d > 0) {
+        cnt++;
+        degs += d;
       }
     }
+  };
+  int64_t u = grp;
+  RowHead ha = load_head(u), hb{0, 0, 0, 0};
+  __asm__ volatile("" ::"v"(ha.seen), "v"(ha.fl), "v"(ha.b), "v"(ha.e));
+  while (u < a.bend) {
+    row(u, ha, hb);
+    u += ngrp;
+    if (u >= a.bend) break;
+    row(u, hb, ha);
+    u += ngrp;
   }
   // Each group counted its scan in every lane; keep one lane's count.
   if (sub != 0) scanned = 0;
@@ -546,6 +577,28 @@ __global__ void k_binit2(BArgs a, const int32_t* srcs, int nsrc) {
 // eight bit-slice masks (bit j of the depth), then lane s's byte is written
 // to depth[s * nloc + u]; consecutive threads write consecutive bytes.
 // perm (may be NULL) maps the caller's vertex ID v to the internal row.
+// Depth bytes of sources 8g .. 8g + 7 from the bit planes m[0..8) (plane j
+// holds bit j of every source's depth): byte k of the result is bit 8g + k
+// of planes 0..7.  The planes' bytes g form an 8 x 8 bit matrix (row j =
+// plane j), transposed by three delta swaps (Hacker's Delight, 7-3).
+__device__ __forceinline__ uint64_t depth_bytes(const uint64_t* m, int g) {
+  uint64_t w = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) w |= ((m[j] >> (8 * g)) & 0xFFull) << (8 * j);
+  uint64_t t;
+  t = (w ^ (w >> 7)) & 0x00AA00AA00AA00AAull;
+  w ^= t ^ (t << 7);
+  t = (w ^ (w >> 14)) & 0x0000CCCC0000CCCCull;
+  w ^= t ^ (t << 14);
+  t = (w ^ (w >> 28)) & 0x00000000F0F0F0F0ull;
+  w ^= t ^ (t << 28);
+  return w;
+}
+
+// depth[s][v] for the caller's vertex v: the level whose frontier word holds
+// lane s, 255 if none.  Levels are bit-sliced into eight planes (level L
+// sets lane s in plane j when bit j of L is set); an unreached lane gets
+// every plane bit, so its byte comes out as 255.
 __global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int nsrc, const int32_t* perm) {
   const int nlev = a.ctrl->nlevels;
   const int64_t gtid = int64_t(blockIdx.x) * kThreads + threadIdx.x;
@@ -562,13 +615,17 @@ __global__ __launch_bounds__(kThreads) void k_bdepth(BArgs a, uint8_t* out, int 
         if ((L >> j) & 1) m[j] |= f;
     }
 #pragma unroll
-    for (int s = 0; s < 64; s++) {
-      if (s < nsrc) {
-        uint32_t d = 0;
+    for (int j = 0; j < 8; j++) m[j] |= ~reached;
 #pragma unroll
-        for (int j = 0; j < 8; j++) d |= static_cast<uint32_t>((m[j] >> s) & 1ull) << j;
-        out[int64_t(s) * a.nloc + v] = ((reached >> s) & 1ull) ? static_cast<uint8_t>(d) : 255;
-      }
+    for (int g = 0; g < 8; g++) {
+      const uint64_t w = depth_bytes(m, g);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (8 * g + k < nsrc) {
+          // Wave-uniform row base, 32-bit vertex offset (nloc < 2^31).
+          uint8_t* row = out + int64_t(8 * g + k) * a.nloc;
+          row[static_cast<uint32_t>(v)] = static_cast<uint8_t>(w >> (8 * k));
+        }
     }
   }
 }
