@@ -786,6 +786,7 @@ struct BpArgs {
   const BlockUnit* units;
   const BlockSplit* splits;
   float* val;
+  float* spare;     // val[m]: the stores of lanes past a window's end
   long long* part;  // (lo, hi) per partial slot
   int32_t nwin;
   int64_t n_units, n_splits;
@@ -888,37 +889,51 @@ __global__ __launch_bounds__(kBPropThreads) void k_bprop(BpArgs b, const float* 
     }
     if (threadIdx.x == 0) se[ns] = len;
     __syncthreads();
-    int c0 = wv * kRound;
-    uint32_t q[kBPer];
+    // A wavefront's next round of source offsets loads at the start of the
+    // current round, into one of two buffers that alternate (no copy, which
+    // would wait).  Vector loads and stores complete in issue order; with
+    // unconditional loads (index clamped into the window) and stores, every
+    // path issues the same count, so the next round waits for its offsets
+    // with a count that leaves this round's stores in flight.
+    auto load_round = [&](int c, uint32_t (&qr)[kBPer]) {
 #pragma unroll
-    for (int k = 0; k < kBPer; k++) {
-      const int i = c0 + lane + 64 * k;
-      q[k] = i < len ? src[i] : 0u;
-    }
-    for (; c0 < len; c0 += kStride) {
-      uint32_t qn[kBPer];
-#pragma unroll
-      for (int k = 0; k < kBPer; k++) {
-        const int i = c0 + kStride + lane + 64 * k;
-        qn[k] = i < len ? src[i] : 0u;
-      }
+      for (int k = 0; k < kBPer; k++) qr[k] = src[min(c + lane + 64 * k, len - 1)];
+    };
+    auto round = [&](int c0, const uint32_t (&q)[kBPer], uint32_t (&qn)[kBPer]) {
+      load_round(c0 + kStride, qn);
       int lo = 0, hi = ns;  // the last segment with se <= c0 (the same for every lane)
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (se[mid] <= c0) lo = mid; else hi = mid;
       }
+      // Every lane stores kBPer times (lanes past the window's end into the
+      // spare slot), so every path issues the same number of stores.
 #pragma unroll
       for (int k = 0; k < kBPer; k++) {
         const int i = c0 + lane + 64 * k;
         int sg = lo;
+        float* dst = b.spare;
         if (i < len) {
           while (se[sg + 1] <= i) sg++;
-          b.val[int64_t(sd[sg]) + i] = win[q[k]];
+          dst = b.val + int64_t(sd[sg]) + i;
         }
+        *dst = win[q[k]];
         lo = __shfl(sg, 63);  // lane 63 holds the group's last edge: the next group starts in its segment or later
       }
+    };
+    if (len > 0) {
+      int c0 = wv * kRound;
+      uint32_t qa[kBPer], qb[kBPer];
+      load_round(c0, qa);
 #pragma unroll
-      for (int k = 0; k < kBPer; k++) q[k] = qn[k];
+      for (int k = 0; k < kBPer; k++) __asm__ volatile("" ::"v"(qa[k]));
+      while (c0 < len) {
+        round(c0, qa, qb);
+        c0 += kStride;
+        if (c0 >= len) break;
+        round(c0, qb, qa);
+        c0 += kStride;
+      }
     }
   }
 }
@@ -1188,6 +1203,7 @@ void launch_iteration(dmm_ctx* ctx, PrArgs a, int mode, int it, const void* gcon
       b.units = static_cast<const BlockUnit*>(ctx->d_bp_units);
       b.splits = static_cast<const BlockSplit*>(ctx->d_bp_splits);
       b.val = static_cast<float*>(ctx->d_bp_val);
+      b.spare = b.val + ctx->m_in;
       b.part = static_cast<long long*>(ctx->d_bp_part);
       b.nwin = static_cast<int32_t>(ctx->n_bp_windows);
       b.n_units = ctx->n_bp_units;

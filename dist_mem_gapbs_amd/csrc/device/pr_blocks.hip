@@ -187,7 +187,9 @@ int pr_blocks_build(dmm_ctx* ctx) {
       (rc = upload_vec(ctx, &ctx->d_bp_brow, bh.brow_off, "PR block row offsets")) ||
       (rc = upload_vec(ctx, &ctx->d_bp_units, bh.units, "PR block units")) ||
       (rc = upload_vec(ctx, &ctx->d_bp_splits, bh.splits, "PR block split rows")) ||
-      (rc = dev_alloc(ctx, &ctx->d_bp_val, sizeof(float) * std::max<int64_t>(1, m), "PR block contributions")) ||
+      // One spare slot past the m contributions takes the propagation
+      // pass's stores from lanes past a window's end.
+      (rc = dev_alloc(ctx, &ctx->d_bp_val, sizeof(float) * (m + 1), "PR block contributions")) ||
       (rc = dev_alloc(ctx, &ctx->d_bp_part, 16 * std::max<int64_t>(1, bh.n_slots), "PR block split partials"))) {
     ctx->bp_bytes = ctx->dev_bytes - held;  // what the failed build holds, freed with the layout
     return rc;
