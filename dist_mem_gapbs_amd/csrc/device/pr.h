@@ -35,7 +35,7 @@ constexpr int64_t kPrSegment = 2048;  // edges per segment
 // the row kernel merges a row's partials in window order.
 constexpr int kLaneEdges = 16;
 constexpr int kChunkEdges = 64 * kLaneEdges;
-constexpr int kChunkPieces = 448;   // pieces one chunk may hold
+constexpr int kChunkPieces = 512;   // pieces one chunk may hold
 // Row blocks: consecutive rows holding at most kBlockRuns runs; rows with
 // kSoloRuns runs or kSoloEdges in-edges or more are blocks of their own.
 // A block's partials (its runs plus chunk cuts) must number at most
